@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""All-pairs Meta-Storms benchmark on one MI355X.
+
+Workload (one step = one pass of the hot path over the batch):
+  workload 1 of the benchmark plan: seeded Yule tree with 99,322 leaves
+  (branch lengths ~ Exp(0.02) clipped to [0, 0.99]), 10,000 samples with
+  200-2,000 nonzero OTUs (Zipf 1.1 popularity, NegBinomial(40, 0.4) counts,
+  copy-number correction), all 49,995,000 sample pairs.
+
+Timed step (inputs resident in HBM as CSR abundances): rebuild the kernel's
+device layout from the samples (walk layout, fixed-point embedding, block
+bitmaps) and compute every pair into a device buffer. The headline is the
+shared-minimum kernel (exact identity of the walk, fixed point, <= 1e-5 vs
+FP64); the bit-exact FP32 walk drop-in is timed beside it.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import importlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # FP32/INT32 VALU lane-ops per second, non-packed
+HBM_PEAK_GBS = 8000.0
+METRIC = "sample-pairs/sec & pair·node updates/sec on 1 MI355X at ≤1e-5 abs err vs FP64"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed_steps(ctx, out, steps, warmup, torch):
+    for _ in range(warmup):
+        ctx.prepare(force=True)
+        ctx.all_pairs_device(out.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.prepare(force=True)
+        ctx.all_pairs_device(out.data_ptr())
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, sample_n, pairs):
+    """Reference Calc_sim timed on the host cores over a bounded pair sample,
+    plus the accuracy spot-check of the GPU results on pairs of that sample."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    sub = counts.subset(range(sample_n))
+    res = {}
+    drv = orc.ref_binary("ref_driver_O3")
+    if drv:
+        with tempfile.TemporaryDirectory() as d:
+            cms.write_db(d, "bench", tree, cp)
+            table = os.path.join(d, "table.txt")
+            cms.write_table(table, sub, tree.leaf_n)
+            env = dict(os.environ, ParallelMETA=d)
+            r = subprocess.run([drv, "-D", "G", "-T", table, "--time", str(pairs), str(threads)], env=env,
+                               capture_output=True, text=True, timeout=600)
+            if r.returncode == 0:
+                j = json.loads(r.stdout.strip().splitlines()[-1])
+                res = {"value": j["pairs_per_sec"], "unit": "sample-pairs/s", "cores": threads, "kind": "reference",
+                       "sample": f"first {j['pairs']} pairs of the upper triangle of the first {sample_n} samples "
+                                 f"(99,322-leaf tree); reference _Comp_Tree::Calc_sim built -O3 -ffp-contract=off, "
+                                 f"OpenMP dynamic,1 on {threads} threads, wall clock {j['seconds']:.2f} s"}
+            else:
+                log(f"reference driver failed: {r.stderr[-500:]}")
+    if not res:
+        dense = sub.dense(tree.leaf_n)
+        t0 = time.perf_counter()
+        got = orc.all_pairs_f32(tree, dense, threads, pairs)
+        sec = time.perf_counter() - t0
+        res = {"value": len(got) / sec, "unit": "sample-pairs/s", "cores": threads, "kind": "port",
+               "sample": f"first {len(got)} pairs of {sample_n} samples, oracle FP32 walk, {threads} threads"}
+    # accuracy of the GPU outputs on 2,000 pairs of the sample vs the oracle
+    rng = np.random.default_rng(7)
+    ii = rng.integers(0, sample_n, 4000)
+    jj = rng.integers(0, sample_n, 4000)
+    keep = ii != jj
+    ii, jj = np.minimum(ii, jj)[keep][:2000].astype(np.int32), np.maximum(ii, jj)[keep][:2000].astype(np.int32)
+    dense = abd.subset(range(sample_n)).dense(tree.leaf_n)
+    f64 = orc.pairs_f64(tree, dense, ii, jj, threads=threads)
+    f32 = orc.pairs_f32(tree, dense, ii, jj, threads=threads)
+    p = cms.packed_index(ii, jj, n_all)
+    acc = {}
+    if gpu_sm is not None:
+        e = np.abs(gpu_sm[p].astype(np.float64) - f64)
+        acc["sharedmin_max_abs_err_vs_fp64"] = float(e.max())
+        acc["sharedmin_rms_err_vs_fp64"] = float(np.sqrt((e ** 2).mean()))
+    if gpu_walk is not None:
+        acc["walk_bitexact_vs_reference_fp32"] = bool(np.array_equal(gpu_walk[p].view(np.uint32),
+                                                                     f32.view(np.uint32)))
+        acc["walk_max_abs_err_vs_fp64"] = float(np.abs(gpu_walk[p].astype(np.float64) - f64).max())
+    acc["reference_fp32_max_abs_err_vs_fp64"] = float(np.abs(f32.astype(np.float64) - f64).max())
+    acc["pairs_checked"] = int(len(ii))
+    return res, acc
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--samples", type=int, default=None, help="override the workload's sample count")
+    ap.add_argument("--walk-steps", type=int, default=1, help="timed steps of the FP32 walk drop-in (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-samples", type=int, default=256)
+    ap.add_argument("--cpu-pairs", type=int, default=32640)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every GPU box here has one, and multi-GPU runs are not part of "
+              "this build (pairs shard without communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a HIP device")
+    cms = importlib.import_module("coordinated-meta-storms_amd")
+
+    t0 = time.perf_counter()
+    tree, cp, counts, abd = cms.workload(1, samples=a.samples)
+    n = abd.n
+    npairs = n * (n - 1) // 2
+    log(f"workload 1: {n} samples, {tree.leaf_n} leaves, nnz {abd.rowptr[-1]}, "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+    out = torch.empty(max(npairs, 1), dtype=torch.float32, device="cuda")
+
+    # headline: shared-minimum kernel
+    sm = cms.Context(0, kernel=cms.KERNEL_SHAREDMIN)
+    sm.set_tree(tree)
+    sm.set_samples_csr(abd)
+    sec = timed_steps(sm, out, a.steps, a.warmup, torch)
+    st = sm.stats()
+    gpu_sm = out[:npairs].cpu().numpy()
+    log(f"sharedmin: {sec * 1e3:.2f} ms/step, pair kernel {st.kernel_ms:.2f} ms, prep {st.prep_ms:.2f} ms, "
+        f"visited pair-entries {st.pair_node_updates:.3e}")
+    sm.close()
+
+    walk = None
+    gpu_walk = None
+    if a.walk_steps > 0:
+        wc = cms.Context(0, kernel=cms.KERNEL_WALK)
+        wc.set_tree(tree)
+        wc.set_samples_csr(abd)
+        wsec = timed_steps(wc, out, a.walk_steps, 0, torch)
+        wst = wc.stats()
+        gpu_walk = out[:npairs].cpu().numpy()
+        wc.close()
+        w_ops = wst.valu_ops / (wst.kernel_ms * 1e-3) / 1e12
+        walk = {"ms_per_step": wsec * 1e3, "pairs_per_s": npairs / wsec,
+                "pair_node_updates_per_s": npairs * tree.order_n / wsec, "kernel_ms": wst.kernel_ms,
+                "roofline": {"bound": "valu", "achieved": w_ops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                             "frac": w_ops / VALU_PEAK_TOPS, "traffic": None,
+                             "ops_per_pair_node": 14}}
+        log(f"walk: {wsec * 1e3:.1f} ms/step, kernel {wst.kernel_ms:.1f} ms, {w_ops:.2f} TOP/s")
+
+    valu_tops = st.valu_ops / (st.kernel_ms * 1e-3) / 1e12
+    res = {
+        "metric": METRIC,
+        "value": npairs / sec,
+        "unit": "sample-pairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 fixed-point (FP64 embedding)",
+        "data": "synthetic (seeded generator; tree 1001, samples 2001, copy numbers 3001)",
+        "config": {"workload": "workload 1: 10,000 samples x 99,322-leaf tree, all 49,995,000 pairs",
+                   "samples": n, "leaves": tree.leaf_n, "pairs": npairs, "kernel": "sharedmin"},
+        "pair_node_updates_per_s": npairs * tree.order_n / sec,
+        "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                     "frac": valu_tops / VALU_PEAK_TOPS, "traffic": None,
+                     "kernel": "sm_tile_kernel<0>", "kernel_ms": st.kernel_ms,
+                     "ops_per_pair_entry": 1.5, "pair_entries": st.pair_node_updates},
+        "prep_ms": st.prep_ms,
+        "plain_fp32_walk": walk,
+    }
+    if walk:
+        res["speedup_vs_plain_fp32_walk"] = walk["ms_per_step"] / (sec * 1e3)
+    if not a.no_cpu:
+        cb, acc = cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n, a.cpu_threads,
+                               min(a.cpu_samples, n), a.cpu_pairs)
+        res["cpu_baseline"] = cb
+        res["accuracy"] = acc
+        log(f"cpu baseline: {cb['value']:.1f} pairs/s ({cb['kind']}, {cb['cores']} threads); accuracy {acc}")
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,497 @@
+"""Python binding of libcms, the MI355X-native all-pairs Meta-Storms library.
+
+The package mirrors the reference's host interface for the `-T` path
+(qdu-bioinfo/Coordinated-Meta-Storms):
+
+* ``Database`` - `_PMDB` + `_Comp_Tree` tree loading (db.h:21-358,
+  comp.h:122-192): db.config, tree/id.txt, tree/order.txt, copy_number.txt.
+* ``Table`` - `_Table_Format::Load_Table` (table_format.h:110-165).
+* ``Table.abundance`` - `_Comp_Tree::Load_abd` (comp.h:243-279).
+* ``Context.all_pairs`` - the all-pairs loop of `Multi_GPU_split_MetaStorm`
+  (cms_cuda.cu:1470-2235) / `Multi_Comp_Table` (comp_sam.cpp:303-369).
+* ``Context.sim_pairs`` - the reference kernels' pair-list boundary
+  (`CalcSimOfDiagonalSquare`, cms_cuda.cu:52-54).
+* ``write_dist`` - `Output_Matrix` (cms_cuda.cu:1305-1342).
+
+Everything is a thin ctypes layer over the C ABI in include/cms.h; the
+distance computation runs only in the gfx950 kernels of libcms.so. If the
+library is missing the import raises (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcms.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+OK = 0
+E_INVAL, E_NOMEM, E_HIP, E_SCHEDULE, E_STATE, E_IO, E_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
+KERNEL_WALK, KERNEL_SHAREDMIN = 0, 1
+STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
+REG_SIZE = 150
+
+
+class CmsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libcms error {code}: {msg}")
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libcms.so not built at {LIB_PATH}; run __graft_entry__.build() "
+                      "(make -C coordinated-meta-storms_amd/csrc)")
+lib = C.CDLL(LIB_PATH)
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+class Opts(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("storage", C.c_int), ("tile_a", C.c_int), ("tile_b", C.c_int),
+                ("chunk", C.c_int), ("use_graph", C.c_int), ("reserved", C.c_int * 8)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("prep_ms", C.c_double), ("pair_node_updates", C.c_double),
+                ("dense_updates", C.c_double), ("valu_ops", C.c_double), ("bytes_moved", C.c_double),
+                ("tiles", C.c_int64), ("nslots", C.c_int), ("scale_bits", C.c_int), ("entries", C.c_int)]
+
+
+class SynthOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_samples", C.c_int), ("leaf_n", C.c_int), ("nnz_lo", C.c_int),
+                ("nnz_hi", C.c_int), ("nnz_log_uniform", C.c_int), ("zipf_s", C.c_double),
+                ("nb_mean", C.c_double), ("nb_dispersion", C.c_double)]
+
+
+def _sig(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("cms_opts_default", None, [C.POINTER(Opts)])
+_sig("cms_create", C.c_int, [C.c_int, C.POINTER(Opts), C.POINTER(_vp)])
+_sig("cms_destroy", None, [_vp])
+_sig("cms_last_error", C.c_char_p, [_vp])
+_sig("cms_set_opts", C.c_int, [_vp, C.POINTER(Opts)])
+_sig("cms_set_tree", C.c_int, [_vp, C.c_int, C.c_int, _i32p, _f32p, _i32p, _f32p, _i32p])
+_sig("cms_set_samples", C.c_int, [_vp, C.c_int, _f32p])
+_sig("cms_set_samples_csr", C.c_int, [_vp, C.c_int, _i64p, _i32p, _f32p])
+_sig("cms_all_pairs", C.c_int, [_vp, _f32p])
+_sig("cms_all_pairs_device", C.c_int, [_vp, _vp, _vp])
+_sig("cms_sim_pairs", C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f32p])
+_sig("cms_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f32p])
+_sig("cms_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
+_sig("cms_prepare", C.c_int, [_vp, C.c_int])
+_sig("cms_validate_tree", C.c_int, [C.c_int, C.c_int, _i32p, _f32p, _i32p, _f32p, _i32p, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)])
+_sig("cms_db_open", C.c_int, [C.c_char_p, C.c_char, C.POINTER(_vp)])
+_sig("cms_db_close", None, [_vp])
+_sig("cms_db_leaf_n", C.c_int, [_vp])
+_sig("cms_db_order_n", C.c_int, [_vp])
+_sig("cms_db_has_tree", C.c_int, [_vp])
+_sig("cms_db_has_cp", C.c_int, [_vp])
+_sig("cms_db_order", C.c_int, [_vp] + [C.POINTER(_vp)] * 5)
+_sig("cms_db_leaf_name", C.c_char_p, [_vp, C.c_int])
+_sig("cms_table_load", C.c_int, [C.c_char_p, C.POINTER(_vp)])
+_sig("cms_table_from_dense", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), _f32p,
+                                      C.POINTER(_vp)])
+_sig("cms_table_free", None, [_vp])
+_sig("cms_table_n", C.c_int, [_vp])
+_sig("cms_table_f", C.c_int, [_vp])
+_sig("cms_table_sample_name", C.c_char_p, [_vp, C.c_int])
+_sig("cms_table_feature_name", C.c_char_p, [_vp, C.c_int])
+_sig("cms_table_skipped", C.c_int, [_vp])
+_sig("cms_table_abundance", C.c_int, [_vp, _vp, C.c_int, _f32p, _vp])
+_sig("cms_write_dist", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), _f32p, C.c_int])
+_sig("cms_synth_tree", C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_double, _i32p, _f32p, _i32p, _f32p, _i32p,
+                                 C.POINTER(C.c_int)])
+_sig("cms_synth_samples", C.c_int, [C.POINTER(SynthOpts), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)])
+_sig("cms_synth_copy_numbers", C.c_int, [C.c_uint64, C.c_int, C.c_int, _f32p])
+_sig("cms_counts_to_abundance_csr", C.c_int, [C.c_int, _i64p, _i32p, _f32p, _vp, _f32p])
+_sig("cms_write_db", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, _i32p, _f32p, _i32p, _f32p, _i32p, _vp])
+_sig("cms_write_table", C.c_int, [C.c_char_p, C.c_int, C.c_int, _i64p, _i32p, _f32p])
+_sig("cms_free", None, [_vp])
+_sig("cms_version", C.c_char_p, [])
+
+#: every symbol declared in include/cms.h (checked by the CPU test suite)
+EXPORTED = [
+    "cms_opts_default", "cms_create", "cms_destroy", "cms_last_error", "cms_set_opts", "cms_set_tree",
+    "cms_set_samples", "cms_set_samples_csr", "cms_all_pairs", "cms_all_pairs_device", "cms_sim_pairs",
+    "cms_tile", "cms_get_stats", "cms_prepare", "cms_validate_tree", "cms_db_open", "cms_db_close", "cms_db_leaf_n", "cms_db_order_n",
+    "cms_db_has_tree", "cms_db_has_cp", "cms_db_order", "cms_db_leaf_name", "cms_table_load",
+    "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
+    "cms_table_feature_name", "cms_table_skipped", "cms_table_abundance", "cms_write_dist",
+    "cms_synth_tree", "cms_synth_samples", "cms_synth_copy_numbers", "cms_counts_to_abundance_csr",
+    "cms_write_db", "cms_write_table", "cms_free", "cms_version",
+]
+
+
+def last_error(ctx=None) -> str:
+    m = lib.cms_last_error(ctx)
+    return m.decode(errors="replace") if m else ""
+
+
+def _check(rc: int, ctx=None):
+    if rc != OK:
+        raise CmsError(rc, last_error(ctx))
+
+
+# ------------------------------------------------------------------ tree / db
+
+@dataclass
+class Tree:
+    """Schedule arrays exactly as in tree/order.txt (comp.h:161-192)."""
+    leaf_n: int
+    order_1: np.ndarray
+    dist_1: np.ndarray
+    order_2: np.ndarray
+    dist_2: np.ndarray
+    order_d: np.ndarray
+    leaf_names: list | None = None
+    max_slots: int = 0
+
+    @property
+    def order_n(self) -> int:
+        return int(self.order_1.shape[0])
+
+    def arrays(self):
+        return (self.order_1, self.dist_1, self.order_2, self.dist_2, self.order_d)
+
+
+def synth_tree(seed: int, leaf_n: int, bl_mean: float, bl_max: float = 0.99) -> Tree:
+    """Seeded Yule tree in Sethi-Ullman order.txt form (cms_synth_tree)."""
+    on = leaf_n - 1
+    o1 = np.zeros(on, np.int32)
+    o2 = np.zeros(on, np.int32)
+    od = np.zeros(on, np.int32)
+    d1 = np.zeros(on, np.float32)
+    d2 = np.zeros(on, np.float32)
+    slots = C.c_int(0)
+    _check(lib.cms_synth_tree(seed, leaf_n, bl_mean, bl_max, o1, d1, o2, d2, od, C.byref(slots)))
+    return Tree(leaf_n, o1, d1, o2, d2, od, [f"T{i}" for i in range(leaf_n)], slots.value)
+
+
+def validate_tree(tree: Tree):
+    """(nslots, is_tree, nonneg_beta) or CmsError(E_SCHEDULE) (cms_validate_tree)."""
+    ns, it, nb = C.c_int(0), C.c_int(0), C.c_int(0)
+    _check(lib.cms_validate_tree(tree.leaf_n, tree.order_n, *[np.ascontiguousarray(a) for a in tree.arrays()],
+                                 C.byref(ns), C.byref(it), C.byref(nb)))
+    return ns.value, bool(it.value), bool(nb.value)
+
+
+class Database:
+    """A ParallelMETA database directory (db.config + tree + copy numbers)."""
+
+    def __init__(self, root: str | None = None, letter: str = "G"):
+        h = _vp()
+        _check(lib.cms_db_open(root.encode() if root else None, letter.encode()[:1], C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.cms_db_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def leaf_n(self) -> int:
+        return lib.cms_db_leaf_n(self._h)
+
+    @property
+    def order_n(self) -> int:
+        return lib.cms_db_order_n(self._h)
+
+    @property
+    def has_tree(self) -> bool:
+        return bool(lib.cms_db_has_tree(self._h))
+
+    @property
+    def has_cp(self) -> bool:
+        return bool(lib.cms_db_has_cp(self._h))
+
+    def leaf_name(self, i: int) -> str:
+        return lib.cms_db_leaf_name(self._h, i).decode()
+
+    def tree(self) -> Tree:
+        ptrs = [_vp() for _ in range(5)]
+        _check(lib.cms_db_order(self._h, *[C.byref(p) for p in ptrs]))
+        n = self.order_n
+
+        def arr(p, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            ct = C.c_int32 if dt == np.int32 else C.c_float
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,)).astype(dt, copy=True)
+
+        return Tree(self.leaf_n, arr(ptrs[0], np.int32), arr(ptrs[1], np.float32), arr(ptrs[2], np.int32),
+                    arr(ptrs[3], np.float32), arr(ptrs[4], np.int32))
+
+
+class Table:
+    """OTU count table, rows = samples (table_format.h:110-165)."""
+
+    def __init__(self, path: str | None = None, *, counts=None, samples=None, features=None):
+        h = _vp()
+        if path is not None:
+            _check(lib.cms_table_load(path.encode(), C.byref(h)))
+        else:
+            counts = np.ascontiguousarray(counts, np.float32)
+            n, f = counts.shape
+            sn = (C.c_char_p * n)(*[s.encode() for s in samples])
+            fn = (C.c_char_p * f)(*[s.encode() for s in features])
+            _check(lib.cms_table_from_dense(n, f, sn, fn, counts, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.cms_table_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def n(self) -> int:
+        return lib.cms_table_n(self._h)
+
+    @property
+    def f(self) -> int:
+        return lib.cms_table_f(self._h)
+
+    @property
+    def skipped(self) -> int:
+        return lib.cms_table_skipped(self._h)
+
+    def sample_names(self) -> list:
+        return [lib.cms_table_sample_name(self._h, i).decode() for i in range(self.n)]
+
+    def feature_names(self) -> list:
+        return [lib.cms_table_feature_name(self._h, j).decode() for j in range(self.f)]
+
+    def abundance(self, db: Database, cp_correct: bool = True):
+        """Dense n x leaf_n percent-scale abundances and mapped counts (comp.h:243-279)."""
+        out = np.zeros((self.n, db.leaf_n), np.float32)
+        mapped = np.zeros(self.n, np.int32)
+        _check(lib.cms_table_abundance(db._h, self._h, int(cp_correct), out.reshape(-1),
+                                       mapped.ctypes.data_as(_vp)))
+        return out, mapped
+
+
+def write_dist(path: str, names, sim_packed: np.ndarray, is_sim: bool = False):
+    """Output_Matrix (cms_cuda.cu:1305-1342)."""
+    n = len(names)
+    arr = (C.c_char_p * max(n, 1))(*[s.encode() for s in names])
+    sim = np.ascontiguousarray(sim_packed, np.float32)
+    if sim.size == 0:
+        sim = np.zeros(1, np.float32)
+    _check(lib.cms_write_dist(path.encode(), n, arr, sim, int(is_sim)))
+
+
+# ------------------------------------------------------------ synthetic data
+
+@dataclass
+class Samples:
+    """CSR samples: row s has (leaf, value) for its nonzero leaves."""
+    rowptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.rowptr.shape[0] - 1)
+
+    def dense(self, leaf_n: int) -> np.ndarray:
+        out = np.zeros((self.n, leaf_n), np.float32)
+        rows = np.repeat(np.arange(self.n), np.diff(self.rowptr))
+        out[rows, self.col] = self.val
+        return out
+
+    def subset(self, idx) -> "Samples":
+        idx = np.asarray(idx)
+        lens = np.diff(self.rowptr)[idx]
+        rp = np.zeros(len(idx) + 1, np.int64)
+        rp[1:] = np.cumsum(lens)
+        col = np.concatenate([self.col[self.rowptr[i]:self.rowptr[i + 1]] for i in idx]) if len(idx) else \
+            np.zeros(0, np.int32)
+        val = np.concatenate([self.val[self.rowptr[i]:self.rowptr[i + 1]] for i in idx]) if len(idx) else \
+            np.zeros(0, np.float32)
+        return Samples(rp, col.astype(np.int32), val.astype(np.float32))
+
+
+def synth_counts(seed, n_samples, leaf_n, nnz_lo, nnz_hi, zipf_s, nb_mean, nb_dispersion,
+                 log_uniform=False) -> Samples:
+    o = SynthOpts(seed, n_samples, leaf_n, nnz_lo, nnz_hi, int(log_uniform), zipf_s, nb_mean, nb_dispersion)
+    rp, col, cnt = _vp(), _vp(), _vp()
+    _check(lib.cms_synth_samples(C.byref(o), C.byref(rp), C.byref(col), C.byref(cnt)))
+    try:
+        rowptr = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int64)), shape=(n_samples + 1,)).copy()
+        nnz = int(rowptr[-1])
+        if nnz:
+            c = np.ctypeslib.as_array(C.cast(col, C.POINTER(C.c_int32)), shape=(nnz,)).copy()
+            v = np.ctypeslib.as_array(C.cast(cnt, C.POINTER(C.c_float)), shape=(nnz,)).copy()
+        else:
+            c, v = np.zeros(0, np.int32), np.zeros(0, np.float32)
+    finally:
+        lib.cms_free(rp)
+        lib.cms_free(col)
+        lib.cms_free(cnt)
+    return Samples(rowptr, c, v)
+
+
+def synth_copy_numbers(seed: int, leaf_n: int, cp_max: int = 15) -> np.ndarray:
+    cp = np.zeros(leaf_n, np.float32)
+    _check(lib.cms_synth_copy_numbers(seed, leaf_n, cp_max, cp))
+    return cp
+
+
+def counts_to_abundance(s: Samples, cp: np.ndarray | None) -> Samples:
+    """Load_abd arithmetic (comp.h:252-275) on CSR counts whose columns are leaves."""
+    out = np.zeros_like(s.val)
+    cpp = None
+    if cp is not None:
+        cp = np.ascontiguousarray(cp, np.float32)
+        cpp = cp.ctypes.data_as(_vp)
+    _check(lib.cms_counts_to_abundance_csr(s.n, s.rowptr, s.col, s.val if s.val.size else np.zeros(1, np.float32),
+                                           cpp, out if out.size else np.zeros(1, np.float32)))
+    return Samples(s.rowptr, s.col, out)
+
+
+def write_db(root: str, name: str, tree: Tree, cp: np.ndarray | None):
+    cpp = None
+    if cp is not None:
+        cp = np.ascontiguousarray(cp, np.float32)
+        cpp = cp.ctypes.data_as(_vp)
+    _check(lib.cms_write_db(root.encode(), name.encode(), tree.leaf_n, tree.order_n, *tree.arrays(), cpp))
+
+
+def write_table(path: str, counts: Samples, leaf_n: int):
+    _check(lib.cms_write_table(path.encode(), counts.n, leaf_n, counts.rowptr, counts.col,
+                               counts.val if counts.val.size else np.zeros(1, np.float32)))
+
+
+#: The workloads of the benchmark plan (tree seed 1000+k, samples 2000+k,
+#: copy numbers 3000+k for configuration k).
+WORKLOADS = {
+    0: dict(leaves=5000, bl_mean=0.03, samples=64, nnz=(100, 800), log_uniform=False, zipf=1.1, nb=(50, 0.5)),
+    1: dict(leaves=99322, bl_mean=0.02, samples=10000, nnz=(200, 2000), log_uniform=False, zipf=1.1, nb=(40, 0.4)),
+    2: dict(leaves=99322, bl_mean=0.02, samples=50000, nnz=(200, 2000), log_uniform=False, zipf=1.1, nb=(40, 0.4)),
+    3: dict(leaves=99322, bl_mean=0.02, samples=100000, nnz=(50, 20000), log_uniform=True, zipf=1.0, nb=(30, 0.3)),
+    4: dict(leaves=99322, bl_mean=0.02, samples=200000, nnz=(50, 20000), log_uniform=True, zipf=1.0, nb=(30, 0.3)),
+}
+
+
+def workload(k: int, samples: int | None = None, cp_correct: bool = True):
+    """Tree, copy numbers, counts and abundances of workload k (optionally fewer samples)."""
+    w = WORKLOADS[k]
+    tree = synth_tree(1000 + k, w["leaves"], w["bl_mean"], 0.99)
+    cp = synth_copy_numbers(3000 + k, w["leaves"], 15)
+    n = w["samples"] if samples is None else samples
+    counts = synth_counts(2000 + k, n, w["leaves"], w["nnz"][0], w["nnz"][1], w["zipf"], w["nb"][0], w["nb"][1],
+                          w["log_uniform"])
+    abd = counts_to_abundance(counts, cp if cp_correct else None)
+    return tree, cp, counts, abd
+
+
+# ------------------------------------------------------------------ context
+
+class Context:
+    """One GPU context (cms_create). Not thread-safe."""
+
+    def __init__(self, device: int = 0, kernel: int = KERNEL_WALK, storage: int = STORE_F32):
+        o = Opts()
+        lib.cms_opts_default(C.byref(o))
+        o.kernel, o.storage = kernel, storage
+        h = _vp()
+        _check(lib.cms_create(device, C.byref(o), C.byref(h)))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.cms_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def error(self) -> str:
+        return last_error(self._h)
+
+    def set_opts(self, kernel: int = KERNEL_WALK, storage: int = STORE_F32):
+        o = Opts()
+        lib.cms_opts_default(C.byref(o))
+        o.kernel, o.storage = kernel, storage
+        _check(lib.cms_set_opts(self._h, C.byref(o)), self._h)
+
+    def set_tree(self, tree: Tree):
+        _check(lib.cms_set_tree(self._h, tree.leaf_n, tree.order_n, *[np.ascontiguousarray(a) for a in tree.arrays()]),
+               self._h)
+
+    def set_samples(self, abd: np.ndarray):
+        abd = np.ascontiguousarray(abd, np.float32)
+        _check(lib.cms_set_samples(self._h, abd.shape[0], abd.reshape(-1) if abd.size else np.zeros(1, np.float32)),
+               self._h)
+        self.n = abd.shape[0]
+
+    def set_samples_csr(self, s: Samples):
+        val = s.val if s.val.size else np.zeros(1, np.float32)
+        col = s.col if s.col.size else np.zeros(1, np.int32)
+        _check(lib.cms_set_samples_csr(self._h, s.n, np.ascontiguousarray(s.rowptr, np.int64), col, val), self._h)
+        self.n = s.n
+
+    def all_pairs(self) -> np.ndarray:
+        out = np.zeros(max(self.n * (self.n - 1) // 2, 1), np.float32)
+        _check(lib.cms_all_pairs(self._h, out), self._h)
+        return out[: self.n * (self.n - 1) // 2]
+
+    def all_pairs_device(self, d_ptr: int, stream: int | None = None):
+        _check(lib.cms_all_pairs_device(self._h, _vp(d_ptr), _vp(stream) if stream else None), self._h)
+
+    def sim_pairs(self, row, col) -> np.ndarray:
+        row = np.ascontiguousarray(row, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        out = np.zeros(max(len(row), 1), np.float32)
+        _check(lib.cms_sim_pairs(self._h, len(row), row if len(row) else np.zeros(1, np.int32),
+                                 col if len(col) else np.zeros(1, np.int32), out), self._h)
+        return out[: len(row)]
+
+    def tile(self, r0: int, nr: int, c0: int, nc: int) -> np.ndarray:
+        out = np.zeros(max(nr * nc, 1), np.float32)
+        _check(lib.cms_tile(self._h, r0, nr, c0, nc, out), self._h)
+        return out[: nr * nc].reshape(nr, nc)
+
+    def prepare(self, force: bool = False):
+        _check(lib.cms_prepare(self._h, int(force)), self._h)
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(lib.cms_get_stats(self._h, C.byref(s)), self._h)
+        return s
+
+
+def packed_index(m, n, N):
+    """Reference packed upper-triangle index p = m*N + n - (m+1)(m+2)/2, m < n."""
+    m = np.asarray(m, np.int64)
+    n = np.asarray(n, np.int64)
+    return m * N + n - (m + 1) * (m + 2) // 2
+
+
+def version() -> str:
+    return lib.cms_version().decode()

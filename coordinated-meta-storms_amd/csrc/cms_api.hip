@@ -1,0 +1,571 @@
+// C ABI of libcms: contexts, device layouts and dispatch to the walk and
+// shared-minimum kernels. Replaces the per-tile, per-GPU host loop of
+// Multi_GPU_split_MetaStorm (cms_cuda.cu:1470-2235): inputs are uploaded once,
+// laid out for the kernels on the device, and every pair is computed by one
+// launch per call on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cms_internal.h"
+#include "kernels.h"
+
+using namespace cms;
+
+struct cms_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cms_opts opts{};
+    std::string err;
+
+    bool has_tree = false;
+    CompiledTree tree;
+    WalkNode *d_nodes = nullptr;
+    WalkChunk *d_chunks = nullptr;
+    int32_t *d_leaf_lp_ptr = nullptr, *d_leaf_lp = nullptr, *d_leaf_uses = nullptr;
+    double *d_beta = nullptr;
+    int32_t *d_entrow = nullptr;
+    int64_t nrows = 0;
+
+    int n = -1;
+    int64_t nnz = 0;
+    bool has_negative = false;
+    int64_t *d_rowptr = nullptr;
+    int32_t *d_col = nullptr;
+    float *d_val = nullptr;
+
+    uint8_t *d_ab = nullptr;
+    int ab_storage = -1;  // storage of the valid walk layout, -1 = none
+    float *d_scale = nullptr, *d_inv = nullptr;
+    size_t cap_ab = 0, cap_scale = 0, cap_inv = 0;
+
+    bool sm_ready = false;
+    uint32_t *d_Y = nullptr, *d_bm = nullptr;
+    double *d_mass = nullptr;
+    size_t cap_Y = 0, cap_bm = 0, cap_mass = 0;
+    int64_t npad = 0, nwords = 0;
+    int scale_bits = 0;
+    unsigned long long *d_counter = nullptr;  // entries visited by shared-minimum tiles
+
+    float *d_out = nullptr;
+    size_t out_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+    bool kern_pending = false, prep_pending = false;
+    cms_stats stats{};
+};
+
+namespace {
+
+int fail(cms_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    set_global_error(msg);
+    return code;
+}
+
+#define HIPCALL(ctx, expr)                                                                         \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? CMS_E_NOMEM : CMS_E_HIP,                  \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <typename T>
+hipError_t dalloc(T *&p, size_t count) {
+    dfree(p);
+    if (count == 0) count = 1;
+    return hipMalloc((void **)&p, count * sizeof(T));
+}
+
+// Reuse an allocation when it is large enough (layouts are rebuilt in place
+// by cms_prepare without returning memory to the driver).
+template <typename T>
+hipError_t dreserve(T *&p, size_t &cap_bytes, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (p && cap_bytes >= bytes) return hipSuccess;
+    dfree(p);
+    cap_bytes = 0;
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e == hipSuccess) cap_bytes = bytes;
+    return e;
+}
+
+void drop_layouts(cms_ctx *c) {
+    dfree(c->d_ab);
+    c->cap_ab = 0;
+    c->ab_storage = -1;
+    dfree(c->d_scale);
+    dfree(c->d_inv);
+    c->cap_scale = c->cap_inv = 0;
+    dfree(c->d_Y);
+    dfree(c->d_bm);
+    dfree(c->d_mass);
+    c->cap_Y = c->cap_bm = c->cap_mass = 0;
+    c->sm_ready = false;
+}
+
+int check_ready(cms_ctx *c) {
+    if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (!c->has_tree) return fail(c, CMS_E_STATE, "no tree loaded (cms_set_tree)");
+    if (c->n < 0) return fail(c, CMS_E_STATE, "no samples loaded (cms_set_samples)");
+    HIPCALL(c, hipSetDevice(c->device));
+    return CMS_OK;
+}
+
+// Block-leaf-major walk layout of the uploaded CSR samples (zeros elsewhere).
+int ensure_walk_layout(cms_ctx *c, int storage) {
+    if (c->ab_storage == storage) return CMS_OK;
+    c->ab_storage = -1;
+    const int64_t nlp = (int64_t)c->tree.leaf_stream.size();
+    const int64_t nblk = (c->n + kWalkBlock - 1) / kWalkBlock;
+    const size_t elem = storage == CMS_STORE_F32 ? 4 : 2;
+    const size_t bytes = (size_t)std::max<int64_t>(nblk, 1) * (size_t)std::max<int64_t>(nlp, 1) * kWalkBlock * elem;
+    HIPCALL(c, dreserve(c->d_ab, c->cap_ab, bytes));
+    HIPCALL(c, hipMemsetAsync(c->d_ab, 0, bytes, c->stream));
+    if (storage != CMS_STORE_F32) {
+        HIPCALL(c, dreserve(c->d_scale, c->cap_scale, (size_t)std::max(c->n, 1)));
+        HIPCALL(c, dreserve(c->d_inv, c->cap_inv, (size_t)std::max(c->n, 1)));
+        HIPCALL(c, launch_sample_scale(c->d_rowptr, c->d_val, c->n, storage, c->d_scale, c->d_inv, c->stream));
+    }
+    HIPCALL(c, launch_walk_scatter(c->d_rowptr, c->d_col, c->d_val, c->n, c->d_leaf_lp_ptr, c->d_leaf_lp, nlp,
+                                   storage, c->d_scale, c->d_ab, c->stream));
+    c->ab_storage = storage;
+    return CMS_OK;
+}
+
+WalkData walk_data(const cms_ctx *c) {
+    WalkData d{};
+    d.nodes = c->d_nodes;
+    d.chunks = c->d_chunks;
+    d.nchunks = (int)c->tree.chunks.size();
+    d.order_n = c->tree.order_n;
+    d.nslots = std::max(c->tree.nslots, 1);
+    d.nlp = (int64_t)c->tree.leaf_stream.size();
+    d.ab = c->d_ab;
+    d.inv_scale = c->d_inv;
+    d.n = c->n;
+    d.storage = c->ab_storage;
+    return d;
+}
+
+int sharedmin_supported(cms_ctx *c) {
+    if (!c->tree.is_tree)
+        return fail(c, CMS_E_UNSUPPORTED, "shared-minimum kernel needs a tree schedule (a value or leaf is read twice)");
+    if (!c->tree.nonneg_beta)
+        return fail(c, CMS_E_UNSUPPORTED, "shared-minimum kernel needs non-negative branch lengths");
+    if (c->has_negative) return fail(c, CMS_E_UNSUPPORTED, "shared-minimum kernel needs non-negative abundances");
+    return CMS_OK;
+}
+
+// Fixed-point embedding Y and the per-block entry bitmaps.
+int ensure_sharedmin(cms_ctx *c) {
+    if (c->sm_ready) return CMS_OK;
+    int rc = sharedmin_supported(c);
+    if (rc) return rc;
+    rc = ensure_walk_layout(c, CMS_STORE_F32);  // the embedding reads the F32 walk layout
+    if (rc) return rc;
+    HIPCALL(c, dreserve(c->d_mass, c->cap_mass, (size_t)std::max(c->n, 1)));
+    HIPCALL(c, launch_sample_mass(c->d_rowptr, c->d_col, c->d_val, c->n, c->d_leaf_uses, c->d_mass, c->stream));
+    std::vector<double> mass(std::max(c->n, 1), 0.0);
+    if (c->n > 0)
+        HIPCALL(c, hipMemcpyAsync(mass.data(), c->d_mass, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    double maxm = 0.0;
+    for (int s = 0; s < c->n; s++) maxm = std::max(maxm, mass[s]);
+    // A column sum of y equals the sample's consumed leaf mass; rounding adds
+    // at most 1/2 per entry. Keep the largest column sum below 2^32.
+    int K = 24;
+    if (maxm > 0) {
+        const double room = 4294967295.0 - (double)c->nrows * 0.5 - 64.0;
+        K = (int)std::floor(std::log2(room / maxm));
+        while (K > -60 && std::ldexp(maxm, K) + (double)c->nrows * 0.5 >= 4294967295.0) K--;
+        K = std::min(K, 60);
+    }
+    c->scale_bits = K;
+    c->npad = ((int64_t)c->n + 63) / 64 * 64;
+    c->nwords = (c->nrows + 31) / 32;
+    const int64_t nblk = c->npad / kSmBlock;
+    const size_t nbm = (size_t)std::max<int64_t>(nblk, 1) * std::max<int64_t>(c->nwords, 1);
+    HIPCALL(c, dreserve(c->d_Y, c->cap_Y, (size_t)std::max<int64_t>(c->nrows, 1) * std::max<int64_t>(c->npad, 1)));
+    HIPCALL(c, dreserve(c->d_bm, c->cap_bm, nbm));
+    HIPCALL(c, hipMemsetAsync(c->d_bm, 0, sizeof(uint32_t) * nbm, c->stream));
+    HIPCALL(c, launch_embed(c->d_nodes, c->tree.order_n, std::max(c->tree.nslots, 1), (const float *)c->d_ab,
+                            (int64_t)c->tree.leaf_stream.size(), c->n, c->npad, c->d_beta, c->d_entrow, c->nrows,
+                            c->nwords, std::ldexp(1.0, K), c->d_Y, c->d_bm, c->stream));
+    c->sm_ready = true;
+    return CMS_OK;
+}
+
+SmData sm_data(const cms_ctx *c) {
+    SmData d{};
+    d.Y = c->d_Y;
+    d.bm = c->d_bm;
+    d.nrows = c->nrows;
+    d.npad = c->npad;
+    d.nwords = c->nwords;
+    d.n = c->n;
+    d.inv_scale = 1.0 / (100.0 * std::ldexp(1.0, c->scale_bits));
+    d.counter = c->d_counter;
+    return d;
+}
+
+int prepare(cms_ctx *c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (c->opts.kernel == CMS_KERNEL_SHAREDMIN) {
+        if (c->sm_ready) return CMS_OK;
+        HIPCALL(c, hipEventRecord(c->evp0, c->stream));
+        rc = ensure_sharedmin(c);
+    } else {
+        if (c->ab_storage == c->opts.storage) return CMS_OK;
+        HIPCALL(c, hipEventRecord(c->evp0, c->stream));
+        rc = ensure_walk_layout(c, c->opts.storage);
+    }
+    if (rc) return rc;
+    HIPCALL(c, hipEventRecord(c->evp1, c->stream));
+    c->prep_pending = true;
+    return CMS_OK;
+}
+
+int upload_csr(cms_ctx *c, int n, const int64_t *rowptr, const int32_t *col, const float *val) {
+    drop_layouts(c);
+    dfree(c->d_rowptr);
+    dfree(c->d_col);
+    dfree(c->d_val);
+    const int64_t nnz = rowptr[n];
+    HIPCALL(c, dalloc(c->d_rowptr, (size_t)n + 1));
+    HIPCALL(c, dalloc(c->d_col, (size_t)std::max<int64_t>(nnz, 1)));
+    HIPCALL(c, dalloc(c->d_val, (size_t)std::max<int64_t>(nnz, 1)));
+    HIPCALL(c, hipMemcpy(c->d_rowptr, rowptr, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        HIPCALL(c, hipMemcpy(c->d_col, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+        HIPCALL(c, hipMemcpy(c->d_val, val, sizeof(float) * nnz, hipMemcpyHostToDevice));
+    }
+    c->n = n;
+    c->nnz = nnz;
+    return CMS_OK;
+}
+
+int ensure_out(cms_ctx *c, size_t count) {
+    if (count <= c->out_cap && c->d_out) return CMS_OK;
+    HIPCALL(c, dalloc(c->d_out, std::max<size_t>(count, 1)));
+    c->out_cap = std::max<size_t>(count, 1);
+    return CMS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cms_opts_default(cms_opts *o) {
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->kernel = CMS_KERNEL_WALK;
+    o->storage = CMS_STORE_F32;
+}
+
+int cms_create(int device, const cms_opts *o, cms_ctx **out) {
+    if (!out) return fail(nullptr, CMS_E_INVAL, "cms_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        return fail(nullptr, CMS_E_HIP, std::string("no HIP device: ") + hipGetErrorString(e));
+    if (device < 0 || device >= count) return fail(nullptr, CMS_E_INVAL, "cms_create: device index out of range");
+    cms_ctx *c = new cms_ctx();
+    c->device = device;
+    cms_opts_default(&c->opts);
+    if (o) c->opts = *o;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->evp0) != hipSuccess || hipEventCreate(&c->evp1) != hipSuccess ||
+        hipMalloc((void **)&c->d_counter, sizeof(unsigned long long)) != hipSuccess) {
+        delete c;
+        return fail(nullptr, CMS_E_HIP, "cms_create: stream/event creation failed");
+    }
+    *out = c;
+    return CMS_OK;
+}
+
+void cms_destroy(cms_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_layouts(c);
+    dfree(c->d_nodes);
+    dfree(c->d_chunks);
+    dfree(c->d_leaf_lp_ptr);
+    dfree(c->d_leaf_lp);
+    dfree(c->d_leaf_uses);
+    dfree(c->d_beta);
+    dfree(c->d_entrow);
+    dfree(c->d_rowptr);
+    dfree(c->d_col);
+    dfree(c->d_val);
+    dfree(c->d_out);
+    dfree(c->d_counter);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evp0) (void)hipEventDestroy(c->evp0);
+    if (c->evp1) (void)hipEventDestroy(c->evp1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *cms_last_error(const cms_ctx *c) { return c ? c->err.c_str() : global_error(); }
+
+int cms_set_opts(cms_ctx *c, const cms_opts *o) {
+    if (!c || !o) return fail(c, CMS_E_INVAL, "cms_set_opts: null argument");
+    if (o->kernel != CMS_KERNEL_WALK && o->kernel != CMS_KERNEL_SHAREDMIN) return fail(c, CMS_E_INVAL, "unknown kernel");
+    if (o->storage < CMS_STORE_F32 || o->storage > CMS_STORE_F16) return fail(c, CMS_E_INVAL, "unknown storage");
+    c->opts = *o;
+    return CMS_OK;
+}
+
+int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const float *d1, const int32_t *o2,
+                 const float *d2, const int32_t *od) {
+    if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    std::string err;
+    CompiledTree t;
+    int rc = compile_tree(leaf_n, order_n, o1, d1, o2, d2, od, kWalkLeafChunk, t, err);
+    if (rc) return fail(c, rc, err);
+    HIPCALL(c, hipSetDevice(c->device));
+    drop_layouts(c);
+    // leaf -> leaf-stream positions (CSR), and consumption counts
+    std::vector<int32_t> ptr((size_t)leaf_n + 1, 0), lp(t.leaf_stream.size()), uses((size_t)std::max(leaf_n, 1), 0);
+    for (int32_t l : t.leaf_stream) ptr[(size_t)l + 1]++;
+    for (int l = 0; l < leaf_n; l++) {
+        uses[l] = ptr[(size_t)l + 1];
+        ptr[(size_t)l + 1] += ptr[l];
+    }
+    {
+        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+        for (size_t k = 0; k < t.leaf_stream.size(); k++) lp[fill[t.leaf_stream[k]]++] = (int32_t)k;
+    }
+    // active entries (beta != 0) get consecutive rows in schedule order
+    std::vector<int32_t> entrow(t.ent_beta.size(), -1);
+    int64_t nrows = 0;
+    for (size_t e = 0; e < t.ent_beta.size(); e++)
+        if (t.ent_beta[e] != 0.0) entrow[e] = (int32_t)nrows++;
+    HIPCALL(c, dalloc(c->d_nodes, t.nodes.size()));
+    HIPCALL(c, dalloc(c->d_chunks, t.chunks.size()));
+    HIPCALL(c, dalloc(c->d_leaf_lp_ptr, ptr.size()));
+    HIPCALL(c, dalloc(c->d_leaf_lp, std::max<size_t>(lp.size(), 1)));
+    HIPCALL(c, dalloc(c->d_leaf_uses, uses.size()));
+    HIPCALL(c, dalloc(c->d_beta, std::max<size_t>(t.ent_beta.size(), 1)));
+    HIPCALL(c, dalloc(c->d_entrow, std::max<size_t>(entrow.size(), 1)));
+    if (!t.nodes.empty()) {
+        HIPCALL(c, hipMemcpy(c->d_nodes, t.nodes.data(), sizeof(WalkNode) * t.nodes.size(), hipMemcpyHostToDevice));
+        HIPCALL(c, hipMemcpy(c->d_chunks, t.chunks.data(), sizeof(WalkChunk) * t.chunks.size(), hipMemcpyHostToDevice));
+        HIPCALL(c, hipMemcpy(c->d_beta, t.ent_beta.data(), sizeof(double) * t.ent_beta.size(), hipMemcpyHostToDevice));
+        HIPCALL(c, hipMemcpy(c->d_entrow, entrow.data(), sizeof(int32_t) * entrow.size(), hipMemcpyHostToDevice));
+    }
+    HIPCALL(c, hipMemcpy(c->d_leaf_lp_ptr, ptr.data(), sizeof(int32_t) * ptr.size(), hipMemcpyHostToDevice));
+    if (!lp.empty()) HIPCALL(c, hipMemcpy(c->d_leaf_lp, lp.data(), sizeof(int32_t) * lp.size(), hipMemcpyHostToDevice));
+    HIPCALL(c, hipMemcpy(c->d_leaf_uses, uses.data(), sizeof(int32_t) * uses.size(), hipMemcpyHostToDevice));
+    c->tree = std::move(t);
+    c->nrows = nrows;
+    c->has_tree = true;
+    c->stats.nslots = c->tree.nslots;
+    c->stats.entries = (int)nrows;
+    return CMS_OK;
+}
+
+int cms_set_samples_csr(cms_ctx *c, int n, const int64_t *rowptr, const int32_t *leaf, const float *val) {
+    if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (!c->has_tree) return fail(c, CMS_E_STATE, "cms_set_samples before cms_set_tree");
+    if (n < 0 || !rowptr || (rowptr[n] > 0 && (!leaf || !val))) return fail(c, CMS_E_INVAL, "cms_set_samples_csr: bad arrays");
+    if (rowptr[0] != 0) return fail(c, CMS_E_INVAL, "rowptr[0] must be 0");
+    bool neg = false;
+    for (int s = 0; s < n; s++) {
+        if (rowptr[s + 1] < rowptr[s]) return fail(c, CMS_E_INVAL, "rowptr not non-decreasing");
+        for (int64_t k = rowptr[s]; k < rowptr[s + 1]; k++) {
+            if (leaf[k] < 0 || leaf[k] >= c->tree.leaf_n) return fail(c, CMS_E_INVAL, "leaf index out of range");
+            if (!(val[k] >= 0.0f)) neg = true;
+        }
+        // a leaf listed twice in one row would be written twice (last wins) in
+        // the dense form; reject to keep both forms identical
+        for (int64_t k = rowptr[s] + 1; k < rowptr[s + 1]; k++)
+            if (leaf[k] <= leaf[k - 1]) return fail(c, CMS_E_INVAL, "CSR columns must be strictly ascending per row");
+    }
+    HIPCALL(c, hipSetDevice(c->device));
+    int rc = upload_csr(c, n, rowptr, leaf, val);
+    if (rc) return rc;
+    c->has_negative = neg;
+    return CMS_OK;
+}
+
+int cms_set_samples(cms_ctx *c, int n, const float *abd) {
+    if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (!c->has_tree) return fail(c, CMS_E_STATE, "cms_set_samples before cms_set_tree");
+    if (n < 0 || (n > 0 && c->tree.leaf_n > 0 && !abd)) return fail(c, CMS_E_INVAL, "cms_set_samples: bad arrays");
+    const int L = c->tree.leaf_n;
+    std::vector<int64_t> rowptr((size_t)n + 1, 0);
+    std::vector<int64_t> cnt(n, 0);
+    for (int s = 0; s < n; s++) {
+        int64_t k = 0;
+        for (int l = 0; l < L; l++) k += abd[(size_t)s * L + l] != 0.0f;
+        cnt[s] = k;
+    }
+    for (int s = 0; s < n; s++) rowptr[s + 1] = rowptr[s] + cnt[s];
+    std::vector<int32_t> col((size_t)std::max<int64_t>(rowptr[n], 1));
+    std::vector<float> val((size_t)std::max<int64_t>(rowptr[n], 1));
+    for (int s = 0; s < n; s++) {
+        int64_t k = rowptr[s];
+        for (int l = 0; l < L; l++) {
+            float v = abd[(size_t)s * L + l];
+            if (v != 0.0f) {
+                col[k] = l;
+                val[k] = v;
+                k++;
+            }
+        }
+    }
+    return cms_set_samples_csr(c, n, rowptr.data(), col.data(), val.data());
+}
+
+int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
+    int rc = prepare(c);
+    if (rc) return rc;
+    if (c->n >= 2 && !d_sim) return fail(c, CMS_E_INVAL, "cms_all_pairs_device: null output");
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (st != c->stream) {
+        // layouts are built on the context stream: order the caller's stream after them
+        HIPCALL(c, hipEventRecord(c->evp1, c->stream));
+        HIPCALL(c, hipStreamWaitEvent(st, c->evp1, 0));
+    }
+    const double pairs = (double)c->n * (c->n - 1) / 2.0;
+    c->stats.dense_updates = pairs * c->tree.order_n;
+    HIPCALL(c, hipEventRecord(c->ev0, st));
+    if (c->opts.kernel == CMS_KERNEL_SHAREDMIN) {
+        HIPCALL(c, launch_zero_counter(c->d_counter, st));
+        HIPCALL(c, launch_sm_all_pairs(sm_data(c), d_sim, st));
+        const int64_t nb = (c->n + kSmBlock - 1) / kSmBlock;
+        c->stats.tiles = nb * (nb + 1) / 2;
+    } else {
+        HIPCALL(c, launch_walk_all_pairs(walk_data(c), d_sim, st));
+        const int64_t nb = (c->n + kWalkBlock - 1) / kWalkBlock;
+        c->stats.tiles = nb * (nb + 1) / 2;
+        c->stats.pair_node_updates = (double)c->stats.tiles * 64.0 * c->tree.order_n;
+        c->stats.valu_ops = c->stats.pair_node_updates * 14.0;
+        const double elem = c->ab_storage == CMS_STORE_F32 ? 4.0 : 2.0;
+        c->stats.bytes_moved = (double)c->stats.tiles * 16.0 * elem * (double)c->tree.leaf_stream.size();
+    }
+    HIPCALL(c, hipEventRecord(c->ev1, st));
+    c->kern_pending = true;
+    return CMS_OK;
+}
+
+// Rebuild the device layouts of the current kernel from the uploaded samples
+// (the walk layout; for the shared-minimum form also the embedding and the
+// bitmaps), reusing allocations.
+int cms_prepare(cms_ctx *c, int force) {
+    if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (force) {
+        c->ab_storage = -1;
+        c->sm_ready = false;
+    }
+    return prepare(c);
+}
+
+int cms_all_pairs(cms_ctx *c, float *sim) {
+    int rc = prepare(c);
+    if (rc) return rc;
+    const size_t np = (size_t)c->n * (c->n > 0 ? c->n - 1 : 0) / 2;
+    if (np == 0) return CMS_OK;
+    if (!sim) return fail(c, CMS_E_INVAL, "cms_all_pairs: null output");
+    rc = ensure_out(c, np);
+    if (rc) return rc;
+    rc = cms_all_pairs_device(c, c->d_out, nullptr);
+    if (rc) return rc;
+    HIPCALL(c, hipMemcpyAsync(sim, c->d_out, sizeof(float) * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return CMS_OK;
+}
+
+int cms_sim_pairs(cms_ctx *c, int64_t n_pairs, const int32_t *row, const int32_t *col, float *sim) {
+    int rc = prepare(c);
+    if (rc) return rc;
+    if (n_pairs < 0 || (n_pairs > 0 && (!row || !col || !sim))) return fail(c, CMS_E_INVAL, "cms_sim_pairs: bad arrays");
+    if (n_pairs == 0) return CMS_OK;
+    for (int64_t i = 0; i < n_pairs; i++)
+        if (row[i] < 0 || row[i] >= c->n || col[i] < 0 || col[i] >= c->n)
+            return fail(c, CMS_E_INVAL, "cms_sim_pairs: sample index out of range");
+    int32_t *d_r = nullptr, *d_c = nullptr;
+    HIPCALL(c, dalloc(d_r, (size_t)n_pairs));
+    HIPCALL(c, dalloc(d_c, (size_t)n_pairs));
+    rc = ensure_out(c, (size_t)n_pairs);
+    if (rc) return rc;
+    HIPCALL(c, hipMemcpyAsync(d_r, row, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
+    HIPCALL(c, hipMemcpyAsync(d_c, col, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
+    if (c->opts.kernel == CMS_KERNEL_SHAREDMIN)
+        HIPCALL(c, launch_sm_pairs(sm_data(c), n_pairs, d_r, d_c, c->d_out, c->stream));
+    else
+        HIPCALL(c, launch_walk_pairs(walk_data(c), n_pairs, d_r, d_c, c->d_out, c->stream));
+    HIPCALL(c, hipMemcpyAsync(sim, c->d_out, sizeof(float) * n_pairs, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    dfree(d_r);
+    dfree(d_c);
+    return CMS_OK;
+}
+
+int cms_tile(cms_ctx *c, int r0, int nr, int c0, int nc, float *out) {
+    int rc = prepare(c);
+    if (rc) return rc;
+    if (r0 < 0 || c0 < 0 || nr < 0 || nc < 0 || (int64_t)r0 + nr > c->n || (int64_t)c0 + nc > c->n)
+        return fail(c, CMS_E_INVAL, "cms_tile: range outside the sample set");
+    const size_t cnt = (size_t)nr * nc;
+    if (cnt == 0) return CMS_OK;
+    if (!out) return fail(c, CMS_E_INVAL, "cms_tile: null output");
+    rc = ensure_out(c, cnt);
+    if (rc) return rc;
+    if (c->opts.kernel == CMS_KERNEL_SHAREDMIN)
+        HIPCALL(c, launch_sm_rect(sm_data(c), r0, nr, c0, nc, c->d_out, c->stream));
+    else
+        HIPCALL(c, launch_walk_rect(walk_data(c), r0, nr, c0, nc, c->d_out, c->stream));
+    HIPCALL(c, hipMemcpyAsync(out, c->d_out, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return CMS_OK;
+}
+
+int cms_get_stats(cms_ctx *c, cms_stats *s) {
+    if (!c || !s) return CMS_E_INVAL;
+    if (c->kern_pending) {
+        float ms = 0;
+        HIPCALL(c, hipEventSynchronize(c->ev1));
+        HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->stats.kernel_ms = ms;
+        if (c->opts.kernel == CMS_KERNEL_SHAREDMIN) {
+            unsigned long long rows = 0;
+            HIPCALL(c, hipMemcpy(&rows, c->d_counter, sizeof rows, hipMemcpyDeviceToHost));
+            // each visited entry is one min+add for all 32 x 32 pairs of the tile
+            c->stats.pair_node_updates = (double)rows * kSmBlock * kSmBlock;
+            c->stats.valu_ops = c->stats.pair_node_updates * 1.5;
+            c->stats.bytes_moved = (double)rows * 2.0 * kSmBlock * 4.0;
+        }
+        c->kern_pending = false;
+    }
+    if (c->prep_pending) {
+        float ms = 0;
+        HIPCALL(c, hipEventSynchronize(c->evp1));
+        HIPCALL(c, hipEventElapsedTime(&ms, c->evp0, c->evp1));
+        c->stats.prep_ms = ms;
+        c->prep_pending = false;
+    }
+    *s = c->stats;
+    s->scale_bits = c->scale_bits;
+    return CMS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Internal host-side structures shared by the loader, the schedule compiler and
+// the HIP backend of libcms. Nothing here is part of the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cms.h"
+
+namespace cms {
+
+// Error text of the last failure without a context (thread-local).
+void set_global_error(const std::string &msg);
+const char *global_error();
+
+// Compiled walk schedule node, 16 bytes, read with one scalar load per node.
+//   code bits  0..7  : slot of child 1 (valid when not a leaf)
+//             8..15  : slot of child 2
+//            16..23  : destination slot
+//            24      : child 1 is a leaf
+//            25      : child 2 is a leaf
+//            26      : destination is written (value is read later)
+//   leafpos           : leaf-stream position of this node's first leaf child
+//   w1, w2            : 1 - dist clamped to MIN_DIST when negative (comp.h:341-344)
+struct WalkNode {
+    uint32_t code;
+    int32_t leafpos;
+    float w1;
+    float w2;
+};
+static_assert(sizeof(WalkNode) == 16, "WalkNode must be 16 bytes");
+
+constexpr uint32_t kC1Leaf = 1u << 24;
+constexpr uint32_t kC2Leaf = 1u << 25;
+constexpr uint32_t kDstLive = 1u << 26;
+
+// A chunk of consecutive schedule nodes whose leaf children occupy at most
+// kLeafChunk consecutive leaf-stream positions. The walk kernel stages one
+// chunk of leaf values in LDS at a time.
+struct WalkChunk {
+    int32_t node_begin;
+    int32_t node_end;
+    int32_t leaf_begin;
+    int32_t leaf_end;
+};
+
+struct CompiledTree {
+    int leaf_n = 0;
+    int order_n = 0;
+    int nslots = 0;          // slots after liveness re-allocation
+    bool is_tree = false;    // every leaf and every value consumed at most once
+    bool nonneg_beta = true; // all 1 - w >= 0 (required by the embedding form)
+    std::vector<WalkNode> nodes;
+    std::vector<int32_t> leaf_stream;  // original leaf index per stream position
+    std::vector<WalkChunk> chunks;
+    int leaf_chunk = 0;
+
+    // Embedding form: one entry per child edge (2 per node).
+    //   entry 2i+j is child j of node i; value y = beta * x(child)
+    //   child_leafpos[e] >= 0: leaf-stream position, else -1 - (producer node)
+    std::vector<int32_t> ent_child;
+    std::vector<double> ent_beta;
+};
+
+// Validate and compile the reference schedule arrays (comp.h:161-192 layout).
+// Returns CMS_OK or CMS_E_SCHEDULE with a message in `err`.
+int compile_tree(int leaf_n, int order_n, const int32_t *o1, const float *d1,
+                 const int32_t *o2, const float *d2, const int32_t *od,
+                 int leaf_chunk, CompiledTree &out, std::string &err);
+
+// Float formatting exactly like std::ostream's default for a double
+// (precision 6, %g conversion). Appends to `s`.
+void append_g6(std::string &s, double v);
+
+} // namespace cms

@@ -1,0 +1,145 @@
+// cuda-comp — command-line driver with the reference's flags and .dist output
+// (Parse_Para / Output_Matrix / main of cms_cuda.cu:1222-1342, 2237-2261),
+// computing -M 0 (weighted Meta-Storms) on the GPU through libcms.
+//
+//   cuda-comp -T table -D G -o out.dist [-r T|F] [-d T|F] [-M 0] [-t n]
+//             [--kernel walk|sharedmin] [--storage f32|bf16|f16] [--device k]
+//
+// The database is read from $ParallelMETA/databases (db.config, tree/id.txt,
+// tree/order.txt, copy_number.txt). Errors exit with status 1 (the reference
+// prints and exits 0 or continues with empty data).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cms.h"
+
+namespace {
+
+[[noreturn]] void printhelp() {
+    printf(" Welcome to Coordinated Meta Storm (MI355X)\n");
+    printf("\tCompute the Meta-Storms distance among samples, default use rRNA copy number correction\n");
+    printf("Usage: \ncuda-comp [Option] Value\nOptions: \n");
+    printf("\t-D (upper) ref database, default is G\n");
+    printf("\t[Input options, required]\n");
+    printf("\t  -T (upper) Input OTU count table (*.OTU.Count) for multi-sample comparison\n");
+    printf("\t[Output options]\n");
+    printf("\t  -o Output file\n");
+    printf("\t  -d Output format, distance (T) or similarity (F), default is T\n");
+    printf("\t[Other options]\n");
+    printf("\t  -M (upper) Distance Metric, 0: Meta-Storms, default is 0\n");
+    printf("\t  -r rRNA copy number correction, T(rue) or F(alse), default is T\n");
+    printf("\t  -t Number of host threads for loading, default is auto\n");
+    printf("\t  --kernel walk|sharedmin  GPU kernel (walk is bit-identical to comp -M 0)\n");
+    printf("\t  --storage f32|bf16|f16   walk abundance storage, default f32\n");
+    printf("\t  --device k               GPU index, default 0\n");
+    printf("\t  -h Help\n");
+    exit(0);
+}
+
+[[noreturn]] void die(const std::string &msg) {
+    fprintf(stderr, "Error: %s\n", msg.c_str());
+    exit(1);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    char ref_db = 'G';
+    std::string table_path, out_path;
+    bool cp_correct = true, is_sim = false;
+    int metric = 0, device = 0;
+    cms_opts opts;
+    cms_opts_default(&opts);
+    if (argc == 1) printhelp();
+    int i = 1;
+    while (i < argc) {
+        std::string a = argv[i];
+        if (a.size() < 2 || a[0] != '-') {
+            fprintf(stderr, "Argument # %d Error : Arguments must start with -\n", i);
+            return 1;
+        }
+        auto need = [&](int k) -> const char * {
+            if (i + k >= argc) die("missing value for " + a);
+            return argv[i + k];
+        };
+        if (a == "--kernel") {
+            std::string v = need(1);
+            if (v == "walk") opts.kernel = CMS_KERNEL_WALK;
+            else if (v == "sharedmin") opts.kernel = CMS_KERNEL_SHAREDMIN;
+            else die("unknown kernel " + v);
+        } else if (a == "--storage") {
+            std::string v = need(1);
+            if (v == "f32") opts.storage = CMS_STORE_F32;
+            else if (v == "bf16") opts.storage = CMS_STORE_BF16;
+            else if (v == "f16") opts.storage = CMS_STORE_F16;
+            else die("unknown storage " + v);
+        } else if (a == "--device") {
+            device = atoi(need(1));
+        } else {
+            switch (a[1]) {
+            case 'D': ref_db = need(1)[0]; break;
+            case 'T': table_path = need(1); break;
+            case 'o': out_path = need(1); break;
+            case 'M': metric = atoi(need(1)); break;
+            case 'r': { const char *v = need(1); if (v[0] == 'f' || v[0] == 'F') cp_correct = false; } break;
+            case 'd': { const char *v = need(1); if (v[0] == 'f' || v[0] == 'F') is_sim = true; } break;
+            case 't': (void)need(1); break;
+            case 'P': case 'c': (void)need(1); break;
+            case 'i': case 'l': case 'p':
+                die("single-file (-i) and list (-l/-p) modes are not part of this build; use -T");
+            case 'h': printhelp();
+            default:
+                fprintf(stderr, "Error: Unrec argument %s\n", argv[i]);
+                printhelp();
+            }
+        }
+        i += 2;
+    }
+    if (table_path.empty()) die("an OTU table (-T) is required");
+    if (metric != 0) die("only -M 0 (weighted Meta-Storms) runs on the GPU in this build");
+
+    auto t0 = std::chrono::steady_clock::now();
+    cms_db *db = nullptr;
+    if (cms_db_open(nullptr, ref_db, &db) != CMS_OK) die(cms_last_error(nullptr));
+    if (!cms_db_has_tree(db)) die("the selected database has no phylogeny tree");
+    cms_table *table = nullptr;
+    if (cms_table_load(table_path.c_str(), &table) != CMS_OK) die(cms_last_error(nullptr));
+    const int n = cms_table_n(table), L = cms_db_leaf_n(db), on = cms_db_order_n(db);
+    std::vector<float> abd((size_t)n * L);
+    if (cms_table_abundance(db, table, cp_correct ? 1 : 0, abd.data(), nullptr) != CMS_OK) die(cms_last_error(nullptr));
+    printf("%d files loaded\n", n);
+    auto t1 = std::chrono::steady_clock::now();
+
+    const int32_t *o1, *o2, *od;
+    const float *d1, *d2;
+    cms_db_order(db, &o1, &d1, &o2, &d2, &od);
+    cms_ctx *ctx = nullptr;
+    if (cms_create(device, &opts, &ctx) != CMS_OK) die(cms_last_error(nullptr));
+    if (cms_set_tree(ctx, L, on, o1, d1, o2, d2, od) != CMS_OK) die(cms_last_error(ctx));
+    if (cms_set_samples(ctx, n, abd.data()) != CMS_OK) die(cms_last_error(ctx));
+    abd.clear();
+    abd.shrink_to_fit();
+    std::vector<float> sim((size_t)n * (n > 0 ? n - 1 : 0) / 2);
+    if (cms_all_pairs(ctx, sim.data()) != CMS_OK) die(cms_last_error(ctx));
+    cms_stats st;
+    cms_get_stats(ctx, &st);
+    auto t2 = std::chrono::steady_clock::now();
+
+    if (out_path.empty()) die("Cannot open output file : ");
+    std::vector<const char *> names(n);
+    for (int k = 0; k < n; k++) names[k] = cms_table_sample_name(table, k);
+    if (cms_write_dist(out_path.c_str(), n, names.data(), sim.data(), is_sim ? 1 : 0) != CMS_OK)
+        die(cms_last_error(nullptr));
+    auto t3 = std::chrono::steady_clock::now();
+    auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+    printf("*** load %.3f s, gpu %.3f s (pair kernel %.3f ms), write %.3f s\n", sec(t0, t1), sec(t1, t2), st.kernel_ms,
+           sec(t2, t3));
+    cms_destroy(ctx);
+    cms_table_free(table);
+    cms_db_close(db);
+    return 0;
+}

@@ -1,0 +1,471 @@
+// Host loader: reference database configuration, tree files, copy numbers,
+// OTU count tables, abundance normalisation and the .dist writer.
+//
+// Behaviour mirrors the reference loaders (cited per function); the code is a
+// fresh implementation with a tokenizer instead of iostreams so that tables of
+// 10^4 x 10^5 entries load in seconds, and with errors returned instead of
+// exit().
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "cms_internal.h"
+
+using cms::set_global_error;
+
+struct cms_db {
+    std::string root, name, path;
+    bool has_tree = false, has_cp = false;
+    std::vector<std::string> ids;                 // tree/id.txt, one per non-empty line
+    std::unordered_map<std::string, int> id_hash; // last index wins (comp.h:126-127)
+    std::vector<int32_t> o1, o2, od;
+    std::vector<float> d1, d2;
+    std::unordered_map<std::string, float> cp;    // first occurrence wins (db.h:269-271)
+};
+
+struct cms_table {
+    std::vector<std::string> features;
+    std::vector<std::string> samples;
+    std::vector<float> abd;  // n x f row-major
+    int skipped = 0;
+};
+
+namespace {
+
+bool read_file(const std::string &path, std::string &buf) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    buf.clear();
+    char tmp[1 << 16];
+    size_t n;
+    while ((n = fread(tmp, 1, sizeof tmp, f)) > 0) buf.append(tmp, n);
+    fclose(f);
+    return true;
+}
+
+// Split into lines like std::getline (the '\n' is dropped, a final line
+// without '\n' still counts, a trailing '\n' does not create an empty line).
+void split_lines(const std::string &buf, std::vector<std::pair<size_t, size_t>> &lines) {
+    lines.clear();
+    size_t start = 0, n = buf.size();
+    while (start < n) {
+        const void *p = memchr(buf.data() + start, '\n', n - start);
+        size_t end = p ? (size_t)((const char *)p - buf.data()) : n;
+        lines.emplace_back(start, end - start);
+        start = end + 1;
+    }
+}
+
+inline bool is_ws(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+// Next whitespace-delimited token in [p, e); returns false at end.
+inline bool next_token(const char *&p, const char *e, const char *&ts, const char *&te) {
+    while (p < e && is_ws(*p)) p++;
+    if (p >= e) return false;
+    ts = p;
+    while (p < e && !is_ws(*p)) p++;
+    te = p;
+    return true;
+}
+
+// Decimal float token as read by `stream >> float`: the longest prefix of the
+// form [sign]digits[.digits][(e|E)[sign]digits]. Returns 1 on a full token,
+// 2 when a valid prefix is followed by other characters (the value counts and
+// the next extraction fails), 0 when no number can be read.
+int parse_float_token(const char *ts, const char *te, float &v) {
+    const char *p = ts;
+    if (p < te && (*p == '+' || *p == '-')) p++;
+    const char *ds = p;
+    while (p < te && *p >= '0' && *p <= '9') p++;
+    bool digits = p > ds;
+    if (p < te && *p == '.') {
+        p++;
+        const char *fs = p;
+        while (p < te && *p >= '0' && *p <= '9') p++;
+        digits = digits || p > fs;
+    }
+    if (!digits) return 0;
+    if (p < te && (*p == 'e' || *p == 'E')) {
+        const char *q = p + 1;
+        if (q < te && (*q == '+' || *q == '-')) q++;
+        const char *es = q;
+        while (q < te && *q >= '0' && *q <= '9') q++;
+        if (q == es) return 0;  // dangling exponent: the stream conversion fails
+        p = q;
+    }
+    char tmp[128];
+    size_t len = (size_t)(p - ts);
+    std::string big;
+    const char *s;
+    if (len < sizeof tmp) {
+        memcpy(tmp, ts, len);
+        tmp[len] = 0;
+        s = tmp;
+    } else {
+        big.assign(ts, len);
+        s = big.c_str();
+    }
+    v = strtof(s, nullptr);
+    return p == te ? 1 : 2;
+}
+
+// `stream >> int` on a token (order.txt fields).
+bool parse_int_token(const char *ts, const char *te, int32_t &v) {
+    const char *p = ts;
+    bool neg = false;
+    if (p < te && (*p == '+' || *p == '-')) { neg = (*p == '-'); p++; }
+    if (p >= te || *p < '0' || *p > '9') return false;
+    long long x = 0;
+    while (p < te && *p >= '0' && *p <= '9') { x = x * 10 + (*p - '0'); if (x > 4000000000LL) x = 4000000000LL; p++; }
+    x = neg ? -x : x;
+    if (x > INT32_MAX) x = INT32_MAX;
+    if (x < INT32_MIN) x = INT32_MIN;
+    v = (int32_t)x;
+    return true;
+}
+
+// Check_OTU (utility.h:181-190): strip an "otu_"/"OTU_" prefix when longer than 4.
+std::string check_otu(const std::string &otu) {
+    if (otu.size() > 4) {
+        if (otu.compare(0, 4, "otu_") == 0 || otu.compare(0, 4, "OTU_") == 0) return otu.substr(4);
+    }
+    return otu;
+}
+
+}  // namespace
+
+extern "C" {
+
+// _PMDB::Load_config (db.h:287-358) + Set_DB (db.h:32-67) + _Comp_Tree::Init
+// (comp.h:122-192) + _PMDB::Load_Copy_Number (db.h:251-280).
+int cms_db_open(const char *root, char db_letter, cms_db **out) {
+    if (!out) { set_global_error("cms_db_open: out is NULL"); return CMS_E_INVAL; }
+    *out = nullptr;
+    std::string r;
+    if (root) r = root;
+    else {
+        const char *env = getenv("ParallelMETA");
+        if (!env) {
+            set_global_error("Please set the environment variable \"ParallelMETA\" to the directory");
+            return CMS_E_INVAL;
+        }
+        r = env;
+    }
+    std::string buf;
+    if (!read_file(r + "/databases/db.config", buf)) {
+        set_global_error("Cannot open reference database configuration: " + r + "/databases/db.config");
+        return CMS_E_IO;
+    }
+    std::vector<std::pair<size_t, size_t>> lines;
+    split_lines(buf, lines);
+    std::map<char, std::vector<std::string>> config;
+    size_t li = 0;
+    while (li < lines.size()) {
+        std::string first = buf.substr(lines[li].first, lines[li].second);
+        li++;
+        if (first.empty() || first[0] == '#') continue;
+        std::vector<std::string> entry{first};
+        for (int k = 0; k < 7; k++) {
+            while (li < lines.size()) {
+                std::string a = buf.substr(lines[li].first, lines[li].second);
+                li++;
+                if (a.empty() || a[0] == '#') continue;
+                entry.push_back(a);
+                break;
+            }
+        }
+        if (entry.size() < 8) break;          // incomplete entry ends parsing
+        if (config.count(entry[0][0])) continue; // duplicate id keeps the first
+        config[entry[0][0]] = entry;
+    }
+    if (config.empty()) {
+        set_global_error("Cannot open reference database configuration (no complete entry)");
+        return CMS_E_IO;
+    }
+    // _Comp_Tree's member _PMDB is default-constructed with 'G' first
+    // (db.h:24-26), which indexes the 'G' entry unconditionally.
+    if (!config.count('G')) {
+        set_global_error("db.config has no entry 'G' (the reference dereferences it unconditionally)");
+        return CMS_E_IO;
+    }
+    char key = config.count(db_letter) ? db_letter : 'G';
+    const std::vector<std::string> &cfg = config[key];
+    auto yes = [](const std::string &s) { return !s.empty() && (s[0] == 'Y' || s[0] == 'y'); };
+    cms_db *db = new cms_db();
+    db->root = r;
+    db->name = cfg[2];
+    db->has_tree = yes(cfg[5]);
+    db->has_cp = yes(cfg[6]);
+    db->path = r + "/databases/" + db->name + "/";
+
+    // tree/id.txt (comp.h:139-159)
+    if (!read_file(db->path + "tree/id.txt", buf)) {
+        set_global_error("Cannot open file : " + db->path + "tree/id.txt");
+        delete db;
+        return CMS_E_IO;
+    }
+    split_lines(buf, lines);
+    for (auto &l : lines)
+        if (l.second) db->ids.emplace_back(buf.substr(l.first, l.second));
+    for (size_t i = 0; i < db->ids.size(); i++) db->id_hash[db->ids[i]] = (int)i;
+
+    // tree/order.txt (comp.h:161-192)
+    if (db->has_tree) {
+        if (!read_file(db->path + "tree/order.txt", buf)) {
+            set_global_error("Cannot open file : " + db->path + "tree/order.txt");
+            delete db;
+            return CMS_E_IO;
+        }
+        split_lines(buf, lines);
+        for (auto &l : lines) {
+            if (!l.second) continue;
+            const char *p = buf.data() + l.first, *e = p + l.second, *ts, *te;
+            int32_t a = 0, b = 0, d = 0;
+            float x = 0, y = 0;
+            bool ok = next_token(p, e, ts, te) && parse_int_token(ts, te, a);
+            ok = ok && next_token(p, e, ts, te) && parse_float_token(ts, te, x) == 1;
+            ok = ok && next_token(p, e, ts, te) && parse_int_token(ts, te, b);
+            ok = ok && next_token(p, e, ts, te) && parse_float_token(ts, te, y) == 1;
+            ok = ok && next_token(p, e, ts, te) && parse_int_token(ts, te, d);
+            (void)ok;  // fields after the first failure stay 0, as with the stream
+            db->o1.push_back(a);
+            db->d1.push_back(x);
+            db->o2.push_back(b);
+            db->d2.push_back(y);
+            db->od.push_back(d);
+        }
+    }
+
+    // copy_number.txt (db.h:251-280): header skipped, "id cp" per line.
+    if (db->has_cp) {
+        if (read_file(db->path + "copy_number.txt", buf)) {
+            split_lines(buf, lines);
+            for (size_t k = 1; k < lines.size(); k++) {
+                const char *p = buf.data() + lines[k].first, *e = p + lines[k].second, *ts, *te;
+                if (!next_token(p, e, ts, te)) continue;
+                std::string id(ts, te);
+                float v;
+                if (!next_token(p, e, ts, te) || parse_float_token(ts, te, v) == 0) continue;
+                if (!db->cp.count(id)) db->cp[id] = v;
+            }
+        } else {
+            fprintf(stderr, "Warning: Cannot open copy number table : %scopy_number.txt, copy number normalization is disabled\n",
+                    db->path.c_str());
+        }
+    }
+    *out = db;
+    return CMS_OK;
+}
+
+void cms_db_close(cms_db *db) { delete db; }
+int cms_db_leaf_n(const cms_db *db) { return db ? (int)db->ids.size() : CMS_E_INVAL; }
+int cms_db_order_n(const cms_db *db) { return db ? (int)db->o1.size() : CMS_E_INVAL; }
+int cms_db_has_tree(const cms_db *db) { return db ? (int)db->has_tree : CMS_E_INVAL; }
+int cms_db_has_cp(const cms_db *db) { return db ? (int)db->has_cp : CMS_E_INVAL; }
+
+int cms_db_order(const cms_db *db, const int32_t **o1, const float **d1, const int32_t **o2,
+                 const float **d2, const int32_t **od) {
+    if (!db) return CMS_E_INVAL;
+    if (o1) *o1 = db->o1.data();
+    if (d1) *d1 = db->d1.data();
+    if (o2) *o2 = db->o2.data();
+    if (d2) *d2 = db->d2.data();
+    if (od) *od = db->od.data();
+    return CMS_OK;
+}
+
+const char *cms_db_leaf_name(const cms_db *db, int i) {
+    if (!db || i < 0 || i >= (int)db->ids.size()) return "NULL";  // Get_Id (comp.h:92-96)
+    return db->ids[i].c_str();
+}
+
+// _Table_Format::Load_Table (table_format.h:110-165).
+int cms_table_load(const char *path, cms_table **out) {
+    if (!path || !out) { set_global_error("cms_table_load: null argument"); return CMS_E_INVAL; }
+    *out = nullptr;
+    std::string buf;
+    if (!read_file(path, buf)) {
+        set_global_error(std::string("Cannot open file : ") + path);
+        return CMS_E_IO;
+    }
+    std::vector<std::pair<size_t, size_t>> lines;
+    split_lines(buf, lines);
+    cms_table *t = new cms_table();
+    if (!lines.empty()) {
+        const char *p = buf.data() + lines[0].first, *e = p + lines[0].second, *ts, *te;
+        if (next_token(p, e, ts, te))  // title token ignored
+            while (next_token(p, e, ts, te)) t->features.emplace_back(ts, te);
+    }
+    const size_t F = t->features.size();
+    const size_t nl = lines.size() > 0 ? lines.size() - 1 : 0;
+    // Parse rows in parallel, keep file order.
+    std::vector<std::string> names(nl);
+    std::vector<char> ok(nl, 0);
+    std::vector<float> vals(nl * F);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long r = 0; r < (long)nl; r++) {
+        const auto &l = lines[r + 1];
+        const char *p = buf.data() + l.first, *e = p + l.second, *ts, *te;
+        if (next_token(p, e, ts, te)) names[r].assign(ts, te);
+        size_t cnt = 0;
+        float *row = vals.data() + (size_t)r * F;
+        while (next_token(p, e, ts, te)) {
+            float v;
+            int st = parse_float_token(ts, te, v);
+            if (st == 0) break;
+            if (cnt < F) row[cnt] = v;
+            cnt++;
+            if (st == 2) break;
+        }
+        ok[r] = (cnt == F);
+    }
+    for (size_t r = 0; r < nl; r++) {
+        if (!ok[r]) {
+            fprintf(stderr, "Error: Sample: %s does not have %zu features\n", names[r].c_str(), F);
+            t->skipped++;
+            continue;
+        }
+        t->samples.push_back(names[r]);
+        t->abd.insert(t->abd.end(), vals.begin() + r * F, vals.begin() + (r + 1) * F);
+    }
+    *out = t;
+    return CMS_OK;
+}
+
+int cms_table_from_dense(int n, int f, const char *const *sample_names,
+                         const char *const *feature_names, const float *counts, cms_table **out) {
+    if (!out || n < 0 || f < 0 || (n && !sample_names) || (f && !feature_names) || (n && f && !counts)) {
+        set_global_error("cms_table_from_dense: bad argument");
+        return CMS_E_INVAL;
+    }
+    cms_table *t = new cms_table();
+    for (int j = 0; j < f; j++) t->features.emplace_back(feature_names[j]);
+    for (int i = 0; i < n; i++) t->samples.emplace_back(sample_names[i]);
+    t->abd.assign(counts, counts + (size_t)n * f);
+    *out = t;
+    return CMS_OK;
+}
+
+void cms_table_free(cms_table *t) { delete t; }
+int cms_table_n(const cms_table *t) { return t ? (int)t->samples.size() : CMS_E_INVAL; }
+int cms_table_f(const cms_table *t) { return t ? (int)t->features.size() : CMS_E_INVAL; }
+const char *cms_table_sample_name(const cms_table *t, int i) {
+    return (t && i >= 0 && i < (int)t->samples.size()) ? t->samples[i].c_str() : nullptr;
+}
+const char *cms_table_feature_name(const cms_table *t, int j) {
+    return (t && j >= 0 && j < (int)t->features.size()) ? t->features[j].c_str() : nullptr;
+}
+int cms_table_skipped(const cms_table *t) { return t ? t->skipped : CMS_E_INVAL; }
+
+// _Comp_Tree::Load_abd(_Table_Format*, ...) (comp.h:243-279):
+//   for each column with value > 0: otu = Check_OTU(name); cp from the copy
+//   number table when correcting; if otu is a tree leaf: x[otu] = v / cp
+//   (a later duplicate overwrites x but both add to the total), total += x
+//   in column order (FP32); total = (float)((double)total / 100.0);
+//   abd[leaf] = x / total.
+int cms_table_abundance(const cms_db *db, const cms_table *t, int cp_correct, float *abd, int *mapped) {
+    if (!db || !t || !abd) { set_global_error("cms_table_abundance: null argument"); return CMS_E_INVAL; }
+    const size_t L = db->ids.size(), F = t->features.size(), N = t->samples.size();
+    std::vector<int32_t> leaf_of(F, -1);
+    std::vector<float> cp_of(F, 1.0f);
+    for (size_t j = 0; j < F; j++) {
+        std::string a = check_otu(t->features[j]);
+        if (cp_correct) {
+            auto it = db->cp.find(a);
+            if (it != db->cp.end()) cp_of[j] = it->second;
+        }
+        auto it = db->id_hash.find(a);
+        if (it != db->id_hash.end()) leaf_of[j] = it->second;
+    }
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long s = 0; s < (long)N; s++) {
+        float *out = abd + (size_t)s * L;
+        memset(out, 0, L * sizeof(float));
+        const float *row = t->abd.data() + (size_t)s * F;
+        std::vector<int32_t> touched;
+        std::vector<char> seen;  // lazily sized
+        float total = 0;
+        for (size_t j = 0; j < F; j++) {
+            if (row[j] > 0 && leaf_of[j] >= 0) {
+                float x = row[j] / cp_of[j];
+                int32_t l = leaf_of[j];
+                out[l] = x;  // later duplicate overwrites
+                touched.push_back(l);
+                total += x;
+            }
+        }
+        total /= 100.0;
+        int m = 0;
+        if (seen.size() < L) seen.assign(L, 0);
+        for (int32_t l : touched) {
+            if (seen[l]) continue;
+            seen[l] = 1;
+            out[l] = out[l] / total;
+            m++;
+        }
+        if (mapped) mapped[s] = m;
+    }
+    return CMS_OK;
+}
+
+// Output_Matrix (cms_cuda.cu:1305-1342, comp_sam.cpp:184-219).
+int cms_write_dist(const char *path, int n, const char *const *names, const float *sim, int is_sim) {
+    if (!path || n < 0 || (n && !names) || (n > 1 && !sim)) {
+        set_global_error("cms_write_dist: bad argument");
+        return CMS_E_INVAL;
+    }
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_global_error(std::string("Cannot open output file : ") + path);
+        return CMS_E_IO;
+    }
+    std::string hdr;
+    for (int i = 0; i < n; i++) { hdr += '\t'; hdr += names[i]; }
+    hdr += '\n';
+    fwrite(hdr.data(), 1, hdr.size(), f);
+    const int batch = 256;
+    std::vector<std::string> rows(batch);
+    for (int r0 = 0; r0 < n; r0 += batch) {
+        int r1 = r0 + batch < n ? r0 + batch : n;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int i = r0; i < r1; i++) {
+            std::string &s = rows[i - r0];
+            s.clear();
+            s.reserve((size_t)n * 10 + 32);
+            s += names[i];
+            for (int j = 0; j < n; j++) {
+                long ii = i <= j ? i : j, jj = i >= j ? i : j;
+                s += '\t';
+                if (ii == jj) {
+                    s += is_sim ? "1" : "0";
+                    continue;
+                }
+                long p = ii * (long)n + jj - (1 + ii + 1) * (ii + 1) / 2;
+                double v = is_sim ? (double)sim[p] : 1.0 - (double)sim[p];
+                cms::append_g6(s, v);
+            }
+            s += '\n';
+        }
+        for (int i = r0; i < r1; i++) fwrite(rows[i - r0].data(), 1, rows[i - r0].size(), f);
+    }
+    if (fclose(f) != 0) {
+        set_global_error(std::string("write failed : ") + path);
+        return CMS_E_IO;
+    }
+    return CMS_OK;
+}
+
+void cms_free(void *p) { free(p); }
+const char *cms_version(void) { return "cms-mi355x 0.1"; }
+
+}  // extern "C"

@@ -1,0 +1,356 @@
+// Shared-minimum kernels: the same similarity as the walk of
+// _Comp_Tree::Calc_sim (comp.h:332-375), from a per-sample embedding.
+//
+// Identity. Let node p of the schedule have children k (leaves or earlier
+// nodes) with weights w_k = 1 - dist_k (MIN_DIST-clamped as in comp.h:343-344).
+// Write x_s(k) for sample s's abundance under k decayed to k
+// (x_s(leaf) = a_s(leaf), x_s(p) = sum_k w_k x_s(k)). The walk's leftovers
+// satisfy RA(k) = x_a(k) - E_k and RB(k) = x_b(k) - E_k with the same E_k, so
+// each shared minimum is min(x_a(k), x_b(k)) - E_k, and E_p = sum_k w_k
+// min(x_a(k), x_b(k)). Summing over the schedule,
+//
+//     100 * sim = sum over child edges e of beta_e * min(x_a(e), x_b(e)),
+//     beta_e = 1 - w_e if the parent's value is read later, 1 for a root.
+//
+// With beta >= 0 (no negative branch lengths) beta * min(x, y) =
+// min(beta x, beta y), so the pair term is a plain histogram intersection of
+// y_s(e) = beta_e x_s(e). Only entries where BOTH samples have mass (the
+// intersection of their root paths) contribute, which is what the tiles below
+// exploit: a tile of 32 x 32 samples visits only the entries set in both
+// blocks' bitmaps.
+//
+// Arithmetic. y is computed in FP64 and stored as fixed point
+// round(y * 2^K) in uint32 with K chosen on the host so that every sample's
+// column sum stays below 2^32 (the column sum equals the sample's total leaf
+// mass, about 100). Pair sums are then exact integer min/add (v_min_u32 +
+// v_add3_u32: 1.5 VALU ops per pair-entry), order-independent and
+// deterministic; the only error is the rounding of y (<= 2^-(K+1) per entry).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace cms {
+
+typedef const __attribute__((address_space(4))) WalkNode *cnode_ptr;
+typedef const __attribute__((address_space(4))) double *cdouble_ptr;
+typedef const __attribute__((address_space(4))) int32_t *cint_ptr;
+
+__device__ __forceinline__ void sm_decode_upper(int64_t t, int64_t nb, int64_t &rb, int64_t &cb) {
+    double nn = (double)nb;
+    double disc = (2.0 * nn + 1.0) * (2.0 * nn + 1.0) - 8.0 * (double)t;
+    int64_t r = (int64_t)(((2.0 * nn + 1.0) - sqrt(disc)) * 0.5);
+    if (r < 0) r = 0;
+    while (r > 0 && r * nb - r * (r - 1) / 2 > t) r--;
+    while ((r + 1) * nb - (r + 1) * r / 2 <= t) r++;
+    rb = r;
+    cb = r + (t - (r * nb - r * (r - 1) / 2));
+}
+
+__device__ __forceinline__ int64_t sm_xcd_remap(int64_t b, int64_t nwg) {
+    const int64_t X = 8;
+    int64_t xcd = b % X, idx = b / X, per = nwg / X, rem = nwg % X;
+    return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+// ------------------------------------------------------------ embedding
+
+__global__ void sample_mass_kernel(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                   const float *__restrict__ val, int n, const int32_t *__restrict__ leaf_uses,
+                                   double *__restrict__ mass) {
+    const int s = blockIdx.x * blockDim.y + threadIdx.y;
+    if (s >= n) return;
+    double m = 0.0;
+    for (int64_t k = rowptr[s] + threadIdx.x; k < rowptr[s + 1]; k += 64) m += (double)val[k] * leaf_uses[col[k]];
+    for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off, 64);
+    if (threadIdx.x == 0) mass[s] = m;
+}
+
+// One wave = 64 consecutive samples (lane = sample). Walks the schedule in
+// FP64, writes Y[row][s] for every active entry and the per-32-sample-block
+// entry bitmaps. Leaf values come from the walk layout, staged per chunk of 64
+// leaf positions.
+constexpr int kEmbedLeafChunk = 64;
+
+__global__ __launch_bounds__(64) void embed_kernel(cnode_ptr nodes, int order_n, int nslots,
+                                                   const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
+                                                   cdouble_ptr ent_beta, cint_ptr ent_row, int64_t nwords,
+                                                   double scale, uint32_t *__restrict__ Y, uint32_t *__restrict__ bm) {
+    extern __shared__ double eslots[];  // nslots * 64
+    __shared__ float stage[kEmbedLeafChunk * 64];
+    const int lane = threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.x * 64 + lane;
+    const bool valid = s < n;
+    const int64_t ab_base = (s >> 3) * nlp * 8 + (s & 7);
+    const int64_t blk0 = (int64_t)blockIdx.x * 2;  // 32-sample blocks of this wave
+    uint32_t bits0 = 0, bits1 = 0;
+    int64_t cur_word = 0;
+    int64_t chunk_lb = -kEmbedLeafChunk;  // leaf positions [chunk_lb, chunk_lb + 64) staged
+
+    for (int i = 0; i < order_n; i++) {
+        const uint32_t code = nodes[i].code;
+        int64_t lp = nodes[i].leafpos;
+        const int nleaf = ((code & kC1Leaf) ? 1 : 0) + ((code & kC2Leaf) ? 1 : 0);
+        if (nleaf && lp + nleaf > chunk_lb + kEmbedLeafChunk) {
+            chunk_lb = lp;
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll 8
+            for (int j = 0; j < kEmbedLeafChunk; j++) {
+                float v = 0.0f;
+                if (valid && chunk_lb + j < nlp) v = ab[ab_base + (chunk_lb + j) * 8];
+                stage[j * 64 + lane] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        double x[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const bool leaf = j == 0 ? (code & kC1Leaf) : (code & kC2Leaf);
+            if (leaf) {
+                x[j] = (double)stage[(lp - chunk_lb) * 64 + lane];
+                lp++;
+            } else {
+                x[j] = eslots[((code >> (8 * j)) & 0xffu) * 64 + lane];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int32_t row = ent_row[2 * (int64_t)i + j];
+            if (row < 0) continue;
+            const double y = ent_beta[2 * (int64_t)i + j] * x[j];
+            const uint32_t q = (uint32_t)rint(y * scale);
+            Y[(int64_t)row * npad + s] = q;
+            const uint64_t m = __ballot(q != 0u);
+            const int64_t w = row >> 5;
+            if (w != cur_word) {
+                if (lane == 0) {
+                    bm[blk0 * nwords + cur_word] = bits0;
+                    bm[(blk0 + 1) * nwords + cur_word] = bits1;
+                }
+                bits0 = bits1 = 0;
+                cur_word = w;
+            }
+            if ((uint32_t)m) bits0 |= 1u << (row & 31);
+            if ((uint32_t)(m >> 32)) bits1 |= 1u << (row & 31);
+        }
+        if (code & kDstLive) {
+            const double xp = (double)nodes[i].w1 * x[0] + (double)nodes[i].w2 * x[1];
+            eslots[((code >> 16) & 0xffu) * 64 + lane] = xp;
+        }
+    }
+    if (lane == 0 && cur_word < nwords) {
+        bm[blk0 * nwords + cur_word] = bits0;
+        bm[(blk0 + 1) * nwords + cur_word] = bits1;
+    }
+}
+
+// ------------------------------------------------------------ pair tiles
+
+// One wave = 32 x 32 pairs, lane (ty, tx) = 4 rows x 4 columns.
+// MODE 0: all pairs (aligned upper-triangle block tiles, packed output).
+// MODE 1: rectangle with arbitrary sample offsets (row-major, diagonal = 1).
+template <int MODE>
+__global__ __launch_bounds__(64 * kSmWaves) void sm_tile_kernel(SmData d, int64_t ntiles, int64_t nb, int r0, int nr,
+                                                                 int c0, int nc, float *__restrict__ out) {
+    __shared__ uint16_t s_list[kSmWaves][64 * 32];
+    __shared__ uint32_t s_stage[kSmWaves][kSmChunk * 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t tile = sm_xcd_remap(blockIdx.x, gridDim.x) * kSmWaves + wave;
+    if (tile >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
+    int64_t abase, bbase, alim, blim;
+    const uint32_t *A0, *A1 = nullptr, *B0, *B1 = nullptr;
+    const int64_t nblk = d.npad / kSmBlock;
+    if constexpr (MODE == 0) {
+        int64_t ra, cb;
+        sm_decode_upper(tile, nb, ra, cb);
+        abase = ra * kSmBlock;
+        bbase = cb * kSmBlock;
+        alim = blim = d.n;
+        A0 = d.bm + ra * d.nwords;
+        B0 = d.bm + cb * d.nwords;
+    } else {
+        const int64_t tr = tile / nb, tc = tile % nb;
+        abase = r0 + tr * kSmBlock;
+        bbase = c0 + tc * kSmBlock;
+        alim = r0 + nr;
+        blim = c0 + nc;
+        const int64_t ba = abase / kSmBlock, bb = bbase / kSmBlock;
+        A0 = d.bm + ba * d.nwords;
+        if ((abase % kSmBlock) && ba + 1 < nblk) A1 = d.bm + (ba + 1) * d.nwords;
+        B0 = d.bm + bb * d.nwords;
+        if ((bbase % kSmBlock) && bb + 1 < nblk) B1 = d.bm + (bb + 1) * d.nwords;
+    }
+    const int ty = lane >> 3, tx = lane & 7;
+    // Column this lane stages: lanes 0..31 the row block, 32..63 the column block.
+    const int64_t my_s = (lane < 32) ? abase + lane : bbase + (lane - 32);
+    const bool s_ok = my_s < d.npad;
+    const uint32_t *Ycol = d.Y + (s_ok ? my_s : 0);
+
+    uint32_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0u;
+
+    uint16_t *L = s_list[wave];
+    uint32_t *S = s_stage[wave];
+    uint32_t visited = 0;
+    for (int64_t wb = 0; wb < d.nwords; wb += 64) {
+        const int64_t wi = wb + lane;
+        uint32_t w = 0u;
+        if (wi < d.nwords) {
+            uint32_t a = A0[wi], b = B0[wi];
+            if constexpr (MODE == 1) {
+                if (A1) a |= A1[wi];
+                if (B1) b |= B1[wi];
+            }
+            w = a & b;
+        }
+        const int cnt = __popc(w);
+        int incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        const int tot = __shfl(incl, 63, 64);
+        if (tot == 0) continue;
+        visited += (uint32_t)tot;
+        int pos = incl - cnt;
+        __builtin_amdgcn_wave_barrier();
+        while (w) {
+            const int b = __ffs(w) - 1;
+            L[pos++] = (uint16_t)(lane * 32 + b);
+            w &= w - 1u;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t *Ywb = Ycol + wb * 32 * d.npad;
+        for (int j = 0; j < tot; j += kSmChunk) {
+            uint32_t v[kSmChunk];
+#pragma unroll
+            for (int k = 0; k < kSmChunk; k++) {
+                v[k] = 0u;
+                if (j + k < tot && s_ok) v[k] = Ywb[(int64_t)L[j + k] * d.npad];
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int k = 0; k < kSmChunk; k++) S[k * 64 + lane] = v[k];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int kn = min(kSmChunk, tot - j);
+            for (int k = 0; k < kn; k += 2) {
+                const uint4 a0 = *reinterpret_cast<const uint4 *>(S + k * 64 + ty * 4);
+                const uint4 b0 = *reinterpret_cast<const uint4 *>(S + k * 64 + 32 + tx * 4);
+                const uint4 a1 = *reinterpret_cast<const uint4 *>(S + (k + 1) * 64 + ty * 4);
+                const uint4 b1 = *reinterpret_cast<const uint4 *>(S + (k + 1) * 64 + 32 + tx * 4);
+                const uint32_t av0[4] = {a0.x, a0.y, a0.z, a0.w}, bv0[4] = {b0.x, b0.y, b0.z, b0.w};
+                const uint32_t av1[4] = {a1.x, a1.y, a1.z, a1.w}, bv1[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++)
+                        acc[ii][jj] = acc[ii][jj] + min(av0[ii], bv0[jj]) + min(av1[ii], bv1[jj]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (d.counter && lane == 0) atomicAdd(d.counter, (unsigned long long)visited);
+#pragma unroll
+    for (int ii = 0; ii < 4; ii++) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            const int64_t row = abase + ty * 4 + ii, col = bbase + tx * 4 + jj;
+            double t = (double)acc[ii][jj] * d.inv_scale;
+            t = t > 1.0 ? 1.0 : t;
+            const float sim = (float)t;
+            if constexpr (MODE == 0) {
+                if (row < col && col < d.n) out[row * (int64_t)d.n + col - (row + 1) * (row + 2) / 2] = sim;
+            } else {
+                if (row < alim && col < blim) out[(row - r0) * (int64_t)nc + (col - c0)] = (row == col) ? 1.0f : sim;
+            }
+        }
+    }
+}
+
+// Explicit pair list: lane = pair; 32-entry groups skipped when no lane's two
+// samples share mass in them.
+__global__ __launch_bounds__(256) void sm_pairs_kernel(SmData d, int64_t n_pairs, const int32_t *__restrict__ prow,
+                                                       const int32_t *__restrict__ pcol, float *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = i < n_pairs;
+    const int64_t a = ok ? prow[i] : 0, b = ok ? pcol[i] : 0;
+    const uint32_t *BA = d.bm + (a / kSmBlock) * d.nwords, *BB = d.bm + (b / kSmBlock) * d.nwords;
+    uint32_t acc = 0u;
+    for (int64_t g = 0; g < d.nwords; g++) {
+        uint32_t m = ok ? (BA[g] & BB[g]) : 0u;
+        if (__ballot(m != 0u) == 0) continue;
+        while (m) {
+            const int bit = __ffs(m) - 1;
+            const int64_t r = g * 32 + bit;
+            acc += min(d.Y[r * d.npad + a], d.Y[r * d.npad + b]);
+            m &= m - 1u;
+        }
+    }
+    double t = (double)acc * d.inv_scale;
+    t = t > 1.0 ? 1.0 : t;
+    if (ok) out[i] = (float)t;
+}
+
+// ------------------------------------------------------------ launchers
+
+__global__ void zero_counter_kernel(unsigned long long *c) { *c = 0ull; }
+
+hipError_t launch_zero_counter(unsigned long long *counter, hipStream_t st) {
+    zero_counter_kernel<<<1, 1, 0, st>>>(counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const float *val, int n,
+                              const int32_t *leaf_uses, double *mass, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    sample_mass_kernel<<<(n + 3) / 4, dim3(64, 4), 0, st>>>(rowptr, col, val, n, leaf_uses, mass);
+    return hipGetLastError();
+}
+
+hipError_t launch_embed(const WalkNode *nodes, int order_n, int nslots, const float *ab, int64_t nlp, int n,
+                        int64_t npad, const double *ent_beta, const int32_t *ent_row, int64_t nrows, int64_t nwords,
+                        double scale, uint32_t *Y, uint32_t *bm, hipStream_t st) {
+    (void)nrows;
+    if (n <= 0 || order_n <= 0) return hipSuccess;
+    const size_t lds = (size_t)(nslots > 0 ? nslots : 1) * 64 * sizeof(double);
+    embed_kernel<<<(unsigned)(npad / 64), 64, lds, st>>>((cnode_ptr)nodes, order_n, nslots, ab, nlp, n, npad,
+                                                         (cdouble_ptr)ent_beta, (cint_ptr)ent_row, nwords, scale, Y, bm);
+    return hipGetLastError();
+}
+
+hipError_t launch_sm_all_pairs(const SmData &d, float *out, hipStream_t st) {
+    const int64_t nb = (d.n + kSmBlock - 1) / kSmBlock;
+    const int64_t ntiles = nb * (nb + 1) / 2;
+    if (ntiles <= 0) return hipSuccess;
+    const int64_t nwg = (ntiles + kSmWaves - 1) / kSmWaves;
+    sm_tile_kernel<0><<<dim3((unsigned)nwg), 64 * kSmWaves, 0, st>>>(d, ntiles, nb, 0, 0, 0, 0, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sm_rect(const SmData &d, int r0, int nr, int c0, int nc, float *out, hipStream_t st) {
+    const int64_t nbr = (nr + kSmBlock - 1) / kSmBlock, nbc = (nc + kSmBlock - 1) / kSmBlock;
+    const int64_t ntiles = nbr * nbc;
+    if (ntiles <= 0) return hipSuccess;
+    const int64_t nwg = (ntiles + kSmWaves - 1) / kSmWaves;
+    sm_tile_kernel<1><<<dim3((unsigned)nwg), 64 * kSmWaves, 0, st>>>(d, ntiles, nbc, r0, nr, c0, nc, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sm_pairs(const SmData &d, int64_t n_pairs, const int32_t *row, const int32_t *col, float *out,
+                           hipStream_t st) {
+    if (n_pairs <= 0) return hipSuccess;
+    sm_pairs_kernel<<<dim3((unsigned)((n_pairs + 255) / 256)), 256, 0, st>>>(d, n_pairs, row, col, out);
+    return hipGetLastError();
+}
+
+}  // namespace cms

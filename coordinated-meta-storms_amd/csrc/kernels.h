@@ -1,0 +1,94 @@
+// Launch wrappers of the gfx950 kernels (host side). Device pointers only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cms_internal.h"
+
+namespace cms {
+
+// Samples per walk block: a wave walks an 8 x 8 tile of pairs (lane = 8 r + c).
+constexpr int kWalkBlock = 8;
+// Leaf-stream positions staged in LDS per chunk.
+constexpr int kWalkLeafChunk = 32;
+// Waves (tiles) per walk workgroup.
+constexpr int kWalkWaves = 4;
+
+// Layout of walk abundances: for sample block b (samples 8b..8b+7) and leaf
+// stream position lp, value of sample 8b+k at ab[(b * nlp + lp) * 8 + k].
+// Stored as float (F32) or as 16-bit words with a per-sample power-of-two
+// scale (BF16 / F16).
+struct WalkData {
+    const WalkNode *nodes;
+    const WalkChunk *chunks;
+    int nchunks;
+    int order_n;
+    int nslots;
+    int64_t nlp;        // leaf stream length
+    const void *ab;     // block-leaf-major values
+    const float *inv_scale;  // per-sample 2^-e (16-bit storage), may be null for F32
+    int n;              // samples
+    int storage;        // CMS_STORE_*
+};
+
+// Scatter CSR samples into the walk layout (buffer zeroed by the caller).
+//   leaf_lp_ptr/leaf_lp: leaf -> leaf-stream positions (CSR over leaves)
+hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const float *val, int n,
+                               const int32_t *leaf_lp_ptr, const int32_t *leaf_lp, int64_t nlp,
+                               int storage, const float *scale, void *ab, hipStream_t st);
+// Per-sample power-of-two scale for 16-bit storage from CSR values.
+hipError_t launch_sample_scale(const int64_t *rowptr, const float *val, int n, int storage, float *scale,
+                               float *inv_scale, hipStream_t st);
+
+// All pairs of samples, packed upper triangle.
+hipError_t launch_walk_all_pairs(const WalkData &d, float *out, hipStream_t st);
+// Rectangle rows [r0, r0+nr) x cols [c0, c0+nc), row-major nr x nc.
+hipError_t launch_walk_rect(const WalkData &d, int r0, int nr, int c0, int nc, float *out, hipStream_t st);
+// Explicit pair list.
+hipError_t launch_walk_pairs(const WalkData &d, int64_t n_pairs, const int32_t *row, const int32_t *col,
+                             float *out, hipStream_t st);
+
+// ---------------------------------------------------------- shared-minimum form
+//
+// 100 * sim(a, b) = sum over child edges e of min(y_a(e), y_b(e)), with
+// y_s(e) = beta_e * x_s(e) >= 0 (see kern_sharedmin.hip). y is stored as
+// fixed point, Y[row][sample] = round(y * 2^K) (uint32), rows = active
+// entries in schedule order, samples padded to a multiple of the block size.
+
+constexpr int kSmBlock = 32;   // samples per block (tile = 32 x 32 pairs per wave)
+constexpr int kSmWaves = 4;    // tiles per workgroup
+constexpr int kSmChunk = 16;   // rows staged per LDS chunk
+
+struct SmData {
+    const uint32_t *Y;      // nrows x npad
+    const uint32_t *bm;     // nblk x nwords bitmaps: bit r of block b = any Y[r][b*32..] > 0
+    int64_t nrows;
+    int64_t npad;
+    int64_t nwords;
+    int n;                  // samples
+    double inv_scale;       // 1 / (100 * 2^K)
+    unsigned long long *counter;  // optional: entries visited (summed over tiles)
+};
+
+hipError_t launch_zero_counter(unsigned long long *counter, hipStream_t st);
+
+// Per-sample sum of leaf values over consumed leaf positions (FP64).
+hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const float *val, int n,
+                              const int32_t *leaf_uses, double *mass, hipStream_t st);
+// Embedding y per (row, sample) into Y (fixed point) and the block bitmaps.
+//   ab: F32 walk layout (block-leaf-major) of the same samples
+//   ent_row[e]: row of entry e in Y or -1 (inactive)
+hipError_t launch_embed(const WalkNode *nodes, int order_n, int nslots, const float *ab, int64_t nlp, int n,
+                        int64_t npad, const double *ent_beta, const int32_t *ent_row, int64_t nrows, int64_t nwords,
+                        double scale, uint32_t *Y, uint32_t *bm, hipStream_t st);
+// All pairs, packed upper triangle.
+hipError_t launch_sm_all_pairs(const SmData &d, float *out, hipStream_t st);
+// Rectangle over sample ranges aligned to nothing: rows [r0, r0+nr) x cols [c0, c0+nc).
+hipError_t launch_sm_rect(const SmData &d, int r0, int nr, int c0, int nc, float *out, hipStream_t st);
+// Explicit pair list.
+hipError_t launch_sm_pairs(const SmData &d, int64_t n_pairs, const int32_t *row, const int32_t *col, float *out,
+                           hipStream_t st);
+
+}  // namespace cms

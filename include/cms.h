@@ -1,0 +1,247 @@
+/*
+ * cms.h — C ABI of the MI355X-native all-pairs Meta-Storms distance library
+ * (libcms.so).
+ *
+ * The reference has no library boundary: its GPU path is a pair of CUDA kernels
+ * launched from a host loop, and its CPU path is a member function. Each entry
+ * point below names the reference interface it replaces (file:line in
+ * qdu-bioinfo/Coordinated-Meta-Storms).
+ *
+ * Conventions
+ *   - Every function returns 0 (CMS_OK) on success or a negative CMS_E_* code.
+ *     Nothing in the library calls exit(); cms_last_error() gives the message
+ *     of the last failure on a context (or of the last context-less failure
+ *     when passed NULL).
+ *   - Host buffers are owned by the caller; the library copies what it needs.
+ *     Device memory is owned by the library (or by the caller for *_device
+ *     entry points, which take device pointers).
+ *   - One context per GPU. A context is not thread-safe; callers serialise.
+ *   - Similarities are float in [0, 1]; the packed upper triangle uses the
+ *     reference's index p = m*N + n - (m+1)(m+2)/2 for m < n
+ *     (cms_cuda.cu:1801-1811, comp_sam.cpp:346).
+ */
+#ifndef CMS_H
+#define CMS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMS_REG_SIZE 150 /* register-slot space of order.txt (comp.h:26) */
+
+enum {
+    CMS_OK = 0,
+    CMS_E_INVAL = -1,      /* bad argument */
+    CMS_E_NOMEM = -2,      /* host or device allocation failed */
+    CMS_E_HIP = -3,        /* HIP runtime error (no device, launch failure) */
+    CMS_E_SCHEDULE = -4,   /* order.txt schedule invalid (slot out of range, read before write, leaf out of range) */
+    CMS_E_STATE = -5,      /* call out of order (no tree / no samples loaded) */
+    CMS_E_IO = -6,         /* file could not be opened or parsed */
+    CMS_E_UNSUPPORTED = -7 /* valid input that the selected kernel does not handle */
+};
+
+/* Kernel families. */
+enum {
+    CMS_KERNEL_WALK = 0,     /* drop-in: per-pair post-order walk, FP32, bit-exact with comp -M 0 */
+    CMS_KERNEL_SHAREDMIN = 1 /* per-sample decayed-mass embedding + fixed-point shared-minimum tiles */
+};
+
+/* Abundance storage of the walk kernel (arithmetic is always FP32). */
+enum {
+    CMS_STORE_F32 = 0,
+    CMS_STORE_BF16 = 1, /* per-sample power-of-two scale, Kahan-compensated total */
+    CMS_STORE_F16 = 2
+};
+
+typedef struct cms_opts {
+    int kernel;       /* CMS_KERNEL_* (default WALK) */
+    int storage;      /* CMS_STORE_* (default F32; WALK only) */
+    int tile_a;       /* SHAREDMIN row-block size (0 = autotuned/default) */
+    int tile_b;       /* SHAREDMIN column-block size (0 = default) */
+    int chunk;        /* entries staged per LDS chunk (0 = default) */
+    int use_graph;    /* capture per-band tile launches in a hipGraph (SHAREDMIN) */
+    int reserved[8];
+} cms_opts;
+
+typedef struct cms_ctx cms_ctx;
+
+/* ---------------------------------------------------------------- context */
+
+void cms_opts_default(cms_opts *o);
+
+/* Replaces the per-GPU setup of Multi_GPU_split_MetaStorm (cms_cuda.cu:1470-1652). */
+int cms_create(int device, const cms_opts *o, cms_ctx **out);
+void cms_destroy(cms_ctx *c);
+const char *cms_last_error(const cms_ctx *c);
+int cms_set_opts(cms_ctx *c, const cms_opts *o);
+
+/* Tree schedule, arrays exactly as loaded from tree/order.txt
+ * (_Comp_Tree::Load_order, comp.h:161-192; getters comp.h:62-84).
+ * Copied; validated (slot range, read-before-write, leaf range). */
+int cms_set_tree(cms_ctx *c, int leaf_n, int order_n,
+                 const int32_t *order_1, const float *dist_1,
+                 const int32_t *order_2, const float *dist_2,
+                 const int32_t *order_d);
+
+/* Dense abundances, n x leaf_n row-major, percent scale as produced by
+ * _Comp_Tree::Load_abd (comp.h:243-279). Replaces the per-sample device
+ * rows of cms_cuda.cu:1617-1652. */
+int cms_set_samples(cms_ctx *c, int n, const float *abd);
+/* Same data in CSR form: row s holds (leaf, value) for its nonzero leaves. */
+int cms_set_samples_csr(cms_ctx *c, int n, const int64_t *rowptr,
+                        const int32_t *leaf, const float *val);
+
+/* All N(N-1)/2 similarities into a caller-owned host buffer (packed upper
+ * triangle). Replaces the tile loop of Multi_GPU_split_MetaStorm
+ * (cms_cuda.cu:1523-2196) and the CPU loop of Multi_Comp_Table
+ * (comp_sam.cpp:338-349). */
+int cms_all_pairs(cms_ctx *c, float *sim_packed);
+/* Same, into a device buffer of N(N-1)/2 floats, enqueued on `stream`
+ * (hipStream_t, NULL = the context's stream). No host synchronisation. */
+int cms_all_pairs_device(cms_ctx *c, float *d_sim_packed, void *stream);
+
+/* Explicit pair list: sim[i] = sim(row[i], col[i]). Host arrays. The
+ * reference kernels' boundary (CalcSimOfDiagonalSquare cms_cuda.cu:52-54 and
+ * CalcSimOfNormalRectangle cms_cuda.cu:495-497 with d_order_row/d_order_col). */
+int cms_sim_pairs(cms_ctx *c, int64_t n_pairs, const int32_t *row,
+                  const int32_t *col, float *sim);
+
+/* Rectangular tile rows [r0, r0+nr) x cols [c0, c0+nc) into a host buffer of
+ * nr*nc floats, row-major. Diagonal entries (same sample) are 1. Replaces one
+ * iteration of the tile loop (cms_cuda.cu:1523-1539, defineOrder 961-985). */
+int cms_tile(cms_ctx *c, int r0, int nr, int c0, int nc, float *out);
+
+/* Rebuild the device layouts of the selected kernel from the uploaded
+ * samples (force != 0: even if current), reusing device allocations: the
+ * block-leaf-major walk layout, and for CMS_KERNEL_SHAREDMIN the fixed-point
+ * embedding and its block bitmaps. Runs implicitly on first use; a caller
+ * timing the whole path from resident inputs calls it once per pass. */
+int cms_prepare(cms_ctx *c, int force);
+
+/* Diagnostics of the last all-pairs run (waits for its kernel to finish). */
+typedef struct cms_stats {
+    double kernel_ms;          /* device time of the pair kernels (HIP events) */
+    double prep_ms;            /* device time of layout / embedding kernels */
+    double pair_node_updates;  /* node-steps executed (pruned count for SHAREDMIN) */
+    double dense_updates;      /* pairs x OrderN */
+    double valu_ops;           /* algorithmic VALU lane-ops executed */
+    double bytes_moved;        /* algorithmic HBM bytes of the pair kernels */
+    int64_t tiles;             /* tiles launched */
+    int nslots;                /* register slots after re-allocation */
+    int scale_bits;            /* SHAREDMIN fixed-point scale (2^scale_bits per percent) */
+    int entries;               /* SHAREDMIN active embedding entries */
+} cms_stats;
+int cms_get_stats(cms_ctx *c, cms_stats *s);
+
+/* ------------------------------------------------- host loader (no GPU) */
+
+/* Validate a schedule without a device (the checks of cms_set_tree):
+ * register range, read-before-write, leaf range. Outputs (optional): live
+ * register slots after re-allocation, whether it is a tree (every leaf and
+ * value read at most once), whether all 1 - w >= 0. */
+int cms_validate_tree(int leaf_n, int order_n, const int32_t *order_1, const float *dist_1,
+                      const int32_t *order_2, const float *dist_2, const int32_t *order_d,
+                      int *nslots, int *is_tree, int *nonneg_beta);
+
+/* Tree database as configured by $ParallelMETA/databases/db.config
+ * (_PMDB, db.h:21-194, 287-358; _Comp_Tree::Init comp.h:122-137). */
+typedef struct cms_db cms_db;
+
+/* root = the ParallelMETA directory (NULL: read the environment variable,
+ * utility.h:27-41). db_letter as for -D; unknown letters fall back to 'G'
+ * (db.h:44). */
+int cms_db_open(const char *root, char db_letter, cms_db **out);
+void cms_db_close(cms_db *db);
+int cms_db_leaf_n(const cms_db *db);
+int cms_db_order_n(const cms_db *db);
+int cms_db_has_tree(const cms_db *db);
+int cms_db_has_cp(const cms_db *db);
+/* Pointers into the database (valid until cms_db_close). */
+int cms_db_order(const cms_db *db, const int32_t **order_1, const float **dist_1,
+                 const int32_t **order_2, const float **dist_2,
+                 const int32_t **order_d);
+const char *cms_db_leaf_name(const cms_db *db, int i);
+
+/* OTU count table (-T), _Table_Format::Load_Table (table_format.h:110-165):
+ * header = title + feature ids, rows = sample id + exactly F numbers; rows of
+ * the wrong width are skipped. */
+typedef struct cms_table cms_table;
+int cms_table_load(const char *path, cms_table **out);
+/* Build a table from memory: n samples, f features, dense counts n x f. */
+int cms_table_from_dense(int n, int f, const char *const *sample_names,
+                         const char *const *feature_names, const float *counts,
+                         cms_table **out);
+void cms_table_free(cms_table *t);
+int cms_table_n(const cms_table *t);
+int cms_table_f(const cms_table *t);
+const char *cms_table_sample_name(const cms_table *t, int i);
+const char *cms_table_feature_name(const cms_table *t, int j);
+int cms_table_skipped(const cms_table *t); /* rows rejected by the width check */
+
+/* Dense percent-scale abundances n x leaf_n, _Comp_Tree::Load_abd
+ * (comp.h:243-279) with Check_OTU (utility.h:181-190) and copy-number
+ * correction (db.h:251-280). mapped[i] (optional) = mapped OTU count. */
+int cms_table_abundance(const cms_db *db, const cms_table *t, int cp_correct,
+                        float *abd, int *mapped);
+
+/* .dist writer, Output_Matrix (cms_cuda.cu:1305-1342 = comp_sam.cpp:184-219):
+ * header "\t"+names; row i = name then "\t"+value per column, value
+ * = 1.0 - (double)sim (0 on the diagonal), or sim (1 on the diagonal) when
+ * is_sim; values printed like std::ostream default (6 significant digits). */
+int cms_write_dist(const char *path, int n, const char *const *names,
+                   const float *sim_packed, int is_sim);
+
+/* ------------------------------------------- synthetic inputs (no GPU) */
+
+/* Seeded synthetic tree: random binary tree (Yule process), branch lengths
+ * ~ Exponential(bl_mean) clipped to [0, bl_max] and rounded to 6 decimals,
+ * order.txt emitted in Sethi-Ullman post-order with lowest-free slot
+ * allocation. Arrays sized leaf_n - 1 (order) are written by the callee into
+ * caller buffers; leaf names are "T<index>". */
+int cms_synth_tree(uint64_t seed, int leaf_n, double bl_mean, double bl_max,
+                   int32_t *order_1, float *dist_1, int32_t *order_2,
+                   float *dist_2, int32_t *order_d, int *max_slots);
+
+typedef struct cms_synth_samples_opts {
+    uint64_t seed;
+    int n_samples;
+    int leaf_n;
+    int nnz_lo, nnz_hi;   /* nonzero OTUs per sample */
+    int nnz_log_uniform;  /* 0: uniform integer in [lo, hi]; 1: log-uniform */
+    double zipf_s;        /* OTU popularity exponent over a random rank order */
+    double nb_mean;       /* NegBinomial mean of counts (Gamma-Poisson) */
+    double nb_dispersion; /* NegBinomial dispersion r (shape of the Gamma) */
+} cms_synth_samples_opts;
+
+/* Counts in CSR form, columns ascending. rowptr has n+1 entries; col/cnt
+ * are allocated by the library (free with cms_free). */
+int cms_synth_samples(const cms_synth_samples_opts *o, int64_t **rowptr,
+                      int32_t **col, float **cnt);
+/* Copy numbers uniform integer in [1, cp_max] per leaf (seeded). */
+int cms_synth_copy_numbers(uint64_t seed, int leaf_n, int cp_max, float *cp);
+/* Percent-scale abundances from CSR counts with the Load_abd arithmetic
+ * (comp.h:252-275): x = count / cp, FP32 total in column order, a = x / total.
+ * cp may be NULL (no correction). Output CSR values in place of counts. */
+int cms_counts_to_abundance_csr(int n, const int64_t *rowptr, const int32_t *col,
+                                const float *cnt, const float *cp, float *abd_val);
+/* Write a ParallelMETA-style database directory: root/databases/db.config
+ * (entry 'G' -> name), root/databases/name/tree/{id,order}.txt and
+ * copy_number.txt (if cp != NULL). */
+int cms_write_db(const char *root, const char *name, int leaf_n, int order_n,
+                 const int32_t *order_1, const float *dist_1,
+                 const int32_t *order_2, const float *dist_2,
+                 const int32_t *order_d, const float *cp);
+/* Write a dense OTU count table (text) from CSR counts over leaf_n columns. */
+int cms_write_table(const char *path, int n, int leaf_n, const int64_t *rowptr,
+                    const int32_t *col, const float *cnt);
+
+void cms_free(void *p);
+const char *cms_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMS_H */

@@ -1,0 +1,256 @@
+/*
+ * cms_oracle.c — CPU oracle of the Meta-Storms all-pairs path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker: tests, the smoke check
+ * and the CPU-baseline leg of the benchmark call it; the product library never
+ * links, loads or calls it.
+ *
+ * Restates, in plain C:
+ *   orc_calc_sim_f32   _Comp_Tree::Calc_sim (comp.h:332-375), FP32, same op
+ *                      order; built with -ffp-contract=off so every operation
+ *                      rounds to float as in the x86-64 reference build.
+ *   orc_calc_sim_f64   the same traversal in FP64 (the FP64 oracle).
+ *   orc_sharedmin_f64  an independent FP64 closed form of the same value:
+ *                      100*sim = sum over child edges e of beta_e *
+ *                      min(x_a(e), x_b(e)), where x_s(e) is sample s's mass
+ *                      under the child, decayed by w along each edge below it,
+ *                      and beta_e = 1 - w_e if the parent's value is consumed,
+ *                      1 if the parent is a root. Valid for tree schedules.
+ *   orc_load_abd       _Comp_Tree::Load_abd over a table row (comp.h:243-279)
+ *                      with the column->leaf map and copy numbers resolved.
+ *   orc_all_pairs_f32  Multi_Comp_Table's pair loop (comp_sam.cpp:321-349),
+ *                      OpenMP dynamic,1, packed upper triangle.
+ *
+ * Pinning: orc_calc_sim_f32 is checked bit-exactly against the compiled
+ * reference (golden fixtures); orc_calc_sim_f64 and orc_sharedmin_f64 are two
+ * different algorithms that must agree to 1e-12; the 4-leaf tree of the
+ * known-answer test fixes all three by hand values.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_REG_SIZE 150
+#define ORC_MIN_DIST 0.00001
+
+/* comp.h:332-375 */
+float orc_calc_sim_f32(int order_n, const int32_t *o1, const float *d1, const int32_t *o2,
+                       const float *d2, const int32_t *od, const float *A, const float *B) {
+    float R1[ORC_REG_SIZE], R2[ORC_REG_SIZE];
+    float total = 0;
+    memset(R1, 0, sizeof R1);
+    memset(R2, 0, sizeof R2);
+    for (int i = 0; i < order_n; i++) {
+        int c1 = o1[i], c2 = o2[i], dst = od[i] + ORC_REG_SIZE;
+        float w1 = 1 - d1[i];
+        float w2 = 1 - d2[i];
+        w1 = (w1 < 0) ? (float)ORC_MIN_DIST : w1;
+        w2 = (w2 < 0) ? (float)ORC_MIN_DIST : w2;
+        float a1, b1, a2, b2;
+        if (c1 >= 0) { a1 = A[c1]; b1 = B[c1]; }
+        else { a1 = R1[c1 + ORC_REG_SIZE]; b1 = R2[c1 + ORC_REG_SIZE]; }
+        if (c2 >= 0) { a2 = A[c2]; b2 = B[c2]; }
+        else { a2 = R1[c2 + ORC_REG_SIZE]; b2 = R2[c2 + ORC_REG_SIZE]; }
+        float m1 = (a1 < b1) ? a1 : b1;
+        float m2 = (a2 < b2) ? a2 : b2;
+        total += m1;
+        total += m2;
+        R1[dst] = (a1 - m1) * w1 + (a2 - m2) * w2;
+        R2[dst] = (b1 - m1) * w1 + (b2 - m2) * w2;
+    }
+    total /= 100.0;
+    total = (total > 1.0) ? 1 : total;
+    total = (total < 0.0) ? 0 : total;
+    return total;
+}
+
+/* The same traversal in FP64 (inputs are the FP32 abundances and weights). */
+double orc_calc_sim_f64(int order_n, const int32_t *o1, const float *d1, const int32_t *o2,
+                        const float *d2, const int32_t *od, const float *A, const float *B) {
+    double R1[ORC_REG_SIZE], R2[ORC_REG_SIZE];
+    double total = 0;
+    memset(R1, 0, sizeof R1);
+    memset(R2, 0, sizeof R2);
+    for (int i = 0; i < order_n; i++) {
+        int c1 = o1[i], c2 = o2[i], dst = od[i] + ORC_REG_SIZE;
+        float w1f = 1 - d1[i], w2f = 1 - d2[i];
+        w1f = (w1f < 0) ? (float)ORC_MIN_DIST : w1f;
+        w2f = (w2f < 0) ? (float)ORC_MIN_DIST : w2f;
+        double w1 = w1f, w2 = w2f;
+        double a1, b1, a2, b2;
+        if (c1 >= 0) { a1 = A[c1]; b1 = B[c1]; }
+        else { a1 = R1[c1 + ORC_REG_SIZE]; b1 = R2[c1 + ORC_REG_SIZE]; }
+        if (c2 >= 0) { a2 = A[c2]; b2 = B[c2]; }
+        else { a2 = R1[c2 + ORC_REG_SIZE]; b2 = R2[c2 + ORC_REG_SIZE]; }
+        double m1 = (a1 < b1) ? a1 : b1;
+        double m2 = (a2 < b2) ? a2 : b2;
+        total += m1;
+        total += m2;
+        R1[dst] = (a1 - m1) * w1 + (a2 - m2) * w2;
+        R2[dst] = (b1 - m1) * w1 + (b2 - m2) * w2;
+    }
+    total /= 100.0;
+    if (total > 1.0) total = 1.0;
+    if (total < 0.0) total = 0.0;
+    return total;
+}
+
+/*
+ * Closed form. Pass 1 finds, for every node, whether its written value is read
+ * later (consumed) and, for every register read, which node produced it.
+ * Pass 2 computes x for both samples per node and accumulates
+ * beta * min(x_a, x_b) per child edge. Returns -1 for schedules that are not
+ * trees (a value read twice), which the identity does not cover.
+ */
+double orc_sharedmin_f64(int order_n, const int32_t *o1, const float *d1, const int32_t *o2,
+                         const float *d2, const int32_t *od, const float *A, const float *B) {
+    int cur[ORC_REG_SIZE];
+    for (int k = 0; k < ORC_REG_SIZE; k++) cur[k] = -1;
+    int *reads = (int *)calloc((size_t)order_n + 1, sizeof(int));
+    int *prod = (int *)malloc(sizeof(int) * (2 * (size_t)order_n + 1));
+    double *xa = (double *)malloc(sizeof(double) * ((size_t)order_n + 1));
+    double *xb = (double *)malloc(sizeof(double) * ((size_t)order_n + 1));
+    double total = 0;
+    int ok = 1;
+    for (int i = 0; i < order_n && ok; i++) {
+        const int32_t ch[2] = {o1[i], o2[i]};
+        for (int j = 0; j < 2; j++) {
+            prod[2 * i + j] = -1;
+            if (ch[j] < 0) {
+                int v = cur[ch[j] + ORC_REG_SIZE];
+                if (v < 0) { ok = 0; break; }
+                prod[2 * i + j] = v;
+                if (++reads[v] > 1) ok = 0;
+            }
+        }
+        cur[od[i] + ORC_REG_SIZE] = i;
+    }
+    if (ok) {
+        for (int i = 0; i < order_n; i++) {
+            const int32_t ch[2] = {o1[i], o2[i]};
+            const float dd[2] = {d1[i], d2[i]};
+            double sa = 0, sb = 0;
+            for (int j = 0; j < 2; j++) {
+                float wf = 1 - dd[j];
+                wf = (wf < 0) ? (float)ORC_MIN_DIST : wf;
+                double w = wf;
+                double ca, cb;
+                if (ch[j] >= 0) { ca = A[ch[j]]; cb = B[ch[j]]; }
+                else { ca = xa[prod[2 * i + j]]; cb = xb[prod[2 * i + j]]; }
+                double beta = reads[i] > 0 ? 1.0 - w : 1.0;
+                total += beta * (ca < cb ? ca : cb);
+                sa += w * ca;
+                sb += w * cb;
+            }
+            xa[i] = sa;
+            xb[i] = sb;
+        }
+    }
+    free(reads);
+    free(prod);
+    free(xa);
+    free(xb);
+    if (!ok) return -1.0;
+    total /= 100.0;
+    if (total > 1.0) total = 1.0;
+    if (total < 0.0) total = 0.0;
+    return total;
+}
+
+/*
+ * comp.h:243-279 for one table row. col_leaf[j] = tree leaf of column j after
+ * Check_OTU (or -1), col_cp[j] = copy number used for column j (1 when not
+ * corrected or absent). Returns the mapped OTU count.
+ */
+int orc_load_abd(int n_feat, const float *row, const int32_t *col_leaf, const float *col_cp, int leaf_n,
+                 float *abd) {
+    memset(abd, 0, sizeof(float) * (size_t)leaf_n);
+    char *seen = (char *)calloc((size_t)leaf_n + 1, 1);
+    int32_t *touched = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n_feat + 1));
+    int nt = 0, mapped = 0;
+    float total = 0;
+    for (int j = 0; j < n_feat; j++) {
+        if (row[j] > 0 && col_leaf[j] >= 0) {
+            float x = row[j] / col_cp[j];
+            abd[col_leaf[j]] = x;
+            touched[nt++] = col_leaf[j];
+            total += x;
+        }
+    }
+    total /= 100.0;
+    for (int k = 0; k < nt; k++) {
+        int32_t l = touched[k];
+        if (seen[l]) continue;
+        seen[l] = 1;
+        abd[l] = abd[l] / total;
+        mapped++;
+    }
+    free(seen);
+    free(touched);
+    return mapped;
+}
+
+/* comp_sam.cpp:321-349: pair list, OpenMP dynamic,1, packed upper triangle.
+ * abd is n x leaf_n. pairs_limit > 0 computes only the first pairs_limit pairs
+ * of the list (a bounded sample for timing). Returns the pairs computed. */
+long orc_all_pairs_f32(int n, int leaf_n, const float *abd, int order_n, const int32_t *o1, const float *d1,
+                       const int32_t *o2, const float *d2, const int32_t *od, float *sim_packed, int threads,
+                       long pairs_limit) {
+    long iter = (long)n * (n - 1) / 2;
+    if (pairs_limit > 0 && pairs_limit < iter) iter = pairs_limit;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    (void)threads;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long p = 0; p < iter; p++) {
+        /* invert p = m*n + k - (m+1)(m+2)/2 */
+        long m = 0, base = 0;
+        {
+            double nn = (double)n;
+            double disc = (2 * nn - 1) * (2 * nn - 1) - 8.0 * (double)p;
+            m = (long)floor(((2 * nn - 1) - sqrt(disc)) / 2);
+            if (m < 0) m = 0;
+            while (m > 0 && m * (long)n - m * (m + 1) / 2 > p) m--;
+            while ((m + 1) * (long)n - (m + 1) * (m + 2) / 2 <= p) m++;
+            base = m * (long)n - m * (m + 1) / 2;
+        }
+        long k = m + 1 + (p - base);
+        sim_packed[p] = orc_calc_sim_f32(order_n, o1, d1, o2, d2, od, abd + (size_t)m * leaf_n,
+                                         abd + (size_t)k * leaf_n);
+    }
+    return iter;
+}
+
+/* Same over an explicit pair list (row[i], col[i]). */
+void orc_pairs_f32(int leaf_n, const float *abd, int order_n, const int32_t *o1, const float *d1,
+                   const int32_t *o2, const float *d2, const int32_t *od, long n_pairs, const int32_t *row,
+                   const int32_t *col, float *sim, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    (void)threads;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < n_pairs; i++)
+        sim[i] = orc_calc_sim_f32(order_n, o1, d1, o2, d2, od, abd + (size_t)row[i] * leaf_n,
+                                  abd + (size_t)col[i] * leaf_n);
+}
+
+void orc_pairs_f64(int leaf_n, const float *abd, int order_n, const int32_t *o1, const float *d1,
+                   const int32_t *o2, const float *d2, const int32_t *od, long n_pairs, const int32_t *row,
+                   const int32_t *col, double *sim, int closed_form, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    (void)threads;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < n_pairs; i++) {
+        const float *A = abd + (size_t)row[i] * leaf_n, *B = abd + (size_t)col[i] * leaf_n;
+        sim[i] = closed_form ? orc_sharedmin_f64(order_n, o1, d1, o2, d2, od, A, B)
+                             : orc_calc_sim_f64(order_n, o1, d1, o2, d2, od, A, B);
+    }
+}
