@@ -36,16 +36,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def timed_steps(ctx, out, steps, warmup, torch):
+def timed_steps(ctx, steps, warmup):
+    """Warmup, then exactly `steps` passes bracketed by device synchronisation."""
     for _ in range(warmup):
         ctx.prepare(force=True)
-        ctx.all_pairs_device(out.data_ptr())
-    torch.cuda.synchronize()
+        ctx.all_pairs_device()
+    ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.prepare(force=True)
-        ctx.all_pairs_device(out.data_ptr())
-    torch.cuda.synchronize()
+        ctx.all_pairs_device()
+    ctx.synchronize()
     return (time.perf_counter() - t0) / steps
 
 
@@ -123,11 +124,11 @@ def main():
               "this build (pairs shard without communication; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
-    if not torch.cuda.is_available():
-        raise SystemExit("bench.py needs a HIP device")
+    # No torch here: libcms owns its device memory, and torch would load a
+    # second HIP runtime into the process.
     cms = importlib.import_module("coordinated-meta-storms_amd")
+    if cms.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP device: " + cms.last_error())
 
     t0 = time.perf_counter()
     tree, cp, counts, abd = cms.workload(1, samples=a.samples)
@@ -135,15 +136,13 @@ def main():
     npairs = n * (n - 1) // 2
     log(f"workload 1: {n} samples, {tree.leaf_n} leaves, nnz {abd.rowptr[-1]}, "
         f"generated in {time.perf_counter() - t0:.1f} s")
-    out = torch.empty(max(npairs, 1), dtype=torch.float32, device="cuda")
-
     # headline: shared-minimum kernel
     sm = cms.Context(0, kernel=cms.KERNEL_SHAREDMIN)
     sm.set_tree(tree)
     sm.set_samples_csr(abd)
-    sec = timed_steps(sm, out, a.steps, a.warmup, torch)
+    sec = timed_steps(sm, a.steps, a.warmup)
     st = sm.stats()
-    gpu_sm = out[:npairs].cpu().numpy()
+    gpu_sm = sm.copy_result(npairs)
     log(f"sharedmin: {sec * 1e3:.2f} ms/step, pair kernel {st.kernel_ms:.2f} ms, prep {st.prep_ms:.2f} ms, "
         f"visited pair-entries {st.pair_node_updates:.3e}")
     sm.close()
@@ -154,9 +153,9 @@ def main():
         wc = cms.Context(0, kernel=cms.KERNEL_WALK)
         wc.set_tree(tree)
         wc.set_samples_csr(abd)
-        wsec = timed_steps(wc, out, a.walk_steps, 0, torch)
+        wsec = timed_steps(wc, a.walk_steps, 0)
         wst = wc.stats()
-        gpu_walk = out[:npairs].cpu().numpy()
+        gpu_walk = wc.copy_result(npairs)
         wc.close()
         w_ops = wst.valu_ops / (wst.kernel_ms * 1e-3) / 1e12
         walk = {"ms_per_step": wsec * 1e3, "pairs_per_s": npairs / wsec,

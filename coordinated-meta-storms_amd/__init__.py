@@ -87,6 +87,9 @@ _sig("cms_set_samples", C.c_int, [_vp, C.c_int, _f32p])
 _sig("cms_set_samples_csr", C.c_int, [_vp, C.c_int, _i64p, _i32p, _f32p])
 _sig("cms_all_pairs", C.c_int, [_vp, _f32p])
 _sig("cms_all_pairs_device", C.c_int, [_vp, _vp, _vp])
+_sig("cms_copy_result", C.c_int, [_vp, _f32p, C.c_int64])
+_sig("cms_synchronize", C.c_int, [_vp])
+_sig("cms_device_count", C.c_int, [])
 _sig("cms_sim_pairs", C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f32p])
 _sig("cms_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f32p])
 _sig("cms_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
@@ -125,7 +128,8 @@ _sig("cms_version", C.c_char_p, [])
 #: every symbol declared in include/cms.h (checked by the CPU test suite)
 EXPORTED = [
     "cms_opts_default", "cms_create", "cms_destroy", "cms_last_error", "cms_set_opts", "cms_set_tree",
-    "cms_set_samples", "cms_set_samples_csr", "cms_all_pairs", "cms_all_pairs_device", "cms_sim_pairs",
+    "cms_set_samples", "cms_set_samples_csr", "cms_all_pairs", "cms_all_pairs_device", "cms_copy_result",
+    "cms_synchronize", "cms_device_count", "cms_sim_pairs",
     "cms_tile", "cms_get_stats", "cms_prepare", "cms_validate_tree", "cms_db_open", "cms_db_close", "cms_db_leaf_n", "cms_db_order_n",
     "cms_db_has_tree", "cms_db_has_cp", "cms_db_order", "cms_db_leaf_name", "cms_table_load",
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
@@ -318,7 +322,7 @@ class Samples:
         return out
 
     def subset(self, idx) -> "Samples":
-        idx = np.asarray(idx)
+        idx = np.asarray(idx, dtype=np.int64)
         lens = np.diff(self.rowptr)[idx]
         rp = np.zeros(len(idx) + 1, np.int64)
         rp[1:] = np.cumsum(lens)
@@ -461,8 +465,21 @@ class Context:
         _check(lib.cms_all_pairs(self._h, out), self._h)
         return out[: self.n * (self.n - 1) // 2]
 
-    def all_pairs_device(self, d_ptr: int, stream: int | None = None):
-        _check(lib.cms_all_pairs_device(self._h, _vp(d_ptr), _vp(stream) if stream else None), self._h)
+    def all_pairs_device(self, d_ptr: int | None = None, stream: int | None = None):
+        """Enqueue all pairs into a device buffer (None: the context's own)."""
+        _check(lib.cms_all_pairs_device(self._h, _vp(d_ptr) if d_ptr else None, _vp(stream) if stream else None),
+               self._h)
+
+    def copy_result(self, count: int | None = None) -> np.ndarray:
+        """Read back the context-owned result buffer (after all_pairs_device)."""
+        if count is None:
+            count = self.n * (self.n - 1) // 2
+        out = np.zeros(max(count, 1), np.float32)
+        _check(lib.cms_copy_result(self._h, out, count), self._h)
+        return out[:count]
+
+    def synchronize(self):
+        _check(lib.cms_synchronize(self._h), self._h)
 
     def sim_pairs(self, row, col) -> np.ndarray:
         row = np.ascontiguousarray(row, np.int32)
@@ -491,6 +508,12 @@ def packed_index(m, n, N):
     m = np.asarray(m, np.int64)
     n = np.asarray(n, np.int64)
     return m * N + n - (m + 1) * (m + 2) // 2
+
+
+def device_count() -> int:
+    """HIP devices visible to libcms (0 when none)."""
+    n = lib.cms_device_count()
+    return max(n, 0)
 
 
 def version() -> str:

@@ -165,6 +165,8 @@ int sharedmin_supported(cms_ctx *c) {
     if (!c->tree.nonneg_beta)
         return fail(c, CMS_E_UNSUPPORTED, "shared-minimum kernel needs non-negative branch lengths");
     if (c->has_negative) return fail(c, CMS_E_UNSUPPORTED, "shared-minimum kernel needs non-negative abundances");
+    if (c->tree.nslots > kEmbedMaxSlots)
+        return fail(c, CMS_E_UNSUPPORTED, "schedule keeps too many registers live for the embedding kernel");
     return CMS_OK;
 }
 
@@ -228,6 +230,10 @@ int prepare(cms_ctx *c) {
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
         rc = ensure_sharedmin(c);
     } else {
+        if (c->tree.nslots > walk_max_slots())
+            return fail(c, CMS_E_UNSUPPORTED,
+                        "schedule keeps " + std::to_string(c->tree.nslots) + " registers live; the walk kernel holds " +
+                            std::to_string(walk_max_slots()) + " (re-order the tree, e.g. Sethi-Ullman)");
         if (c->ab_storage == c->opts.storage) return CMS_OK;
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
         rc = ensure_walk_layout(c, c->opts.storage);
@@ -438,7 +444,11 @@ int cms_set_samples(cms_ctx *c, int n, const float *abd) {
 int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
     int rc = prepare(c);
     if (rc) return rc;
-    if (c->n >= 2 && !d_sim) return fail(c, CMS_E_INVAL, "cms_all_pairs_device: null output");
+    if (!d_sim) {
+        rc = ensure_out(c, (size_t)c->n * (c->n > 0 ? c->n - 1 : 0) / 2);
+        if (rc) return rc;
+        d_sim = c->d_out;
+    }
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     if (st != c->stream) {
         // layouts are built on the context stream: order the caller's stream after them
@@ -465,6 +475,34 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
     HIPCALL(c, hipEventRecord(c->ev1, st));
     c->kern_pending = true;
     return CMS_OK;
+}
+
+int cms_copy_result(cms_ctx *c, float *host, int64_t count) {
+    if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (count < 0 || (size_t)count > c->out_cap || (count > 0 && !host))
+        return fail(c, CMS_E_INVAL, "cms_copy_result: count exceeds the result buffer");
+    if (count == 0) return CMS_OK;
+    HIPCALL(c, hipSetDevice(c->device));
+    HIPCALL(c, hipMemcpyAsync(host, c->d_out, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return CMS_OK;
+}
+
+int cms_synchronize(cms_ctx *c) {
+    if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    HIPCALL(c, hipSetDevice(c->device));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return CMS_OK;
+}
+
+int cms_device_count(void) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        set_global_error(std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+        return e == hipErrorNoDevice ? 0 : CMS_E_HIP;
+    }
+    return n;
 }
 
 // Rebuild the device layouts of the current kernel from the uploaded samples

@@ -323,6 +323,7 @@ hipError_t launch_embed(const WalkNode *nodes, int order_n, int nslots, const fl
     (void)nrows;
     if (n <= 0 || order_n <= 0) return hipSuccess;
     const size_t lds = (size_t)(nslots > 0 ? nslots : 1) * 64 * sizeof(double);
+    if (lds + sizeof(float) * kEmbedLeafChunk * 64 > 65536) return hipErrorInvalidConfiguration;
     embed_kernel<<<(unsigned)(npad / 64), 64, lds, st>>>((cnode_ptr)nodes, order_n, nslots, ab, nlp, n, npad,
                                                          (cdouble_ptr)ent_beta, (cint_ptr)ent_row, nwords, scale, Y, bm);
     return hipGetLastError();

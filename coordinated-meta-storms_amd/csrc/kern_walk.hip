@@ -73,7 +73,8 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
     extern __shared__ float lds[];
     constexpr bool KAHAN = (STORE != CMS_STORE_F32);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t tile = xcd_remap(blockIdx.x, gridDim.x) * kWalkWaves + wave;
+    const int waves_per_wg = (int)(blockDim.x >> 6);
+    const int64_t tile = xcd_remap(blockIdx.x, gridDim.x) * waves_per_wg + wave;
     if (tile >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
     const int per_wave = 16 * kWalkLeafChunk + 2 * 64 * d.nslots;
     float *stage = lds + (size_t)wave * per_wave;
@@ -347,21 +348,34 @@ hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const 
     return hipGetLastError();
 }
 
+// Waves per workgroup that fit the 64 KiB workgroup LDS limit (0 = none).
+static inline int waves_for_lds(size_t per_wave) {
+    const size_t limit = 65536;
+    int w = kWalkWaves;
+    while (w > 0 && per_wave * (size_t)w > limit) w--;
+    return w;
+}
+
+int walk_max_slots() { return (int)((65536 / sizeof(float) - 16 * kWalkLeafChunk) / 128); }
+
 template <int MODE>
 static hipError_t launch_tiles(const WalkData &d, int64_t ntiles, int64_t nb, int r0, int nr, int c0, int nc,
                                float *out, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    const int64_t nwg = (ntiles + kWalkWaves - 1) / kWalkWaves;
-    const size_t lds = walk_lds_per_wave(d.nslots) * kWalkWaves;
+    const int waves = waves_for_lds(walk_lds_per_wave(d.nslots));
+    if (waves == 0) return hipErrorInvalidConfiguration;
+    const int64_t nwg = (ntiles + waves - 1) / waves;
+    const size_t lds = walk_lds_per_wave(d.nslots) * waves;
+    const dim3 grid((unsigned)nwg), block(64 * waves);
     switch (d.storage) {
     case CMS_STORE_F32:
-        walk_tile_kernel<CMS_STORE_F32, MODE><<<dim3((unsigned)nwg), 64 * kWalkWaves, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
+        walk_tile_kernel<CMS_STORE_F32, MODE><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
         break;
     case CMS_STORE_BF16:
-        walk_tile_kernel<CMS_STORE_BF16, MODE><<<dim3((unsigned)nwg), 64 * kWalkWaves, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
+        walk_tile_kernel<CMS_STORE_BF16, MODE><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
         break;
     default:
-        walk_tile_kernel<CMS_STORE_F16, MODE><<<dim3((unsigned)nwg), 64 * kWalkWaves, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
+        walk_tile_kernel<CMS_STORE_F16, MODE><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
         break;
     }
     return hipGetLastError();
@@ -381,17 +395,20 @@ hipError_t launch_walk_rect(const WalkData &d, int r0, int nr, int c0, int nc, f
 hipError_t launch_walk_pairs(const WalkData &d, int64_t n_pairs, const int32_t *row, const int32_t *col, float *out,
                              hipStream_t st) {
     if (n_pairs <= 0) return hipSuccess;
-    const int64_t nwg = (n_pairs + 255) / 256;
-    const size_t lds = (size_t)4 * 64 * d.nslots * sizeof(float2);
+    const int waves = waves_for_lds((size_t)64 * d.nslots * sizeof(float2));
+    if (waves == 0) return hipErrorInvalidConfiguration;
+    const int64_t nwg = (n_pairs + 64 * waves - 1) / (64 * waves);
+    const size_t lds = (size_t)waves * 64 * d.nslots * sizeof(float2);
+    const dim3 grid((unsigned)nwg), block(64 * waves);
     switch (d.storage) {
     case CMS_STORE_F32:
-        walk_pairs_kernel<CMS_STORE_F32><<<dim3((unsigned)nwg), 256, lds, st>>>(d, n_pairs, row, col, out);
+        walk_pairs_kernel<CMS_STORE_F32><<<grid, block, lds, st>>>(d, n_pairs, row, col, out);
         break;
     case CMS_STORE_BF16:
-        walk_pairs_kernel<CMS_STORE_BF16><<<dim3((unsigned)nwg), 256, lds, st>>>(d, n_pairs, row, col, out);
+        walk_pairs_kernel<CMS_STORE_BF16><<<grid, block, lds, st>>>(d, n_pairs, row, col, out);
         break;
     default:
-        walk_pairs_kernel<CMS_STORE_F16><<<dim3((unsigned)nwg), 256, lds, st>>>(d, n_pairs, row, col, out);
+        walk_pairs_kernel<CMS_STORE_F16><<<grid, block, lds, st>>>(d, n_pairs, row, col, out);
         break;
     }
     return hipGetLastError();

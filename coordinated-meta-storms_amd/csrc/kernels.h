@@ -33,6 +33,11 @@ struct WalkData {
     int storage;        // CMS_STORE_*
 };
 
+// Largest live-register count the walk kernels can hold in LDS.
+int walk_max_slots();
+// Largest live-register count of the embedding kernel.
+constexpr int kEmbedMaxSlots = 96;
+
 // Scatter CSR samples into the walk layout (buffer zeroed by the caller).
 //   leaf_lp_ptr/leaf_lp: leaf -> leaf-stream positions (CSR over leaves)
 hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const float *val, int n,

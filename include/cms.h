@@ -100,8 +100,17 @@ int cms_set_samples_csr(cms_ctx *c, int n, const int64_t *rowptr,
  * (comp_sam.cpp:338-349). */
 int cms_all_pairs(cms_ctx *c, float *sim_packed);
 /* Same, into a device buffer of N(N-1)/2 floats, enqueued on `stream`
- * (hipStream_t, NULL = the context's stream). No host synchronisation. */
+ * (hipStream_t, NULL = the context's stream). No host synchronisation.
+ * d_sim_packed NULL = a device buffer owned by the context (read it back with
+ * cms_copy_result). */
 int cms_all_pairs_device(cms_ctx *c, float *d_sim_packed, void *stream);
+/* Copy the first `count` floats of the context-owned result buffer to host
+ * memory (waits for the context's stream). */
+int cms_copy_result(cms_ctx *c, float *host, int64_t count);
+/* Wait for all work enqueued by the context. */
+int cms_synchronize(cms_ctx *c);
+/* Number of HIP devices visible to the library (negative on error). */
+int cms_device_count(void);
 
 /* Explicit pair list: sim[i] = sim(row[i], col[i]). Host arrays. The
  * reference kernels' boundary (CalcSimOfDiagonalSquare cms_cuda.cu:52-54 and

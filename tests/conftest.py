@@ -28,9 +28,12 @@ def orc():
 
 
 def has_gpu() -> bool:
+    """Device check through libcms itself. torch is deliberately not imported
+    in processes that load libcms: it bundles its own HIP runtime, and two HIP
+    runtimes in one process do not share the device."""
     try:
-        import torch
-        return torch.cuda.is_available()
+        cms = importlib.import_module("coordinated-meta-storms_amd")
+        return cms.device_count() > 0
     except Exception:
         return False
 
