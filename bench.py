@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=256)
     ap.add_argument("--cpu-pairs", type=int, default=32640)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run a plain copy kernel of this many bytes (traffic-counter reference)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every GPU box here has one, and multi-GPU runs are not part of "
@@ -191,6 +193,11 @@ def main():
     }
     if walk:
         res["speedup_vs_plain_fp32_walk"] = walk["ms_per_step"] / (sec * 1e3)
+    if a.copy_calib > 0:
+        with cms.Context(0) as cc:
+            ms = cc.calibrate_copy(a.copy_calib, 5)
+        res["copy_calibration"] = {"bytes": a.copy_calib, "ms": ms, "GBps": 2 * a.copy_calib / ms / 1e6}
+        log(f"copy calibration: {a.copy_calib} B in {ms:.3f} ms ({2 * a.copy_calib / ms / 1e6:.0f} GB/s read+write)")
     if not a.no_cpu:
         cb, acc = cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n, a.cpu_threads,
                                min(a.cpu_samples, n), a.cpu_pairs)

@@ -90,6 +90,7 @@ _sig("cms_all_pairs_device", C.c_int, [_vp, _vp, _vp])
 _sig("cms_copy_result", C.c_int, [_vp, _f32p, C.c_int64])
 _sig("cms_synchronize", C.c_int, [_vp])
 _sig("cms_device_count", C.c_int, [])
+_sig("cms_calibrate_copy", C.c_int, [_vp, C.c_int64, C.c_int, C.POINTER(C.c_double)])
 _sig("cms_sim_pairs", C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f32p])
 _sig("cms_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f32p])
 _sig("cms_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
@@ -129,7 +130,7 @@ _sig("cms_version", C.c_char_p, [])
 EXPORTED = [
     "cms_opts_default", "cms_create", "cms_destroy", "cms_last_error", "cms_set_opts", "cms_set_tree",
     "cms_set_samples", "cms_set_samples_csr", "cms_all_pairs", "cms_all_pairs_device", "cms_copy_result",
-    "cms_synchronize", "cms_device_count", "cms_sim_pairs",
+    "cms_synchronize", "cms_device_count", "cms_calibrate_copy", "cms_sim_pairs",
     "cms_tile", "cms_get_stats", "cms_prepare", "cms_validate_tree", "cms_db_open", "cms_db_close", "cms_db_leaf_n", "cms_db_order_n",
     "cms_db_has_tree", "cms_db_has_cp", "cms_db_order", "cms_db_leaf_name", "cms_table_load",
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
@@ -480,6 +481,12 @@ class Context:
 
     def synchronize(self):
         _check(lib.cms_synchronize(self._h), self._h)
+
+    def calibrate_copy(self, nbytes: int, reps: int = 5) -> float:
+        """Average ms of a plain device copy of nbytes (traffic-counter reference)."""
+        ms = C.c_double(0)
+        _check(lib.cms_calibrate_copy(self._h, nbytes, reps, C.byref(ms)), self._h)
+        return ms.value
 
     def sim_pairs(self, row, col) -> np.ndarray:
         row = np.ascontiguousarray(row, np.int32)

@@ -505,6 +505,30 @@ int cms_device_count(void) {
     return n;
 }
 
+int cms_calibrate_copy(cms_ctx *c, int64_t bytes, int reps, double *ms) {
+    if (!c || bytes <= 0 || reps <= 0 || !ms) return fail(c, CMS_E_INVAL, "cms_calibrate_copy: bad argument");
+    HIPCALL(c, hipSetDevice(c->device));
+    uint8_t *a = nullptr, *b = nullptr;
+    HIPCALL(c, hipMalloc((void **)&a, (size_t)bytes));
+    hipError_t e = hipMalloc((void **)&b, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        HIPCALL(c, e);
+    }
+    HIPCALL(c, hipMemsetAsync(a, 1, (size_t)bytes, c->stream));
+    HIPCALL(c, launch_calib_copy(a, b, bytes, c->stream));  // warm-up
+    HIPCALL(c, hipEventRecord(c->evp0, c->stream));
+    for (int r = 0; r < reps; r++) HIPCALL(c, launch_calib_copy(a, b, bytes, c->stream));
+    HIPCALL(c, hipEventRecord(c->evp1, c->stream));
+    HIPCALL(c, hipEventSynchronize(c->evp1));
+    float t = 0;
+    HIPCALL(c, hipEventElapsedTime(&t, c->evp0, c->evp1));
+    *ms = (double)t / reps;
+    dfree(a);
+    dfree(b);
+    return CMS_OK;
+}
+
 // Rebuild the device layouts of the current kernel from the uploaded samples
 // (the walk layout; for the shared-minimum form also the embedding and the
 // bitmaps), reusing allocations.

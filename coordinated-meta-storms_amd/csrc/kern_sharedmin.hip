@@ -310,6 +310,21 @@ hipError_t launch_zero_counter(unsigned long long *counter, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Plain streaming copy (16-byte loads and stores, grid-stride): the reference
+// point that converts the FETCH_SIZE / WRITE_SIZE counters into bytes.
+__global__ __launch_bounds__(256) void calib_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                         int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_calib_copy(const void *src, void *dst, int64_t bytes, hipStream_t st) {
+    const int64_t n16 = bytes / 16;
+    if (n16 <= 0) return hipSuccess;
+    calib_copy_kernel<<<4096, 256, 0, st>>>((const uint4 *)src, (uint4 *)dst, n16);
+    return hipGetLastError();
+}
+
 hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const float *val, int n,
                               const int32_t *leaf_uses, double *mass, hipStream_t st) {
     if (n <= 0) return hipSuccess;

@@ -78,6 +78,7 @@ struct SmData {
 };
 
 hipError_t launch_zero_counter(unsigned long long *counter, hipStream_t st);
+hipError_t launch_calib_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
 
 // Per-sample sum of leaf values over consumed leaf positions (FP64).
 hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const float *val, int n,

@@ -111,6 +111,11 @@ int cms_copy_result(cms_ctx *c, float *host, int64_t count);
 int cms_synchronize(cms_ctx *c);
 /* Number of HIP devices visible to the library (negative on error). */
 int cms_device_count(void);
+/* Diagnostic, no counterpart in the reference: run a plain device-to-device
+ * copy kernel of `bytes` bytes `reps` times on the context's stream and return
+ * its average duration in *ms. Used to calibrate hardware traffic counters
+ * (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
+int cms_calibrate_copy(cms_ctx *c, int64_t bytes, int reps, double *ms);
 
 /* Explicit pair list: sim[i] = sim(row[i], col[i]). Host arrays. The
  * reference kernels' boundary (CalcSimOfDiagonalSquare cms_cuda.cu:52-54 and
