@@ -28,8 +28,8 @@ struct cms_ctx {
     WalkNode *d_nodes = nullptr;
     WalkChunk *d_chunks = nullptr;
     int32_t *d_leaf_lp_ptr = nullptr, *d_leaf_lp = nullptr, *d_leaf_uses = nullptr;
-    double *d_beta = nullptr;
-    int32_t *d_entrow = nullptr;
+    EmbedNode *d_embnodes = nullptr;
+    int32_t *d_seg_first = nullptr, *d_seg_last = nullptr, *d_top_nodes = nullptr, *d_top_src = nullptr;
     int64_t nrows = 0;
 
     int n = -1;
@@ -47,6 +47,8 @@ struct cms_ctx {
     bool sm_ready = false;
     uint32_t *d_Y = nullptr, *d_bm = nullptr;
     double *d_mass = nullptr;
+    double *d_xroot = nullptr;
+    size_t cap_xroot = 0;
     size_t cap_Y = 0, cap_bm = 0, cap_mass = 0;
     int64_t npad = 0, nwords = 0;
     int scale_bits = 0;
@@ -111,6 +113,8 @@ void drop_layouts(cms_ctx *c) {
     dfree(c->d_Y);
     dfree(c->d_bm);
     dfree(c->d_mass);
+    dfree(c->d_xroot);
+    c->cap_xroot = 0;
     c->cap_Y = c->cap_bm = c->cap_mass = 0;
     c->sm_ready = false;
 }
@@ -202,9 +206,28 @@ int ensure_sharedmin(cms_ctx *c) {
     HIPCALL(c, dreserve(c->d_Y, c->cap_Y, (size_t)std::max<int64_t>(c->nrows, 1) * std::max<int64_t>(c->npad, 1)));
     HIPCALL(c, dreserve(c->d_bm, c->cap_bm, nbm));
     HIPCALL(c, hipMemsetAsync(c->d_bm, 0, sizeof(uint32_t) * nbm, c->stream));
-    HIPCALL(c, launch_embed(c->d_nodes, c->tree.order_n, std::max(c->tree.nslots, 1), (const float *)c->d_ab,
-                            (int64_t)c->tree.leaf_stream.size(), c->n, c->npad, c->d_beta, c->d_entrow, c->nrows,
-                            c->nwords, std::ldexp(1.0, K), c->d_Y, c->d_bm, c->stream));
+    HIPCALL(c, dreserve(c->d_xroot, c->cap_xroot,
+                        (size_t)std::max<size_t>(c->tree.seg_first.size(), 1) * std::max<int64_t>(c->npad, 1)));
+    EmbedData ed{};
+    ed.nodes = c->d_embnodes;
+    ed.order_n = c->tree.order_n;
+    ed.nslots = std::max(c->tree.nslots, 1);
+    ed.seg_first = c->d_seg_first;
+    ed.seg_last = c->d_seg_last;
+    ed.nseg = (int)c->tree.seg_first.size();
+    ed.top_nodes = c->d_top_nodes;
+    ed.top_src = c->d_top_src;
+    ed.ntop = (int)c->tree.top_nodes.size();
+    ed.ab = (const float *)c->d_ab;
+    ed.nlp = (int64_t)c->tree.leaf_stream.size();
+    ed.n = c->n;
+    ed.npad = c->npad;
+    ed.nwords = c->nwords;
+    ed.scale = std::ldexp(1.0, K);
+    ed.Y = c->d_Y;
+    ed.bm = c->d_bm;
+    ed.xroot = c->d_xroot;
+    HIPCALL(c, launch_embed(ed, c->stream));
     c->sm_ready = true;
     return CMS_OK;
 }
@@ -314,8 +337,11 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_leaf_lp_ptr);
     dfree(c->d_leaf_lp);
     dfree(c->d_leaf_uses);
-    dfree(c->d_beta);
-    dfree(c->d_entrow);
+    dfree(c->d_embnodes);
+    dfree(c->d_seg_first);
+    dfree(c->d_seg_last);
+    dfree(c->d_top_nodes);
+    dfree(c->d_top_src);
     dfree(c->d_rowptr);
     dfree(c->d_col);
     dfree(c->d_val);
@@ -369,13 +395,38 @@ int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const f
     HIPCALL(c, dalloc(c->d_leaf_lp_ptr, ptr.size()));
     HIPCALL(c, dalloc(c->d_leaf_lp, std::max<size_t>(lp.size(), 1)));
     HIPCALL(c, dalloc(c->d_leaf_uses, uses.size()));
-    HIPCALL(c, dalloc(c->d_beta, std::max<size_t>(t.ent_beta.size(), 1)));
-    HIPCALL(c, dalloc(c->d_entrow, std::max<size_t>(entrow.size(), 1)));
+    plan_segments(t, kEmbedSegMax);
+    std::vector<int32_t> export_seg(t.nodes.size(), -1);
+    for (size_t k = 0; k < t.seg_first.size(); k++)
+        if (t.seg_out[k]) export_seg[t.seg_last[k]] = (int32_t)k;
+    std::vector<EmbedNode> emb(t.nodes.size());
+    for (size_t i = 0; i < t.nodes.size(); i++) {
+        EmbedNode &e = emb[i];
+        e.code = t.nodes[i].code;
+        e.leafpos = t.nodes[i].leafpos;
+        e.w1 = t.nodes[i].w1;
+        e.w2 = t.nodes[i].w2;
+        e.row1 = entrow[2 * i];
+        e.row2 = entrow[2 * i + 1];
+        e.pad0 = export_seg[i];
+        e.pad1 = 0;
+        e.beta1 = t.ent_beta[2 * i];
+        e.beta2 = t.ent_beta[2 * i + 1];
+    }
+    HIPCALL(c, dalloc(c->d_embnodes, std::max<size_t>(emb.size(), 1)));
+    auto upload_i32 = [&](int32_t *&d, const std::vector<int32_t> &v) -> hipError_t {
+        hipError_t e = dalloc(d, std::max<size_t>(v.size(), 1));
+        if (e == hipSuccess && !v.empty()) e = hipMemcpy(d, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice);
+        return e;
+    };
+    HIPCALL(c, upload_i32(c->d_seg_first, t.seg_first));
+    HIPCALL(c, upload_i32(c->d_seg_last, t.seg_last));
+    HIPCALL(c, upload_i32(c->d_top_nodes, t.top_nodes));
+    HIPCALL(c, upload_i32(c->d_top_src, t.top_src));
     if (!t.nodes.empty()) {
         HIPCALL(c, hipMemcpy(c->d_nodes, t.nodes.data(), sizeof(WalkNode) * t.nodes.size(), hipMemcpyHostToDevice));
         HIPCALL(c, hipMemcpy(c->d_chunks, t.chunks.data(), sizeof(WalkChunk) * t.chunks.size(), hipMemcpyHostToDevice));
-        HIPCALL(c, hipMemcpy(c->d_beta, t.ent_beta.data(), sizeof(double) * t.ent_beta.size(), hipMemcpyHostToDevice));
-        HIPCALL(c, hipMemcpy(c->d_entrow, entrow.data(), sizeof(int32_t) * entrow.size(), hipMemcpyHostToDevice));
+        HIPCALL(c, hipMemcpy(c->d_embnodes, emb.data(), sizeof(EmbedNode) * emb.size(), hipMemcpyHostToDevice));
     }
     HIPCALL(c, hipMemcpy(c->d_leaf_lp_ptr, ptr.data(), sizeof(int32_t) * ptr.size(), hipMemcpyHostToDevice));
     if (!lp.empty()) HIPCALL(c, hipMemcpy(c->d_leaf_lp, lp.data(), sizeof(int32_t) * lp.size(), hipMemcpyHostToDevice));

@@ -61,7 +61,23 @@ struct CompiledTree {
     //   child_leafpos[e] >= 0: leaf-stream position, else -1 - (producer node)
     std::vector<int32_t> ent_child;
     std::vector<double> ent_beta;
+
+    // Subtree segments for the parallel embedding: node ranges
+    // [seg_first, seg_last] that are complete subtrees, contiguous in the
+    // schedule, of at most seg_max nodes. seg_out[k] = 1 when the segment
+    // root's value is read by a later node. Nodes outside every segment are
+    // "top" nodes, evaluated after all segments; top_src[2t+j] says where
+    // child j of top node t (node index top_nodes[t]) comes from:
+    // -1 = its leaf or register as in the walk schedule, k >= 0 = the
+    // output of segment k.
+    std::vector<int32_t> seg_first, seg_last, seg_out;
+    std::vector<int32_t> top_nodes, top_src;
 };
+
+// Split a tree schedule into subtree segments (fills the seg_* and top_*
+// fields). Schedules whose subtrees are not contiguous get no segments: every
+// node is then a top node.
+void plan_segments(CompiledTree &t, int seg_max);
 
 // Validate and compile the reference schedule arrays (comp.h:161-192 layout).
 // Returns CMS_OK or CMS_E_SCHEDULE with a message in `err`.

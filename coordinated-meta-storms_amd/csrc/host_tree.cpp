@@ -202,3 +202,51 @@ extern "C" int cms_validate_tree(int leaf_n, int order_n, const int32_t *o1, con
     if (nonneg_beta) *nonneg_beta = t.nonneg_beta ? 1 : 0;
     return CMS_OK;
 }
+
+namespace cms {
+
+void plan_segments(CompiledTree &t, int seg_max) {
+    const int n = t.order_n;
+    t.seg_first.clear();
+    t.seg_last.clear();
+    t.seg_out.clear();
+    t.top_nodes.clear();
+    t.top_src.clear();
+    std::vector<int32_t> size(n, 1), first(n), consumer(n, -1);
+    std::vector<char> contig(n, 1);
+    for (int i = 0; i < n; i++) {
+        first[i] = i;
+        for (int j = 0; j < 2; j++) {
+            const int32_t c = t.ent_child[2 * (size_t)i + j];
+            if (c >= 0) continue;
+            const int p = -1 - c;
+            size[i] += size[p];
+            first[i] = std::min(first[i], first[p]);
+            if (!contig[p]) contig[i] = 0;
+            consumer[p] = i;
+        }
+        if (i - first[i] + 1 != size[i]) contig[i] = 0;
+    }
+    auto fits = [&](int i) { return t.is_tree && contig[i] && size[i] <= seg_max; };
+    std::vector<int32_t> seg_of(n, -1);  // segment whose root is node i
+    std::vector<char> in_seg(n, 0);
+    for (int i = 0; i < n; i++) {
+        if (!fits(i)) continue;
+        if (consumer[i] >= 0 && fits(consumer[i])) continue;  // not maximal
+        seg_of[i] = (int32_t)t.seg_first.size();
+        t.seg_first.push_back(first[i]);
+        t.seg_last.push_back(i);
+        t.seg_out.push_back(consumer[i] >= 0 ? 1 : 0);
+        for (int k = first[i]; k <= i; k++) in_seg[k] = 1;
+    }
+    for (int i = 0; i < n; i++) {
+        if (in_seg[i]) continue;
+        t.top_nodes.push_back(i);
+        for (int j = 0; j < 2; j++) {
+            const int32_t c = t.ent_child[2 * (size_t)i + j];
+            t.top_src.push_back(c < 0 ? seg_of[-1 - c] : -1);
+        }
+    }
+}
+
+}  // namespace cms

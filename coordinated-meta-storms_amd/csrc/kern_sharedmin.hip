@@ -65,84 +65,159 @@ __global__ void sample_mass_kernel(const int64_t *__restrict__ rowptr, const int
     if (threadIdx.x == 0) mass[s] = m;
 }
 
-// One wave = 64 consecutive samples (lane = sample). Walks the schedule in
-// FP64, writes Y[row][s] for every active entry and the per-32-sample-block
-// entry bitmaps. Leaf values come from the walk layout, staged per chunk of 64
-// leaf positions.
-constexpr int kEmbedLeafChunk = 64;
+// Per-sample embedding. Every wave handles 64 consecutive samples
+// (lane = sample) and walks part of the schedule in FP64:
+//   embed_seg_kernel  one subtree segment per wave (grid.y = segment): its
+//                     contiguous node range, staged in chunks of 64 node
+//                     records plus the leaf values they consume;
+//   embed_top_kernel  the nodes above the segments, reading segment outputs
+//                     from xroot[segment][sample].
+// Each node writes Y[row][s] for its active child edges and ORs the
+// per-32-sample-block bits of those rows into the bitmaps.
+constexpr int kEmbedNodeChunk = 64;
+constexpr int kEmbedLeafChunk = 2 * kEmbedNodeChunk;
 
-__global__ __launch_bounds__(64) void embed_kernel(cnode_ptr nodes, int order_n, int nslots,
-                                                   const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
-                                                   cdouble_ptr ent_beta, cint_ptr ent_row, int64_t nwords,
-                                                   double scale, uint32_t *__restrict__ Y, uint32_t *__restrict__ bm) {
+struct BitmapWriter {
+    uint32_t bits0 = 0, bits1 = 0;
+    int64_t cur = -1;
+    __device__ __forceinline__ void flush(uint32_t *bm, int64_t blk0, int64_t nwords, int lane) {
+        if (cur >= 0 && lane == 0) {
+            if (bits0) atomicOr(bm + blk0 * nwords + cur, bits0);
+            if (bits1) atomicOr(bm + (blk0 + 1) * nwords + cur, bits1);
+        }
+        bits0 = bits1 = 0;
+    }
+    __device__ __forceinline__ void add(int32_t row, uint32_t q, uint32_t *bm, int64_t blk0, int64_t nwords,
+                                        int lane) {
+        const uint64_t m = __ballot(q != 0u);
+        const int64_t w = row >> 5;
+        if (w != cur) {
+            flush(bm, blk0, nwords, lane);
+            cur = w;
+        }
+        if ((uint32_t)m) bits0 |= 1u << (row & 31);
+        if ((uint32_t)(m >> 32)) bits1 |= 1u << (row & 31);
+    }
+};
+
+__global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restrict__ nodes,
+                                                       const int32_t *__restrict__ seg_first,
+                                                       const int32_t *__restrict__ seg_last, int order_n,
+                                                       const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
+                                                       int64_t nwords, double scale, uint32_t *__restrict__ Y,
+                                                       uint32_t *__restrict__ bm, double *__restrict__ xroot) {
     extern __shared__ double eslots[];  // nslots * 64
     __shared__ float stage[kEmbedLeafChunk * 64];
+    __shared__ EmbedNode snodes[kEmbedNodeChunk];
     const int lane = threadIdx.x;
     const int64_t s = (int64_t)blockIdx.x * 64 + lane;
     const bool valid = s < n;
-    const int64_t ab_base = (s >> 3) * nlp * 8 + (s & 7);
+    const float *abs_ = ab + (valid ? (s >> 3) * nlp * 8 + (s & 7) : 0);
     const int64_t blk0 = (int64_t)blockIdx.x * 2;  // 32-sample blocks of this wave
-    uint32_t bits0 = 0, bits1 = 0;
-    int64_t cur_word = 0;
-    int64_t chunk_lb = -kEmbedLeafChunk;  // leaf positions [chunk_lb, chunk_lb + 64) staged
+    const int first = seg_first[blockIdx.y], last = seg_last[blockIdx.y];
+    BitmapWriter bw;
 
-    for (int i = 0; i < order_n; i++) {
-        const uint32_t code = nodes[i].code;
-        int64_t lp = nodes[i].leafpos;
-        const int nleaf = ((code & kC1Leaf) ? 1 : 0) + ((code & kC2Leaf) ? 1 : 0);
-        if (nleaf && lp + nleaf > chunk_lb + kEmbedLeafChunk) {
-            chunk_lb = lp;
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll 8
-            for (int j = 0; j < kEmbedLeafChunk; j++) {
-                float v = 0.0f;
-                if (valid && chunk_lb + j < nlp) v = ab[ab_base + (chunk_lb + j) * 8];
-                stage[j * 64 + lane] = v;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int base = first; base <= last; base += kEmbedNodeChunk) {
+        const int cnt = min(kEmbedNodeChunk, last + 1 - base);
+        __builtin_amdgcn_wave_barrier();
+        if (lane < cnt) snodes[lane] = nodes[base + lane];
+        const int lb = nodes[base].leafpos;
+        const int le = (base + cnt < order_n) ? nodes[base + cnt].leafpos : (int)nlp;
+        const int nl = le - lb;  // <= 2 * cnt
+        for (int j0 = 0; j0 < nl; j0 += 16) {
+            float v[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) v[k] = (valid && j0 + k < nl) ? abs_[(int64_t)(lb + j0 + k) * 8] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                if (j0 + k < nl) stage[(j0 + k) * 64 + lane] = v[k];
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k = 0; k < cnt; k++) {
+            const uint32_t code = __builtin_amdgcn_readfirstlane(snodes[k].code);
+            int lp = __builtin_amdgcn_readfirstlane(snodes[k].leafpos) - lb;
+            const int32_t rows[2] = {__builtin_amdgcn_readfirstlane(snodes[k].row1),
+                                     __builtin_amdgcn_readfirstlane(snodes[k].row2)};
+            const int32_t exp_seg = __builtin_amdgcn_readfirstlane(snodes[k].pad0);
+            double x[2];
+            if (code & kC1Leaf) {
+                x[0] = (double)stage[lp * 64 + lane];
+                lp++;
+            } else {
+                x[0] = eslots[(code & 0xffu) * 64 + lane];
+            }
+            if (code & kC2Leaf)
+                x[1] = (double)stage[lp * 64 + lane];
+            else
+                x[1] = eslots[((code >> 8) & 0xffu) * 64 + lane];
+            const double betas[2] = {snodes[k].beta1, snodes[k].beta2};
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                if (rows[j] < 0) continue;
+                const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
+                Y[(int64_t)rows[j] * npad + s] = q;
+                bw.add(rows[j], q, bm, blk0, nwords, lane);
+            }
+            if (code & kDstLive) {
+                const double xp = (double)snodes[k].w1 * x[0] + (double)snodes[k].w2 * x[1];
+                if (exp_seg >= 0)
+                    xroot[(int64_t)exp_seg * npad + s] = xp;  // consumed by a top node
+                else
+                    eslots[((code >> 16) & 0xffu) * 64 + lane] = xp;
+            }
+        }
+    }
+    bw.flush(bm, blk0, nwords, lane);
+}
+
+__global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restrict__ nodes,
+                                                       const int32_t *__restrict__ top_nodes,
+                                                       const int32_t *__restrict__ top_src, int ntop,
+                                                       const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
+                                                       int64_t nwords, double scale, uint32_t *__restrict__ Y,
+                                                       uint32_t *__restrict__ bm, const double *__restrict__ xroot) {
+    extern __shared__ double eslots[];
+    const int lane = threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.x * 64 + lane;
+    const bool valid = s < n;
+    const float *abs_ = ab + (valid ? (s >> 3) * nlp * 8 + (s & 7) : 0);
+    const int64_t blk0 = (int64_t)blockIdx.x * 2;
+    BitmapWriter bw;
+    for (int t = 0; t < ntop; t++) {
+        const int i = top_nodes[t];
+        const EmbedNode nd = nodes[i];
+        const uint32_t code = __builtin_amdgcn_readfirstlane(nd.code);
+        int64_t lp = __builtin_amdgcn_readfirstlane(nd.leafpos);
+        const int32_t src[2] = {top_src[2 * t], top_src[2 * t + 1]};
         double x[2];
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const bool leaf = j == 0 ? (code & kC1Leaf) : (code & kC2Leaf);
             if (leaf) {
-                x[j] = (double)stage[(lp - chunk_lb) * 64 + lane];
+                x[j] = valid ? (double)abs_[lp * 8] : 0.0;
                 lp++;
+            } else if (src[j] >= 0) {
+                x[j] = xroot[(int64_t)src[j] * npad + s];
             } else {
                 x[j] = eslots[((code >> (8 * j)) & 0xffu) * 64 + lane];
             }
         }
+        const int32_t rows[2] = {__builtin_amdgcn_readfirstlane(nd.row1), __builtin_amdgcn_readfirstlane(nd.row2)};
+        const double betas[2] = {nd.beta1, nd.beta2};
 #pragma unroll
         for (int j = 0; j < 2; j++) {
-            const int32_t row = ent_row[2 * (int64_t)i + j];
-            if (row < 0) continue;
-            const double y = ent_beta[2 * (int64_t)i + j] * x[j];
-            const uint32_t q = (uint32_t)rint(y * scale);
-            Y[(int64_t)row * npad + s] = q;
-            const uint64_t m = __ballot(q != 0u);
-            const int64_t w = row >> 5;
-            if (w != cur_word) {
-                if (lane == 0) {
-                    bm[blk0 * nwords + cur_word] = bits0;
-                    bm[(blk0 + 1) * nwords + cur_word] = bits1;
-                }
-                bits0 = bits1 = 0;
-                cur_word = w;
-            }
-            if ((uint32_t)m) bits0 |= 1u << (row & 31);
-            if ((uint32_t)(m >> 32)) bits1 |= 1u << (row & 31);
+            if (rows[j] < 0) continue;
+            const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
+            Y[(int64_t)rows[j] * npad + s] = q;
+            bw.add(rows[j], q, bm, blk0, nwords, lane);
         }
-        if (code & kDstLive) {
-            const double xp = (double)nodes[i].w1 * x[0] + (double)nodes[i].w2 * x[1];
-            eslots[((code >> 16) & 0xffu) * 64 + lane] = xp;
-        }
+        if (code & kDstLive)
+            eslots[((code >> 16) & 0xffu) * 64 + lane] = (double)nd.w1 * x[0] + (double)nd.w2 * x[1];
+        __builtin_amdgcn_wave_barrier();
     }
-    if (lane == 0 && cur_word < nwords) {
-        bm[blk0 * nwords + cur_word] = bits0;
-        bm[(blk0 + 1) * nwords + cur_word] = bits1;
-    }
+    bw.flush(bm, blk0, nwords, lane);
 }
 
 // ------------------------------------------------------------ pair tiles
@@ -332,15 +407,22 @@ hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const f
     return hipGetLastError();
 }
 
-hipError_t launch_embed(const WalkNode *nodes, int order_n, int nslots, const float *ab, int64_t nlp, int n,
-                        int64_t npad, const double *ent_beta, const int32_t *ent_row, int64_t nrows, int64_t nwords,
-                        double scale, uint32_t *Y, uint32_t *bm, hipStream_t st) {
-    (void)nrows;
-    if (n <= 0 || order_n <= 0) return hipSuccess;
-    const size_t lds = (size_t)(nslots > 0 ? nslots : 1) * 64 * sizeof(double);
-    if (lds + sizeof(float) * kEmbedLeafChunk * 64 > 65536) return hipErrorInvalidConfiguration;
-    embed_kernel<<<(unsigned)(npad / 64), 64, lds, st>>>((cnode_ptr)nodes, order_n, nslots, ab, nlp, n, npad,
-                                                         (cdouble_ptr)ent_beta, (cint_ptr)ent_row, nwords, scale, Y, bm);
+hipError_t launch_embed(const EmbedData &e, hipStream_t st) {
+    if (e.n <= 0 || e.order_n <= 0) return hipSuccess;
+    const size_t lds = (size_t)(e.nslots > 0 ? e.nslots : 1) * 64 * sizeof(double);
+    if (lds + sizeof(float) * kEmbedLeafChunk * 64 + sizeof(EmbedNode) * kEmbedNodeChunk > 65536)
+        return hipErrorInvalidConfiguration;
+    const unsigned waves = (unsigned)(e.npad / 64);
+    if (e.nseg > 0) {
+        embed_seg_kernel<<<dim3(waves, (unsigned)e.nseg), 64, lds, st>>>(
+            e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.nlp, e.n, e.npad, e.nwords, e.scale, e.Y, e.bm,
+            e.xroot);
+        hipError_t r = hipGetLastError();
+        if (r != hipSuccess) return r;
+    }
+    if (e.ntop > 0)
+        embed_top_kernel<<<waves, 64, lds, st>>>(e.nodes, e.top_nodes, e.top_src, e.ntop, e.ab, e.nlp, e.n, e.npad,
+                                                 e.nwords, e.scale, e.Y, e.bm, e.xroot);
     return hipGetLastError();
 }
 

@@ -10,8 +10,8 @@
 //    schedule order), staged per chunk of 32 leaf positions into LDS with a
 //    one-chunk register prefetch, so the walk reads LDS only;
 //  * the live register slots (re-allocated by liveness on the host, ~10 for
-//    GreenGenes-sized trees instead of 150) sit in LDS as float2 per lane,
-//    conflict-free (slot * 64 + lane);
+//    GreenGenes-sized trees instead of 150) sit in LDS as two planes (sample
+//    a, sample b), one dword per lane at slot * 64 + lane: bank-conflict free;
 //  * no FMA contraction (built with -ffp-contract=off) and the reference's
 //    order of the two total additions, so F32 results are bit-identical to
 //    the CPU `comp -M 0`;
@@ -78,7 +78,10 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
     if (tile >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
     const int per_wave = 16 * kWalkLeafChunk + 2 * 64 * d.nslots;
     float *stage = lds + (size_t)wave * per_wave;
-    float2 *slots = reinterpret_cast<float2 *>(stage + 16 * kWalkLeafChunk);
+    // register slots as two planes (sample a, sample b): stride-4 dword access
+    // per lane, bank-conflict free
+    float *slotA = stage + 16 * kWalkLeafChunk;
+    float *slotB = slotA + 64 * d.nslots;
 
     int64_t rowbase, colbase;
     int64_t rowlim, collim;
@@ -148,17 +151,15 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
                 b1 = stage[lp * 16 + 8 + c];
                 lp++;
             } else {
-                float2 v = slots[(code & 0xffu) * 64 + lane];
-                a1 = v.x;
-                b1 = v.y;
+                a1 = slotA[(code & 0xffu) * 64 + lane];
+                b1 = slotB[(code & 0xffu) * 64 + lane];
             }
             if (code & kC2Leaf) {
                 a2 = stage[lp * 16 + r];
                 b2 = stage[lp * 16 + 8 + c];
             } else {
-                float2 v = slots[((code >> 8) & 0xffu) * 64 + lane];
-                a2 = v.x;
-                b2 = v.y;
+                a2 = slotA[((code >> 8) & 0xffu) * 64 + lane];
+                b2 = slotB[((code >> 8) & 0xffu) * 64 + lane];
             }
             const float m1 = fminf(a1, b1), m2 = fminf(a2, b2);
             if constexpr (KAHAN) {
@@ -176,7 +177,8 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
             if (code & kDstLive) {
                 float ra = (a1 - m1) * w1 + (a2 - m2) * w2;
                 float rb_ = (b1 - m1) * w1 + (b2 - m2) * w2;
-                slots[((code >> 16) & 0xffu) * 64 + lane] = make_float2(ra, rb_);
+                slotA[((code >> 16) & 0xffu) * 64 + lane] = ra;
+                slotB[((code >> 16) & 0xffu) * 64 + lane] = rb_;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -216,7 +218,8 @@ __global__ __launch_bounds__(256) void walk_pairs_kernel(WalkData d, int64_t n_p
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
     if (i0 + wave * 64 >= n_pairs) return;
-    float2 *slots = reinterpret_cast<float2 *>(lds) + (size_t)wave * 64 * d.nslots;
+    float *slotA = lds + (size_t)wave * 2 * 64 * d.nslots;
+    float *slotB = slotA + 64 * d.nslots;
     const int64_t pi = i0 + threadIdx.x;
     const bool ok = pi < n_pairs;
     const int64_t ra = ok ? prow[pi] : 0, rc = ok ? pcol[pi] : 0;
@@ -235,17 +238,15 @@ __global__ __launch_bounds__(256) void walk_pairs_kernel(WalkData d, int64_t n_p
             b1 = load_ab<STORE>(d.ab, offb + lp * 8, invb);
             lp++;
         } else {
-            float2 v = slots[(code & 0xffu) * 64 + lane];
-            a1 = v.x;
-            b1 = v.y;
+            a1 = slotA[(code & 0xffu) * 64 + lane];
+            b1 = slotB[(code & 0xffu) * 64 + lane];
         }
         if (code & kC2Leaf) {
             a2 = load_ab<STORE>(d.ab, offa + lp * 8, inva);
             b2 = load_ab<STORE>(d.ab, offb + lp * 8, invb);
         } else {
-            float2 v = slots[((code >> 8) & 0xffu) * 64 + lane];
-            a2 = v.x;
-            b2 = v.y;
+            a2 = slotA[((code >> 8) & 0xffu) * 64 + lane];
+            b2 = slotB[((code >> 8) & 0xffu) * 64 + lane];
         }
         const float m1 = fminf(a1, b1), m2 = fminf(a2, b2);
         if constexpr (KAHAN) {
@@ -263,7 +264,8 @@ __global__ __launch_bounds__(256) void walk_pairs_kernel(WalkData d, int64_t n_p
         if (code & kDstLive) {
             float xa = (a1 - m1) * w1 + (a2 - m2) * w2;
             float xb = (b1 - m1) * w1 + (b2 - m2) * w2;
-            slots[((code >> 16) & 0xffu) * 64 + lane] = make_float2(xa, xb);
+            slotA[((code >> 16) & 0xffu) * 64 + lane] = xa;
+            slotB[((code >> 16) & 0xffu) * 64 + lane] = xb;
         }
     }
     float sim;

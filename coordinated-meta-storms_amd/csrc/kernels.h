@@ -36,7 +36,9 @@ struct WalkData {
 // Largest live-register count the walk kernels can hold in LDS.
 int walk_max_slots();
 // Largest live-register count of the embedding kernel.
-constexpr int kEmbedMaxSlots = 96;
+constexpr int kEmbedMaxSlots = 56;
+// Largest subtree segment (nodes) one embedding wave walks.
+constexpr int kEmbedSegMax = 512;
 
 // Scatter CSR samples into the walk layout (buffer zeroed by the caller).
 //   leaf_lp_ptr/leaf_lp: leaf -> leaf-stream positions (CSR over leaves)
@@ -86,9 +88,37 @@ hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const f
 // Embedding y per (row, sample) into Y (fixed point) and the block bitmaps.
 //   ab: F32 walk layout (block-leaf-major) of the same samples
 //   ent_row[e]: row of entry e in Y or -1 (inactive)
-hipError_t launch_embed(const WalkNode *nodes, int order_n, int nslots, const float *ab, int64_t nlp, int n,
-                        int64_t npad, const double *ent_beta, const int32_t *ent_row, int64_t nrows, int64_t nwords,
-                        double scale, uint32_t *Y, uint32_t *bm, hipStream_t st);
+// One schedule node as the embedding kernel reads it (48 bytes): the walk
+// node plus the Y rows and beta of its two child edges (row -1 = inactive).
+struct EmbedNode {
+    uint32_t code;
+    int32_t leafpos;
+    float w1, w2;
+    int32_t row1, row2;
+    int32_t pad0, pad1;
+    double beta1, beta2;
+};
+static_assert(sizeof(EmbedNode) == 48, "EmbedNode must be 48 bytes");
+
+struct EmbedData {
+    const EmbedNode *nodes;  // pad0 = segment whose output this node is (-1: none)
+    int order_n;
+    int nslots;
+    const int32_t *seg_first, *seg_last;  // nseg subtree segments
+    int nseg;
+    const int32_t *top_nodes, *top_src;   // ntop nodes above the segments
+    int ntop;
+    const float *ab;  // F32 walk layout
+    int64_t nlp;
+    int n;
+    int64_t npad;
+    int64_t nwords;
+    double scale;     // 2^K
+    uint32_t *Y;
+    uint32_t *bm;     // zeroed by the caller; bits are ORed in
+    double *xroot;    // nseg x npad
+};
+hipError_t launch_embed(const EmbedData &e, hipStream_t st);
 // All pairs, packed upper triangle.
 hipError_t launch_sm_all_pairs(const SmData &d, float *out, hipStream_t st);
 // Rectangle over sample ranges aligned to nothing: rows [r0, r0+nr) x cols [c0, c0+nc).
