@@ -107,6 +107,129 @@ def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, s
     return res, acc
 
 
+def traffic_from_profiles():
+    """HBM bytes per launch of the pair kernel, as measured by the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (profiles/traffic_workload1.txt:
+    a line 'traffic_bytes_per_launch <bytes>'), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic_workload1.txt")
+    try:
+        with open(p) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 2 and parts[0] == "traffic_bytes_per_launch":
+                    return float(parts[1])
+    except OSError:
+        pass
+    return None
+
+
+def run_batched(cms, a):
+    """Workload 3: 100,000 samples, nonzero OTUs log-uniform in [50, 20,000]
+    (Zipf 1.0, NegBinomial(30, 0.3)), 99,322-leaf tree; all ~5e9 pairs
+    streamed to pinned host memory in bands (binned samples, persistent
+    kernel, autotuned variant, hipGraph-replayed band launches)."""
+    import numpy as np
+    t0 = time.perf_counter()
+    tree, cp, counts, abd = cms.workload(3, samples=a.samples)
+    del counts
+    n = abd.n
+    npairs = n * (n - 1) // 2
+    log(f"workload 3: {n} samples, nnz {abd.rowptr[-1]}, generated in {time.perf_counter() - t0:.1f} s")
+    ctx = cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=a.bin, autotune=a.autotune, variant=a.variant,
+                      use_graph=a.graph)
+    ctx.set_tree(tree)
+    ctx.set_samples_csr(abd)
+    t1 = time.perf_counter()
+    ctx.prepare(force=True)
+    ctx.synchronize()
+    prep_s = time.perf_counter() - t1
+    got = {"pairs": 0, "bands": 0, "sum": 0.0}
+
+    def fn(first, count, sims):
+        got["pairs"] += count
+        got["bands"] += 1
+        got["sum"] += float(sims[:: max(1, count // 1024)].sum())
+        log(f"band {got['bands']}: {got['pairs'] / npairs * 100:.1f} % of pairs")
+
+    t2 = time.perf_counter()
+    ctx.all_pairs_stream(fn, a.band_pairs)
+    stream_s = time.perf_counter() - t2
+    st = ctx.stats()
+    valu_tops = st.valu_ops / (st.kernel_ms * 1e-3) / 1e12 if st.kernel_ms > 0 else 0.0
+    res = {"metric": METRIC, "value": npairs / (stream_s + prep_s), "unit": "sample-pairs/s", "n_gpus": 1,
+           "steps": 1, "warmup": 0, "ms_per_step": (stream_s + prep_s) * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u32 fixed-point (FP64 embedding)",
+           "data": "synthetic (seeded generator; tree 1003, samples 2003, copy numbers 3003)",
+           "config": {"workload": "workload 3: 100,000 samples (log-uniform 50-20,000 OTUs) x 99,322-leaf tree, "
+                                  "streamed to host", "samples": n, "pairs": npairs, "binned": bool(a.bin),
+                      "variant": st.variant, "bands": st.bands, "band_pairs": a.band_pairs,
+                      "hipgraph": bool(a.graph)},
+           "prep_s": prep_s, "stream_s": stream_s, "pairs_streamed": got["pairs"],
+           "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                        "frac": valu_tops / VALU_PEAK_TOPS, "traffic": None, "kernel": "sm_persistent_kernel",
+                        "kernel_ms": st.kernel_ms, "pair_entries": st.pair_node_updates,
+                        "tune_ms": list(st.tune_ms)}}
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
+def run_precision(cms, a):
+    """Workload 2: 50,000 samples of the workload-1 generator. The walk with
+    abundances stored BF16 and F16 (per-sample power-of-two scale, FP32
+    propagation, Kahan-compensated FP32 total, FP64 final division): all
+    ~1.25e9 pairs timed, then max / RMS error vs the CPU FP64 oracle on random
+    pairs (the oracle runs here only as the checker)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    t0 = time.perf_counter()
+    tree, cp, counts, abd = cms.workload(2, samples=a.samples)
+    del counts
+    n = abd.n
+    npairs = n * (n - 1) // 2
+    log(f"workload 2: {n} samples, generated in {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(11)
+    m = a.error_pairs
+    ii = rng.integers(0, n, 2 * m)
+    jj = rng.integers(0, n, 2 * m)
+    keep = ii != jj
+    ii, jj = np.minimum(ii, jj)[keep][:m].astype(np.int32), np.maximum(ii, jj)[keep][:m].astype(np.int32)
+    # FP64 oracle over the sampled pairs, from the samples they touch
+    used = np.unique(np.concatenate([ii, jj]))
+    remap = np.full(n, -1, np.int64)
+    remap[used] = np.arange(len(used))
+    t1 = time.perf_counter()
+    dense = abd.subset(used).dense(tree.leaf_n)
+    f64 = orc.pairs_f64(tree, dense, remap[ii].astype(np.int32), remap[jj].astype(np.int32), threads=a.cpu_threads)
+    del dense
+    log(f"FP64 oracle on {len(ii)} pairs in {time.perf_counter() - t1:.1f} s")
+    runs = {}
+    for name, storage in (("bf16", cms.STORE_BF16), ("f16", cms.STORE_F16), ("f32", cms.STORE_F32)):
+        with cms.Context(0, kernel=cms.KERNEL_WALK, storage=storage) as ctx:
+            ctx.set_tree(tree)
+            ctx.set_samples_csr(abd)
+            sec = timed_steps(ctx, 1, 0)
+            st = ctx.stats()
+            sims = ctx.sim_pairs(ii, jj)
+        e = np.abs(sims.astype(np.float64) - f64)
+        runs[name] = {"ms_per_step": sec * 1e3, "pairs_per_s": npairs / sec, "kernel_ms": st.kernel_ms,
+                      "max_abs_err_vs_fp64": float(e.max()), "rms_err_vs_fp64": float(np.sqrt((e ** 2).mean()))}
+        log(f"walk {name}: {sec:.1f} s, max err {e.max():.3e}, rms {np.sqrt((e ** 2).mean()):.3e}")
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=a.bin, autotune=a.autotune) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        sec = timed_steps(ctx, 1, 1)
+        sims = ctx.sim_pairs(ii, jj)
+    e = np.abs(sims.astype(np.float64) - f64)
+    runs["sharedmin_u32"] = {"ms_per_step": sec * 1e3, "pairs_per_s": npairs / sec,
+                             "max_abs_err_vs_fp64": float(e.max()), "rms_err_vs_fp64": float(np.sqrt((e ** 2).mean()))}
+    log(f"sharedmin: {sec:.2f} s, max err {e.max():.3e}")
+    res = {"metric": "max/RMS abs error vs FP64 on random pairs, and pairs/s", "workload": "workload 2: 50,000 samples "
+           "x 99,322-leaf tree (~1.25e9 pairs)", "samples": n, "pairs": npairs, "error_pairs": int(len(ii)),
+           "runs": runs}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +243,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (traffic-counter reference)")
+    ap.add_argument("--bin", type=int, default=1, help="bin samples by sparsity (shared-minimum path)")
+    ap.add_argument("--autotune", type=int, default=1, help="autotune the persistent-kernel variant")
+    ap.add_argument("--variant", type=int, default=0, help="force a persistent-kernel variant (1..6)")
+    ap.add_argument("--workload", type=int, default=1, choices=[1, 2, 3],
+                    help="1: headline all-pairs; 2: precision (16-bit walk storage, 50,000 samples); "
+                         "3: batched stream (100,000 samples of uneven sparsity)")
+    ap.add_argument("--band-pairs", type=int, default=1 << 28, help="pairs per streamed band (workload 3)")
+    ap.add_argument("--graph", type=int, default=1, help="replay streamed bands from a hipGraph (workload 3)")
+    ap.add_argument("--error-pairs", type=int, default=1000000, help="random pairs checked vs FP64 (workload 2)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every GPU box here has one, and multi-GPU runs are not part of "
@@ -131,6 +263,10 @@ def main():
     cms = importlib.import_module("coordinated-meta-storms_amd")
     if cms.device_count() < 1:
         raise SystemExit("bench.py needs a HIP device: " + cms.last_error())
+    if a.workload == 3:
+        return run_batched(cms, a)
+    if a.workload == 2:
+        return run_precision(cms, a)
 
     t0 = time.perf_counter()
     tree, cp, counts, abd = cms.workload(1, samples=a.samples)
@@ -139,7 +275,7 @@ def main():
     log(f"workload 1: {n} samples, {tree.leaf_n} leaves, nnz {abd.rowptr[-1]}, "
         f"generated in {time.perf_counter() - t0:.1f} s")
     # headline: shared-minimum kernel
-    sm = cms.Context(0, kernel=cms.KERNEL_SHAREDMIN)
+    sm = cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=a.bin, autotune=a.autotune, variant=a.variant)
     sm.set_tree(tree)
     sm.set_samples_csr(abd)
     sec = timed_steps(sm, a.steps, a.warmup)
@@ -182,12 +318,14 @@ def main():
         "dtype": "u32 fixed-point (FP64 embedding)",
         "data": "synthetic (seeded generator; tree 1001, samples 2001, copy numbers 3001)",
         "config": {"workload": "workload 1: 10,000 samples x 99,322-leaf tree, all 49,995,000 pairs",
-                   "samples": n, "leaves": tree.leaf_n, "pairs": npairs, "kernel": "sharedmin"},
+                   "samples": n, "leaves": tree.leaf_n, "pairs": npairs, "kernel": "sharedmin",
+                   "binned": bool(a.bin), "variant": st.variant},
         "pair_node_updates_per_s": npairs * tree.order_n / sec,
         "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                     "frac": valu_tops / VALU_PEAK_TOPS, "traffic": None,
-                     "kernel": "sm_tile_kernel<0>", "kernel_ms": st.kernel_ms,
-                     "ops_per_pair_entry": 1.5, "pair_entries": st.pair_node_updates},
+                     "frac": valu_tops / VALU_PEAK_TOPS, "traffic": traffic_from_profiles(),
+                     "kernel": "sm_persistent_kernel", "kernel_ms": st.kernel_ms,
+                     "ops_per_pair_entry": 1.5, "pair_entries": st.pair_node_updates,
+                     "tune_ms": list(st.tune_ms)},
         "prep_ms": st.prep_ms,
         "plain_fp32_walk": walk,
     }

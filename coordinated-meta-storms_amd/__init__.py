@@ -32,6 +32,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 OK = 0
 E_INVAL, E_NOMEM, E_HIP, E_SCHEDULE, E_STATE, E_IO, E_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
 KERNEL_WALK, KERNEL_SHAREDMIN = 0, 1
+METRIC_WEIGHTED, METRIC_UNWEIGHTED = 0, 1
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 REG_SIZE = 150
 
@@ -55,13 +56,15 @@ _vp = C.c_void_p
 
 class Opts(C.Structure):
     _fields_ = [("kernel", C.c_int), ("storage", C.c_int), ("tile_a", C.c_int), ("tile_b", C.c_int),
-                ("chunk", C.c_int), ("use_graph", C.c_int), ("reserved", C.c_int * 8)]
+                ("chunk", C.c_int), ("use_graph", C.c_int), ("metric", C.c_int), ("bin", C.c_int),
+                ("autotune", C.c_int), ("variant", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("prep_ms", C.c_double), ("pair_node_updates", C.c_double),
                 ("dense_updates", C.c_double), ("valu_ops", C.c_double), ("bytes_moved", C.c_double),
-                ("tiles", C.c_int64), ("nslots", C.c_int), ("scale_bits", C.c_int), ("entries", C.c_int)]
+                ("tiles", C.c_int64), ("nslots", C.c_int), ("scale_bits", C.c_int), ("entries", C.c_int),
+                ("variant", C.c_int), ("tune_ms", C.c_double * 6), ("bands", C.c_int64)]
 
 
 class SynthOpts(C.Structure):
@@ -91,6 +94,9 @@ _sig("cms_copy_result", C.c_int, [_vp, _f32p, C.c_int64])
 _sig("cms_synchronize", C.c_int, [_vp])
 _sig("cms_device_count", C.c_int, [])
 _sig("cms_calibrate_copy", C.c_int, [_vp, C.c_int64, C.c_int, C.POINTER(C.c_double)])
+_BAND_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_float))
+_sig("cms_all_pairs_stream", C.c_int, [_vp, C.c_int64, _BAND_FN, _vp])
+_sig("cms_sample_order", C.c_int, [_vp, _i32p])
 _sig("cms_sim_pairs", C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f32p])
 _sig("cms_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f32p])
 _sig("cms_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
@@ -130,7 +136,8 @@ _sig("cms_version", C.c_char_p, [])
 EXPORTED = [
     "cms_opts_default", "cms_create", "cms_destroy", "cms_last_error", "cms_set_opts", "cms_set_tree",
     "cms_set_samples", "cms_set_samples_csr", "cms_all_pairs", "cms_all_pairs_device", "cms_copy_result",
-    "cms_synchronize", "cms_device_count", "cms_calibrate_copy", "cms_sim_pairs",
+    "cms_synchronize", "cms_device_count", "cms_calibrate_copy", "cms_all_pairs_stream",
+    "cms_sample_order", "cms_sim_pairs",
     "cms_tile", "cms_get_stats", "cms_prepare", "cms_validate_tree", "cms_db_open", "cms_db_close", "cms_db_leaf_n", "cms_db_order_n",
     "cms_db_has_tree", "cms_db_has_cp", "cms_db_order", "cms_db_leaf_name", "cms_table_load",
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
@@ -413,10 +420,13 @@ def workload(k: int, samples: int | None = None, cp_correct: bool = True):
 class Context:
     """One GPU context (cms_create). Not thread-safe."""
 
-    def __init__(self, device: int = 0, kernel: int = KERNEL_WALK, storage: int = STORE_F32):
+    def __init__(self, device: int = 0, kernel: int = KERNEL_WALK, storage: int = STORE_F32,
+                 metric: int = METRIC_WEIGHTED, **opts):
         o = Opts()
         lib.cms_opts_default(C.byref(o))
-        o.kernel, o.storage = kernel, storage
+        o.kernel, o.storage, o.metric = kernel, storage, metric
+        for k, v in opts.items():  # bin, autotune, variant, use_graph
+            setattr(o, k, int(v))
         h = _vp()
         _check(lib.cms_create(device, C.byref(o), C.byref(h)))
         self._h = h
@@ -439,10 +449,12 @@ class Context:
     def error(self) -> str:
         return last_error(self._h)
 
-    def set_opts(self, kernel: int = KERNEL_WALK, storage: int = STORE_F32):
+    def set_opts(self, kernel: int = KERNEL_WALK, storage: int = STORE_F32, metric: int = METRIC_WEIGHTED, **opts):
         o = Opts()
         lib.cms_opts_default(C.byref(o))
-        o.kernel, o.storage = kernel, storage
+        o.kernel, o.storage, o.metric = kernel, storage, metric
+        for k, v in opts.items():
+            setattr(o, k, int(v))
         _check(lib.cms_set_opts(self._h, C.byref(o)), self._h)
 
     def set_tree(self, tree: Tree):
@@ -470,6 +482,20 @@ class Context:
         """Enqueue all pairs into a device buffer (None: the context's own)."""
         _check(lib.cms_all_pairs_device(self._h, _vp(d_ptr) if d_ptr else None, _vp(stream) if stream else None),
                self._h)
+
+    def all_pairs_stream(self, fn, band_pairs: int = 0):
+        """Streamed all-pairs: fn(first, count, sims) per band, sims a float32
+        view valid during the call, indices in the stored-position packed
+        triangle (see sample_order)."""
+        def cb(_user, first, count, ptr):
+            fn(int(first), int(count), np.ctypeslib.as_array(ptr, shape=(int(count),)))
+        cfn = _BAND_FN(cb)
+        _check(lib.cms_all_pairs_stream(self._h, band_pairs, cfn, None), self._h)
+
+    def sample_order(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1), np.int32)
+        _check(lib.cms_sample_order(self._h, out), self._h)
+        return out[: self.n]
 
     def copy_result(self, count: int | None = None) -> np.ndarray:
         """Read back the context-owned result buffer (after all_pairs_device)."""

@@ -28,7 +28,8 @@ struct cms_ctx {
     WalkNode *d_nodes = nullptr;
     WalkChunk *d_chunks = nullptr;
     int32_t *d_leaf_lp_ptr = nullptr, *d_leaf_lp = nullptr, *d_leaf_uses = nullptr;
-    EmbedNode *d_embnodes = nullptr;
+    EmbedNode *d_embnodes = nullptr, *d_embnodes_u = nullptr;
+    int64_t nrows_u = 0;
     int32_t *d_seg_first = nullptr, *d_seg_last = nullptr, *d_top_nodes = nullptr, *d_top_src = nullptr;
     int64_t nrows = 0;
 
@@ -45,6 +46,19 @@ struct cms_ctx {
     size_t cap_ab = 0, cap_scale = 0, cap_inv = 0;
 
     bool sm_ready = false;
+    int sm_metric = -1;
+    int64_t sm_rows = 0;
+    bool ab_binned = false;          // walk layout holds samples at stored (binned) positions
+    bool sm_binned = false;          // the shared-minimum layout is binned
+    std::vector<int32_t> h_perm;     // stored position -> sample (binned layout)
+    int32_t *d_perm = nullptr, *d_pos = nullptr;
+    unsigned long long *d_work = nullptr;  // persistent-kernel tile counter
+    int variant = -1;                // tuned persistent-kernel variant
+    int grid_waves = 0;              // persistent grid size in waves
+    hipStream_t copy_stream = nullptr;
+    hipGraphExec_t band_exec = nullptr;
+    unsigned long long *d_colsum = nullptr;
+    size_t cap_colsum = 0;
     uint32_t *d_Y = nullptr, *d_bm = nullptr;
     double *d_mass = nullptr;
     double *d_xroot = nullptr;
@@ -114,10 +128,15 @@ void drop_layouts(cms_ctx *c) {
     dfree(c->d_bm);
     dfree(c->d_mass);
     dfree(c->d_xroot);
-    c->cap_xroot = 0;
+    dfree(c->d_colsum);
+    c->cap_xroot = c->cap_colsum = 0;
     c->cap_Y = c->cap_bm = c->cap_mass = 0;
     c->sm_ready = false;
 }
+
+// The shared-minimum kernels serve CMS_KERNEL_SHAREDMIN and every
+// unweighted request (the walk computes the weighted metric only).
+bool use_sm(const cms_ctx *c) { return c->opts.kernel == CMS_KERNEL_SHAREDMIN || c->opts.metric == CMS_METRIC_UNWEIGHTED; }
 
 int check_ready(cms_ctx *c) {
     if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
@@ -127,9 +146,10 @@ int check_ready(cms_ctx *c) {
     return CMS_OK;
 }
 
-// Block-leaf-major walk layout of the uploaded CSR samples (zeros elsewhere).
-int ensure_walk_layout(cms_ctx *c, int storage) {
-    if (c->ab_storage == storage) return CMS_OK;
+// Block-leaf-major walk layout of the uploaded CSR samples (zeros elsewhere),
+// samples at their stored positions when `binned` (shared-minimum only).
+int ensure_walk_layout(cms_ctx *c, int storage, bool binned = false) {
+    if (c->ab_storage == storage && c->ab_binned == binned) return CMS_OK;
     c->ab_storage = -1;
     const int64_t nlp = (int64_t)c->tree.leaf_stream.size();
     const int64_t nblk = (c->n + kWalkBlock - 1) / kWalkBlock;
@@ -143,8 +163,9 @@ int ensure_walk_layout(cms_ctx *c, int storage) {
         HIPCALL(c, launch_sample_scale(c->d_rowptr, c->d_val, c->n, storage, c->d_scale, c->d_inv, c->stream));
     }
     HIPCALL(c, launch_walk_scatter(c->d_rowptr, c->d_col, c->d_val, c->n, c->d_leaf_lp_ptr, c->d_leaf_lp, nlp,
-                                   storage, c->d_scale, c->d_ab, c->stream));
+                                   storage, c->d_scale, binned ? c->d_pos : nullptr, c->d_ab, c->stream));
     c->ab_storage = storage;
+    c->ab_binned = binned;
     return CMS_OK;
 }
 
@@ -174,42 +195,55 @@ int sharedmin_supported(cms_ctx *c) {
     return CMS_OK;
 }
 
-// Fixed-point embedding Y and the per-block entry bitmaps.
+// Largest fixed-point scale K with max column sum * 2^K + rounding below 2^32.
+int choose_scale_bits(double max_colsum, int64_t nrows) {
+    if (!(max_colsum > 0)) return 24;
+    const double room = 4294967295.0 - (double)nrows * 0.5 - 64.0;
+    int K = (int)std::floor(std::log2(room / max_colsum));
+    while (K > -60 && std::ldexp(max_colsum, K) + (double)nrows * 0.5 >= 4294967295.0) K--;
+    return std::min(K, 60);
+}
+
+// Fixed-point embedding Y, the per-block entry bitmaps and per-sample column
+// sums, for the context's metric.
 int ensure_sharedmin(cms_ctx *c) {
-    if (c->sm_ready) return CMS_OK;
+    const int metric = c->opts.metric;
+    if (c->sm_ready && c->sm_metric == metric && c->sm_binned == (c->opts.bin != 0)) return CMS_OK;
+    c->sm_ready = false;
     int rc = sharedmin_supported(c);
     if (rc) return rc;
-    rc = ensure_walk_layout(c, CMS_STORE_F32);  // the embedding reads the F32 walk layout
-    if (rc) return rc;
-    HIPCALL(c, dreserve(c->d_mass, c->cap_mass, (size_t)std::max(c->n, 1)));
-    HIPCALL(c, launch_sample_mass(c->d_rowptr, c->d_col, c->d_val, c->n, c->d_leaf_uses, c->d_mass, c->stream));
-    std::vector<double> mass(std::max(c->n, 1), 0.0);
-    if (c->n > 0)
-        HIPCALL(c, hipMemcpyAsync(mass.data(), c->d_mass, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
-    HIPCALL(c, hipStreamSynchronize(c->stream));
-    double maxm = 0.0;
-    for (int s = 0; s < c->n; s++) maxm = std::max(maxm, mass[s]);
-    // A column sum of y equals the sample's consumed leaf mass; rounding adds
-    // at most 1/2 per entry. Keep the largest column sum below 2^32.
-    int K = 24;
-    if (maxm > 0) {
-        const double room = 4294967295.0 - (double)c->nrows * 0.5 - 64.0;
-        K = (int)std::floor(std::log2(room / maxm));
-        while (K > -60 && std::ldexp(maxm, K) + (double)c->nrows * 0.5 >= 4294967295.0) K--;
-        K = std::min(K, 60);
+    const bool binned = c->opts.bin != 0;
+    if (binned) {
+        // stored order: densest samples first, so 32-sample blocks have
+        // similar sparsity and sparse blocks have small root-path unions
+        std::vector<int64_t> rp((size_t)c->n + 1);
+        if (c->n > 0) HIPCALL(c, hipMemcpy(rp.data(), c->d_rowptr, sizeof(int64_t) * (c->n + 1), hipMemcpyDeviceToHost));
+        c->h_perm.resize(c->n);
+        for (int s = 0; s < c->n; s++) c->h_perm[s] = s;
+        std::stable_sort(c->h_perm.begin(), c->h_perm.end(),
+                         [&](int32_t a, int32_t b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+        std::vector<int32_t> pos(c->n);
+        for (int p = 0; p < c->n; p++) pos[c->h_perm[p]] = p;
+        HIPCALL(c, dalloc(c->d_perm, (size_t)std::max(c->n, 1)));
+        HIPCALL(c, dalloc(c->d_pos, (size_t)std::max(c->n, 1)));
+        if (c->n > 0) {
+            HIPCALL(c, hipMemcpy(c->d_perm, c->h_perm.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
+            HIPCALL(c, hipMemcpy(c->d_pos, pos.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
+        }
     }
-    c->scale_bits = K;
+    rc = ensure_walk_layout(c, CMS_STORE_F32, binned);  // the embedding reads the F32 walk layout
+    if (rc) return rc;
+    c->sm_binned = binned;
+    const bool unw = metric == CMS_METRIC_UNWEIGHTED;
+    const int64_t nrows = unw ? c->nrows_u : c->nrows;
     c->npad = ((int64_t)c->n + 63) / 64 * 64;
-    c->nwords = (c->nrows + 31) / 32;
-    const int64_t nblk = c->npad / kSmBlock;
-    const size_t nbm = (size_t)std::max<int64_t>(nblk, 1) * std::max<int64_t>(c->nwords, 1);
-    HIPCALL(c, dreserve(c->d_Y, c->cap_Y, (size_t)std::max<int64_t>(c->nrows, 1) * std::max<int64_t>(c->npad, 1)));
-    HIPCALL(c, dreserve(c->d_bm, c->cap_bm, nbm));
-    HIPCALL(c, hipMemsetAsync(c->d_bm, 0, sizeof(uint32_t) * nbm, c->stream));
+    c->nwords = (nrows + 31) / 32;
     HIPCALL(c, dreserve(c->d_xroot, c->cap_xroot,
                         (size_t)std::max<size_t>(c->tree.seg_first.size(), 1) * std::max<int64_t>(c->npad, 1)));
+    HIPCALL(c, dreserve(c->d_colsum, c->cap_colsum, (size_t)std::max<int64_t>(c->npad, 1)));
+    HIPCALL(c, dreserve(c->d_mass, c->cap_mass, (size_t)std::max<int64_t>(c->npad, 1)));
     EmbedData ed{};
-    ed.nodes = c->d_embnodes;
+    ed.nodes = unw ? c->d_embnodes_u : c->d_embnodes;
     ed.order_n = c->tree.order_n;
     ed.nslots = std::max(c->tree.nslots, 1);
     ed.seg_first = c->d_seg_first;
@@ -223,11 +257,40 @@ int ensure_sharedmin(cms_ctx *c) {
     ed.n = c->n;
     ed.npad = c->npad;
     ed.nwords = c->nwords;
+    ed.xroot = c->d_xroot;
+    ed.unweighted = unw ? 1 : 0;
+    ed.colsum = c->d_colsum;
+    ed.colsum_f64 = c->d_mass;
+    if (unw) {
+        // column sums of y (shared weight bound) from a first, write-free pass
+        HIPCALL(c, hipMemsetAsync(c->d_mass, 0, sizeof(double) * std::max<int64_t>(c->npad, 1), c->stream));
+        ed.sums_only = 1;
+        HIPCALL(c, launch_embed(ed, c->stream));
+        ed.sums_only = 0;
+    } else {
+        // a column sum of y equals the sample's consumed leaf mass
+        HIPCALL(c, launch_sample_mass(c->d_rowptr, c->d_col, c->d_val, c->n, c->d_leaf_uses, c->d_mass, c->stream));
+    }
+    std::vector<double> mass(std::max(c->n, 1), 0.0);
+    if (c->n > 0)
+        HIPCALL(c, hipMemcpyAsync(mass.data(), c->d_mass, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    double maxm = 0.0;
+    for (int s = 0; s < c->n; s++) maxm = std::max(maxm, mass[s]);
+    const int K = choose_scale_bits(maxm, nrows);
+    c->scale_bits = K;
+    const int64_t nblk = c->npad / kSmBlock;
+    const size_t nbm = (size_t)std::max<int64_t>(nblk, 1) * std::max<int64_t>(c->nwords, 1);
+    HIPCALL(c, dreserve(c->d_Y, c->cap_Y, (size_t)std::max<int64_t>(nrows, 1) * std::max<int64_t>(c->npad, 1)));
+    HIPCALL(c, dreserve(c->d_bm, c->cap_bm, nbm));
+    HIPCALL(c, hipMemsetAsync(c->d_bm, 0, sizeof(uint32_t) * nbm, c->stream));
+    HIPCALL(c, hipMemsetAsync(c->d_colsum, 0, sizeof(unsigned long long) * std::max<int64_t>(c->npad, 1), c->stream));
     ed.scale = std::ldexp(1.0, K);
     ed.Y = c->d_Y;
     ed.bm = c->d_bm;
-    ed.xroot = c->d_xroot;
     HIPCALL(c, launch_embed(ed, c->stream));
+    c->sm_rows = nrows;
+    c->sm_metric = metric;
     c->sm_ready = true;
     return CMS_OK;
 }
@@ -236,20 +299,24 @@ SmData sm_data(const cms_ctx *c) {
     SmData d{};
     d.Y = c->d_Y;
     d.bm = c->d_bm;
-    d.nrows = c->nrows;
+    d.nrows = c->sm_rows;
     d.npad = c->npad;
     d.nwords = c->nwords;
     d.n = c->n;
     d.inv_scale = 1.0 / (100.0 * std::ldexp(1.0, c->scale_bits));
     d.counter = c->d_counter;
+    d.unweighted = c->sm_metric == CMS_METRIC_UNWEIGHTED ? 1 : 0;
+    d.colsum = c->d_colsum;
+    d.perm = c->sm_binned ? c->d_perm : nullptr;
+    d.pos = c->sm_binned ? c->d_pos : nullptr;
     return d;
 }
 
 int prepare(cms_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
-    if (c->opts.kernel == CMS_KERNEL_SHAREDMIN) {
-        if (c->sm_ready) return CMS_OK;
+    if (use_sm(c)) {
+        if (c->sm_ready && c->sm_metric == c->opts.metric && c->sm_binned == (c->opts.bin != 0)) return CMS_OK;
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
         rc = ensure_sharedmin(c);
     } else {
@@ -257,9 +324,9 @@ int prepare(cms_ctx *c) {
             return fail(c, CMS_E_UNSUPPORTED,
                         "schedule keeps " + std::to_string(c->tree.nslots) + " registers live; the walk kernel holds " +
                             std::to_string(walk_max_slots()) + " (re-order the tree, e.g. Sethi-Ullman)");
-        if (c->ab_storage == c->opts.storage) return CMS_OK;
+        if (c->ab_storage == c->opts.storage && !c->ab_binned) return CMS_OK;
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
-        rc = ensure_walk_layout(c, c->opts.storage);
+        rc = ensure_walk_layout(c, c->opts.storage, false);
     }
     if (rc) return rc;
     HIPCALL(c, hipEventRecord(c->evp1, c->stream));
@@ -338,6 +405,7 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_leaf_lp);
     dfree(c->d_leaf_uses);
     dfree(c->d_embnodes);
+    dfree(c->d_embnodes_u);
     dfree(c->d_seg_first);
     dfree(c->d_seg_last);
     dfree(c->d_top_nodes);
@@ -347,6 +415,11 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_val);
     dfree(c->d_out);
     dfree(c->d_counter);
+    dfree(c->d_work);
+    dfree(c->d_perm);
+    dfree(c->d_pos);
+    if (c->band_exec) (void)hipGraphExecDestroy(c->band_exec);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->evp0) (void)hipEventDestroy(c->evp0);
@@ -360,6 +433,7 @@ const char *cms_last_error(const cms_ctx *c) { return c ? c->err.c_str() : globa
 int cms_set_opts(cms_ctx *c, const cms_opts *o) {
     if (!c || !o) return fail(c, CMS_E_INVAL, "cms_set_opts: null argument");
     if (o->kernel != CMS_KERNEL_WALK && o->kernel != CMS_KERNEL_SHAREDMIN) return fail(c, CMS_E_INVAL, "unknown kernel");
+    if (o->metric != CMS_METRIC_WEIGHTED && o->metric != CMS_METRIC_UNWEIGHTED) return fail(c, CMS_E_INVAL, "unknown metric");
     if (o->storage < CMS_STORE_F32 || o->storage > CMS_STORE_F16) return fail(c, CMS_E_INVAL, "unknown storage");
     c->opts = *o;
     return CMS_OK;
@@ -399,7 +473,14 @@ int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const f
     std::vector<int32_t> export_seg(t.nodes.size(), -1);
     for (size_t k = 0; k < t.seg_first.size(); k++)
         if (t.seg_out[k]) export_seg[t.seg_last[k]] = (int32_t)k;
-    std::vector<EmbedNode> emb(t.nodes.size());
+    // unweighted form: every edge with w != 0 is a row, coefficient w
+    std::vector<int32_t> entrow_u(t.ent_beta.size(), -1);
+    int64_t nrows_u = 0;
+    for (size_t i = 0; i < t.nodes.size(); i++) {
+        if (t.nodes[i].w1 != 0.0f) entrow_u[2 * i] = (int32_t)nrows_u++;
+        if (t.nodes[i].w2 != 0.0f) entrow_u[2 * i + 1] = (int32_t)nrows_u++;
+    }
+    std::vector<EmbedNode> emb(t.nodes.size()), emb_u(t.nodes.size());
     for (size_t i = 0; i < t.nodes.size(); i++) {
         EmbedNode &e = emb[i];
         e.code = t.nodes[i].code;
@@ -412,8 +493,15 @@ int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const f
         e.pad1 = 0;
         e.beta1 = t.ent_beta[2 * i];
         e.beta2 = t.ent_beta[2 * i + 1];
+        EmbedNode &u = emb_u[i];
+        u = e;
+        u.row1 = entrow_u[2 * i];
+        u.row2 = entrow_u[2 * i + 1];
+        u.beta1 = (double)t.nodes[i].w1;
+        u.beta2 = (double)t.nodes[i].w2;
     }
     HIPCALL(c, dalloc(c->d_embnodes, std::max<size_t>(emb.size(), 1)));
+    HIPCALL(c, dalloc(c->d_embnodes_u, std::max<size_t>(emb_u.size(), 1)));
     auto upload_i32 = [&](int32_t *&d, const std::vector<int32_t> &v) -> hipError_t {
         hipError_t e = dalloc(d, std::max<size_t>(v.size(), 1));
         if (e == hipSuccess && !v.empty()) e = hipMemcpy(d, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice);
@@ -427,12 +515,14 @@ int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const f
         HIPCALL(c, hipMemcpy(c->d_nodes, t.nodes.data(), sizeof(WalkNode) * t.nodes.size(), hipMemcpyHostToDevice));
         HIPCALL(c, hipMemcpy(c->d_chunks, t.chunks.data(), sizeof(WalkChunk) * t.chunks.size(), hipMemcpyHostToDevice));
         HIPCALL(c, hipMemcpy(c->d_embnodes, emb.data(), sizeof(EmbedNode) * emb.size(), hipMemcpyHostToDevice));
+        HIPCALL(c, hipMemcpy(c->d_embnodes_u, emb_u.data(), sizeof(EmbedNode) * emb_u.size(), hipMemcpyHostToDevice));
     }
     HIPCALL(c, hipMemcpy(c->d_leaf_lp_ptr, ptr.data(), sizeof(int32_t) * ptr.size(), hipMemcpyHostToDevice));
     if (!lp.empty()) HIPCALL(c, hipMemcpy(c->d_leaf_lp, lp.data(), sizeof(int32_t) * lp.size(), hipMemcpyHostToDevice));
     HIPCALL(c, hipMemcpy(c->d_leaf_uses, uses.data(), sizeof(int32_t) * uses.size(), hipMemcpyHostToDevice));
     c->tree = std::move(t);
     c->nrows = nrows;
+    c->nrows_u = nrows_u;
     c->has_tree = true;
     c->stats.nslots = c->tree.nslots;
     c->stats.entries = (int)nrows;
@@ -492,6 +582,59 @@ int cms_set_samples(cms_ctx *c, int n, const float *abd) {
     return cms_set_samples_csr(c, n, rowptr.data(), col.data(), val.data());
 }
 
+}  // extern "C"
+
+namespace {
+
+// Persistent-kernel variant: forced by opts.variant, else the default, or the
+// fastest of all variants timed once on the first (heaviest) tiles when
+// opts.autotune is set. Tuning writes into `out` (later overwritten).
+int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, hipStream_t st) {
+    if (!c->d_work) HIPCALL(c, dalloc(c->d_work, 1));
+    if (c->grid_waves == 0) {
+        int cus = 0;
+        HIPCALL(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        c->grid_waves = std::max(cus, 1) * 32;  // 8 waves per SIMD
+    }
+    if (c->opts.variant >= 1 && c->opts.variant <= kSmVariants) {
+        c->variant = c->opts.variant - 1;
+        return CMS_OK;
+    }
+    if (c->variant >= 0) return CMS_OK;
+    if (!c->opts.autotune || ntiles <= 0) {
+        c->variant = 1;  // 16-entry LDS chunk, 4 waves per workgroup
+        return CMS_OK;
+    }
+    const int64_t sample = std::min<int64_t>(ntiles, 8192);
+    float best = 1e30f;
+    int best_v = 1;
+    for (int v = 0; v < kSmVariants; v++) {
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            HIPCALL(c, launch_zero_counter(c->d_work, st));
+            HIPCALL(c, hipEventRecord(c->evp0, st));
+            HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, 0, sample, c->d_work, out, 0, 0, c->grid_waves, st));
+            HIPCALL(c, hipEventRecord(c->evp1, st));
+            HIPCALL(c, hipEventSynchronize(c->evp1));
+            float ms = 0;
+            HIPCALL(c, hipEventElapsedTime(&ms, c->evp0, c->evp1));
+            ms_min = std::min(ms_min, ms);
+        }
+        c->stats.tune_ms[v] = ms_min;
+        if (ms_min < best) {
+            best = ms_min;
+            best_v = v;
+        }
+    }
+    c->variant = best_v;
+    c->prep_pending = false;  // tuning reused the prep events
+    return CMS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
     int rc = prepare(c);
     if (rc) return rc;
@@ -508,12 +651,17 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
     }
     const double pairs = (double)c->n * (c->n - 1) / 2.0;
     c->stats.dense_updates = pairs * c->tree.order_n;
-    HIPCALL(c, hipEventRecord(c->ev0, st));
-    if (c->opts.kernel == CMS_KERNEL_SHAREDMIN) {
-        HIPCALL(c, launch_zero_counter(c->d_counter, st));
-        HIPCALL(c, launch_sm_all_pairs(sm_data(c), d_sim, st));
+    if (!use_sm(c)) HIPCALL(c, hipEventRecord(c->ev0, st));
+    if (use_sm(c)) {
         const int64_t nb = (c->n + kSmBlock - 1) / kSmBlock;
-        c->stats.tiles = nb * (nb + 1) / 2;
+        const int64_t ntiles = nb * (nb + 1) / 2;
+        rc = pick_variant(c, nb, ntiles, d_sim, st);
+        if (rc) return rc;
+        HIPCALL(c, hipEventRecord(c->ev0, st));  // after any tuning launches
+        HIPCALL(c, launch_zero_counter(c->d_counter, st));
+        HIPCALL(c, launch_zero_counter(c->d_work, st));
+        HIPCALL(c, launch_sm_persistent(sm_data(c), c->variant, nb, 0, ntiles, c->d_work, d_sim, 0, 0, c->grid_waves, st));
+        c->stats.tiles = ntiles;
     } else {
         HIPCALL(c, launch_walk_all_pairs(walk_data(c), d_sim, st));
         const int64_t nb = (c->n + kWalkBlock - 1) / kWalkBlock;
@@ -622,7 +770,7 @@ int cms_sim_pairs(cms_ctx *c, int64_t n_pairs, const int32_t *row, const int32_t
     if (rc) return rc;
     HIPCALL(c, hipMemcpyAsync(d_r, row, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
     HIPCALL(c, hipMemcpyAsync(d_c, col, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
-    if (c->opts.kernel == CMS_KERNEL_SHAREDMIN)
+    if (use_sm(c))
         HIPCALL(c, launch_sm_pairs(sm_data(c), n_pairs, d_r, d_c, c->d_out, c->stream));
     else
         HIPCALL(c, launch_walk_pairs(walk_data(c), n_pairs, d_r, d_c, c->d_out, c->stream));
@@ -643,12 +791,154 @@ int cms_tile(cms_ctx *c, int r0, int nr, int c0, int nc, float *out) {
     if (!out) return fail(c, CMS_E_INVAL, "cms_tile: null output");
     rc = ensure_out(c, cnt);
     if (rc) return rc;
-    if (c->opts.kernel == CMS_KERNEL_SHAREDMIN)
+    if (use_sm(c) && c->sm_binned) {
+        // stored order differs from the caller's: go through the pair list
+        std::vector<int32_t> pr(cnt), pc(cnt);
+        for (int i = 0; i < nr; i++)
+            for (int j = 0; j < nc; j++) {
+                pr[(size_t)i * nc + j] = r0 + i;
+                pc[(size_t)i * nc + j] = c0 + j;
+            }
+        rc = cms_sim_pairs(c, (int64_t)cnt, pr.data(), pc.data(), out);
+        if (rc) return rc;
+        for (int i = 0; i < nr; i++)
+            for (int j = 0; j < nc; j++)
+                if (r0 + i == c0 + j) out[(size_t)i * nc + j] = 1.0f;
+        return CMS_OK;
+    }
+    if (use_sm(c))
         HIPCALL(c, launch_sm_rect(sm_data(c), r0, nr, c0, nc, c->d_out, c->stream));
     else
         HIPCALL(c, launch_walk_rect(walk_data(c), r0, nr, c0, nc, c->d_out, c->stream));
     HIPCALL(c, hipMemcpyAsync(out, c->d_out, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->stream));
     HIPCALL(c, hipStreamSynchronize(c->stream));
+    return CMS_OK;
+}
+
+int cms_sample_order(cms_ctx *c, int32_t *order) {
+    int rc = prepare(c);
+    if (rc) return rc;
+    if (!order && c->n > 0) return fail(c, CMS_E_INVAL, "cms_sample_order: null output");
+    for (int p = 0; p < c->n; p++) order[p] = (use_sm(c) && c->sm_binned) ? c->h_perm[p] : p;
+    return CMS_OK;
+}
+
+int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *user) {
+    int rc = prepare(c);
+    if (rc) return rc;
+    if (!use_sm(c)) return fail(c, CMS_E_UNSUPPORTED, "streamed all-pairs runs on the shared-minimum kernel");
+    if (!fn) return fail(c, CMS_E_INVAL, "cms_all_pairs_stream: null callback");
+    const int64_t n = c->n, nb = (n + kSmBlock - 1) / kSmBlock;
+    if (n < 2) return CMS_OK;
+    if (band_pairs <= 0) band_pairs = (int64_t)1 << 28;
+    // pairs with stored row index in [0, r) = packed index of (r, r+1)
+    auto row_start = [&](int64_t r) { return r * n - r * (r + 1) / 2; };
+    auto tile_start = [&](int64_t rb) { return rb * nb - rb * (rb - 1) / 2; };
+    // bands of whole row blocks, each at least one block, about band_pairs pairs
+    std::vector<int64_t> cuts{0};
+    while (cuts.back() < nb) {
+        int64_t b = cuts.back() + 1;
+        while (b < nb && row_start(std::min(b * kSmBlock, n)) - row_start(cuts.back() * kSmBlock) < band_pairs) b++;
+        cuts.push_back(b);
+    }
+    int64_t cap = 0;
+    for (size_t k = 0; k + 1 < cuts.size(); k++)
+        cap = std::max(cap, row_start(std::min(cuts[k + 1] * kSmBlock, n)) - row_start(cuts[k] * kSmBlock));
+    float *dbuf[2] = {nullptr, nullptr}, *hbuf[2] = {nullptr, nullptr};
+    hipEvent_t done_k[2] = {nullptr, nullptr}, done_c[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        for (int i = 0; i < 2; i++) {
+            if (dbuf[i]) (void)hipFree(dbuf[i]);
+            if (hbuf[i]) (void)hipHostFree(hbuf[i]);
+            if (done_k[i]) (void)hipEventDestroy(done_k[i]);
+            if (done_c[i]) (void)hipEventDestroy(done_c[i]);
+        }
+    };
+#define STREAMCALL(expr)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            cleanup();                                                                          \
+            return fail(c, e_ == hipErrorOutOfMemory ? CMS_E_NOMEM : CMS_E_HIP,                 \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+        }                                                                                       \
+    } while (0)
+    for (int i = 0; i < 2; i++) {
+        STREAMCALL(hipMalloc((void **)&dbuf[i], sizeof(float) * cap));
+        STREAMCALL(hipHostMalloc((void **)&hbuf[i], sizeof(float) * cap, hipHostMallocDefault));
+        STREAMCALL(hipEventCreateWithFlags(&done_k[i], hipEventDisableTiming));
+        STREAMCALL(hipEventCreateWithFlags(&done_c[i], hipEventDisableTiming));
+    }
+    if (!c->copy_stream) STREAMCALL(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    rc = pick_variant(c, nb, nb * (nb + 1) / 2, dbuf[0], c->stream);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    const SmData d = sm_data(c);
+    const int nbands = (int)cuts.size() - 1;
+    std::vector<int64_t> first(nbands), count(nbands);
+    STREAMCALL(hipEventRecord(c->ev0, c->stream));
+    STREAMCALL(launch_zero_counter(c->d_counter, c->stream));
+    for (int k = 0; k <= nbands; k++) {
+        if (k < nbands) {
+            const int b = k & 1;
+            const int64_t r0 = cuts[k] * kSmBlock, r1 = std::min(cuts[k + 1] * kSmBlock, n);
+            first[k] = row_start(r0);
+            count[k] = row_start(r1) - first[k];
+            const int64_t t0 = tile_start(cuts[k]), t1 = tile_start(cuts[k + 1]);
+            if (k >= 2) STREAMCALL(hipStreamWaitEvent(c->stream, done_c[b], 0));  // device buffer free again
+            if (c->opts.use_graph) {
+                // capture this band's launches and replay them as a graph; later
+                // bands update the instantiated graph in place
+                hipGraph_t g = nullptr;
+                STREAMCALL(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                hipError_t e1 = launch_zero_counter(c->d_work, c->stream);
+                hipError_t e2 = launch_sm_persistent(d, c->variant, nb, t0, t1 - t0, c->d_work, dbuf[b], 1, first[k],
+                                                     c->grid_waves, c->stream);
+                STREAMCALL(hipStreamEndCapture(c->stream, &g));
+                STREAMCALL(e1);
+                STREAMCALL(e2);
+                bool updated = false;
+                if (c->band_exec) {
+                    hipGraphExecUpdateResult res;
+                    hipGraphNode_t err_node = nullptr;
+                    updated = hipGraphExecUpdate(c->band_exec, g, &err_node, &res) == hipSuccess &&
+                              res == hipGraphExecUpdateSuccess;
+                    if (!updated) {
+                        (void)hipGraphExecDestroy(c->band_exec);
+                        c->band_exec = nullptr;
+                    }
+                }
+                if (!c->band_exec) STREAMCALL(hipGraphInstantiate(&c->band_exec, g, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(g);
+                STREAMCALL(hipGraphLaunch(c->band_exec, c->stream));
+            } else {
+                STREAMCALL(launch_zero_counter(c->d_work, c->stream));
+                STREAMCALL(launch_sm_persistent(d, c->variant, nb, t0, t1 - t0, c->d_work, dbuf[b], 1, first[k],
+                                                c->grid_waves, c->stream));
+            }
+            STREAMCALL(hipEventRecord(done_k[b], c->stream));
+            STREAMCALL(hipStreamWaitEvent(c->copy_stream, done_k[b], 0));
+            STREAMCALL(hipMemcpyAsync(hbuf[b], dbuf[b], sizeof(float) * count[k], hipMemcpyDeviceToHost,
+                                      c->copy_stream));
+            STREAMCALL(hipEventRecord(done_c[b], c->copy_stream));
+        }
+        if (k >= 1) {
+            const int pb = (k - 1) & 1;
+            STREAMCALL(hipEventSynchronize(done_c[pb]));
+            fn(user, first[k - 1], count[k - 1], hbuf[pb]);
+        }
+    }
+    STREAMCALL(hipEventRecord(c->ev1, c->stream));
+    STREAMCALL(hipStreamSynchronize(c->stream));
+    STREAMCALL(hipStreamSynchronize(c->copy_stream));
+#undef STREAMCALL
+    cleanup();
+    c->stats.tiles = nb * (nb + 1) / 2;
+    c->stats.bands = nbands;
+    c->stats.dense_updates = (double)n * (n - 1) / 2.0 * c->tree.order_n;
+    c->kern_pending = true;
     return CMS_OK;
 }
 
@@ -659,7 +949,7 @@ int cms_get_stats(cms_ctx *c, cms_stats *s) {
         HIPCALL(c, hipEventSynchronize(c->ev1));
         HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         c->stats.kernel_ms = ms;
-        if (c->opts.kernel == CMS_KERNEL_SHAREDMIN) {
+        if (use_sm(c)) {
             unsigned long long rows = 0;
             HIPCALL(c, hipMemcpy(&rows, c->d_counter, sizeof rows, hipMemcpyDeviceToHost));
             // each visited entry is one min+add for all 32 x 32 pairs of the tile
@@ -678,6 +968,7 @@ int cms_get_stats(cms_ctx *c, cms_stats *s) {
     }
     *s = c->stats;
     s->scale_bits = c->scale_bits;
+    s->variant = c->variant + 1;
     return CMS_OK;
 }
 

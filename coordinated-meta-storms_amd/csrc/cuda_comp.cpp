@@ -30,7 +30,7 @@ namespace {
     printf("\t  -o Output file\n");
     printf("\t  -d Output format, distance (T) or similarity (F), default is T\n");
     printf("\t[Other options]\n");
-    printf("\t  -M (upper) Distance Metric, 0: Meta-Storms, default is 0\n");
+    printf("\t  -M (upper) Distance Metric, 0: Meta-Storms; 1: Meta-Storms-unweighted, default is 0\n");
     printf("\t  -r rRNA copy number correction, T(rue) or F(alse), default is T\n");
     printf("\t  -t Number of host threads for loading, default is auto\n");
     printf("\t  --kernel walk|sharedmin  GPU kernel (walk is bit-identical to comp -M 0)\n");
@@ -100,7 +100,9 @@ int main(int argc, char **argv) {
         i += 2;
     }
     if (table_path.empty()) die("an OTU table (-T) is required");
-    if (metric != 0) die("only -M 0 (weighted Meta-Storms) runs on the GPU in this build");
+    if (metric != 0 && metric != 1)
+        die("only -M 0 (Meta-Storms) and -M 1 (unweighted Meta-Storms) run on the GPU in this build");
+    opts.metric = metric;  // -M 1 runs on the shared-minimum kernels
 
     auto t0 = std::chrono::steady_clock::now();
     cms_db *db = nullptr;

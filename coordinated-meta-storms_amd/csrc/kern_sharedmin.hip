@@ -27,6 +27,8 @@
 // deterministic; the only error is the rounding of y (<= 2^-(K+1) per entry).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace cms {
@@ -100,12 +102,19 @@ struct BitmapWriter {
     }
 };
 
+// UNW: unweighted form (comp.h:378-446): x is presence (leaf a > 0, parent =
+// either child present) and the edge coefficient is w, so the pair sum is
+// the weight shared by both samples. SUMS: only accumulate each sample's
+// FP64 column sum of y (first pass, to choose the fixed-point scale).
+template <bool UNW, bool SUMS>
 __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restrict__ nodes,
                                                        const int32_t *__restrict__ seg_first,
                                                        const int32_t *__restrict__ seg_last, int order_n,
                                                        const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
                                                        int64_t nwords, double scale, uint32_t *__restrict__ Y,
-                                                       uint32_t *__restrict__ bm, double *__restrict__ xroot) {
+                                                       uint32_t *__restrict__ bm, double *__restrict__ xroot,
+                                                       unsigned long long *__restrict__ colsum,
+                                                       double *__restrict__ colsum_f64) {
     extern __shared__ double eslots[];  // nslots * 64
     __shared__ float stage[kEmbedLeafChunk * 64];
     __shared__ EmbedNode snodes[kEmbedNodeChunk];
@@ -116,6 +125,8 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
     const int64_t blk0 = (int64_t)blockIdx.x * 2;  // 32-sample blocks of this wave
     const int first = seg_first[blockIdx.y], last = seg_last[blockIdx.y];
     BitmapWriter bw;
+    unsigned long long qsum = 0;
+    double ysum = 0.0;
 
     for (int base = first; base <= last; base += kEmbedNodeChunk) {
         const int cnt = min(kEmbedNodeChunk, last + 1 - base);
@@ -144,24 +155,32 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
             double x[2];
             if (code & kC1Leaf) {
                 x[0] = (double)stage[lp * 64 + lane];
+                if (UNW) x[0] = x[0] > 0.0 ? 1.0 : 0.0;
                 lp++;
             } else {
                 x[0] = eslots[(code & 0xffu) * 64 + lane];
             }
-            if (code & kC2Leaf)
+            if (code & kC2Leaf) {
                 x[1] = (double)stage[lp * 64 + lane];
-            else
+                if (UNW) x[1] = x[1] > 0.0 ? 1.0 : 0.0;
+            } else {
                 x[1] = eslots[((code >> 8) & 0xffu) * 64 + lane];
+            }
             const double betas[2] = {snodes[k].beta1, snodes[k].beta2};
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 if (rows[j] < 0) continue;
+                if (SUMS) {
+                    ysum += betas[j] * x[j];
+                    continue;
+                }
                 const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
                 Y[(int64_t)rows[j] * npad + s] = q;
+                qsum += q;
                 bw.add(rows[j], q, bm, blk0, nwords, lane);
             }
             if (code & kDstLive) {
-                const double xp = (double)snodes[k].w1 * x[0] + (double)snodes[k].w2 * x[1];
+                const double xp = UNW ? fmax(x[0], x[1]) : (double)snodes[k].w1 * x[0] + (double)snodes[k].w2 * x[1];
                 if (exp_seg >= 0)
                     xroot[(int64_t)exp_seg * npad + s] = xp;  // consumed by a top node
                 else
@@ -169,15 +188,23 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
             }
         }
     }
+    if (SUMS) {
+        if (valid) atomicAdd(colsum_f64 + s, ysum);
+        return;
+    }
     bw.flush(bm, blk0, nwords, lane);
+    if (valid && qsum) atomicAdd(colsum + s, qsum);
 }
 
+template <bool UNW, bool SUMS>
 __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restrict__ nodes,
                                                        const int32_t *__restrict__ top_nodes,
                                                        const int32_t *__restrict__ top_src, int ntop,
                                                        const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
                                                        int64_t nwords, double scale, uint32_t *__restrict__ Y,
-                                                       uint32_t *__restrict__ bm, const double *__restrict__ xroot) {
+                                                       uint32_t *__restrict__ bm, const double *__restrict__ xroot,
+                                                       unsigned long long *__restrict__ colsum,
+                                                       double *__restrict__ colsum_f64) {
     extern __shared__ double eslots[];
     const int lane = threadIdx.x;
     const int64_t s = (int64_t)blockIdx.x * 64 + lane;
@@ -185,6 +212,8 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
     const float *abs_ = ab + (valid ? (s >> 3) * nlp * 8 + (s & 7) : 0);
     const int64_t blk0 = (int64_t)blockIdx.x * 2;
     BitmapWriter bw;
+    unsigned long long qsum = 0;
+    double ysum = 0.0;
     for (int t = 0; t < ntop; t++) {
         const int i = top_nodes[t];
         const EmbedNode nd = nodes[i];
@@ -197,6 +226,7 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
             const bool leaf = j == 0 ? (code & kC1Leaf) : (code & kC2Leaf);
             if (leaf) {
                 x[j] = valid ? (double)abs_[lp * 8] : 0.0;
+                if (UNW) x[j] = x[j] > 0.0 ? 1.0 : 0.0;
                 lp++;
             } else if (src[j] >= 0) {
                 x[j] = xroot[(int64_t)src[j] * npad + s];
@@ -209,77 +239,59 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             if (rows[j] < 0) continue;
+            if (SUMS) {
+                ysum += betas[j] * x[j];
+                continue;
+            }
             const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
             Y[(int64_t)rows[j] * npad + s] = q;
+            qsum += q;
             bw.add(rows[j], q, bm, blk0, nwords, lane);
         }
         if (code & kDstLive)
-            eslots[((code >> 16) & 0xffu) * 64 + lane] = (double)nd.w1 * x[0] + (double)nd.w2 * x[1];
+            eslots[((code >> 16) & 0xffu) * 64 + lane] =
+                UNW ? fmax(x[0], x[1]) : (double)nd.w1 * x[0] + (double)nd.w2 * x[1];
         __builtin_amdgcn_wave_barrier();
     }
+    if (SUMS) {
+        if (valid) atomicAdd(colsum_f64 + s, ysum);
+        return;
+    }
     bw.flush(bm, blk0, nwords, lane);
+    if (valid && qsum) atomicAdd(colsum + s, qsum);
 }
 
 // ------------------------------------------------------------ pair tiles
 
 // One wave = 32 x 32 pairs, lane (ty, tx) = 4 rows x 4 columns.
-// MODE 0: all pairs (aligned upper-triangle block tiles, packed output).
-// MODE 1: rectangle with arbitrary sample offsets (row-major, diagonal = 1).
-template <int MODE>
-__global__ __launch_bounds__(64 * kSmWaves) void sm_tile_kernel(SmData d, int64_t ntiles, int64_t nb, int r0, int nr,
-                                                                 int c0, int nc, float *__restrict__ out) {
-    __shared__ uint16_t s_list[kSmWaves][64 * 32];
-    __shared__ uint32_t s_stage[kSmWaves][kSmChunk * 64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t tile = sm_xcd_remap(blockIdx.x, gridDim.x) * kSmWaves + wave;
-    if (tile >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
-    int64_t abase, bbase, alim, blim;
-    const uint32_t *A0, *A1 = nullptr, *B0, *B1 = nullptr;
-    const int64_t nblk = d.npad / kSmBlock;
-    if constexpr (MODE == 0) {
-        int64_t ra, cb;
-        sm_decode_upper(tile, nb, ra, cb);
-        abase = ra * kSmBlock;
-        bbase = cb * kSmBlock;
-        alim = blim = d.n;
-        A0 = d.bm + ra * d.nwords;
-        B0 = d.bm + cb * d.nwords;
-    } else {
-        const int64_t tr = tile / nb, tc = tile % nb;
-        abase = r0 + tr * kSmBlock;
-        bbase = c0 + tc * kSmBlock;
-        alim = r0 + nr;
-        blim = c0 + nc;
-        const int64_t ba = abase / kSmBlock, bb = bbase / kSmBlock;
-        A0 = d.bm + ba * d.nwords;
-        if ((abase % kSmBlock) && ba + 1 < nblk) A1 = d.bm + (ba + 1) * d.nwords;
-        B0 = d.bm + bb * d.nwords;
-        if ((bbase % kSmBlock) && bb + 1 < nblk) B1 = d.bm + (bb + 1) * d.nwords;
-    }
+//
+// Visit the entries set in both blocks' bitmaps (OR of two blocks when a
+// side is not block-aligned): per 64 bitmap words, compact the set bits into
+// an LDS list, stage their 2 x 32 Y values in LDS CHUNK rows at a time, and
+// accumulate min/add into the lane's 4 x 4 registers. Returns the number of
+// entries visited.
+template <int CHUNK>
+__device__ __forceinline__ uint32_t sm_tile_accumulate(const SmData &d, const uint32_t *A0, const uint32_t *A1,
+                                                       const uint32_t *B0, const uint32_t *B1, int64_t abase,
+                                                       int64_t bbase, uint16_t *L, uint32_t *S, int lane,
+                                                       uint32_t (&acc)[4][4]) {
     const int ty = lane >> 3, tx = lane & 7;
     // Column this lane stages: lanes 0..31 the row block, 32..63 the column block.
     const int64_t my_s = (lane < 32) ? abase + lane : bbase + (lane - 32);
     const bool s_ok = my_s < d.npad;
     const uint32_t *Ycol = d.Y + (s_ok ? my_s : 0);
-
-    uint32_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0u;
-
-    uint16_t *L = s_list[wave];
-    uint32_t *S = s_stage[wave];
     uint32_t visited = 0;
     for (int64_t wb = 0; wb < d.nwords; wb += 64) {
         const int64_t wi = wb + lane;
         uint32_t w = 0u;
         if (wi < d.nwords) {
             uint32_t a = A0[wi], b = B0[wi];
-            if constexpr (MODE == 1) {
-                if (A1) a |= A1[wi];
-                if (B1) b |= B1[wi];
-            }
+            if (A1) a |= A1[wi];
+            if (B1) b |= B1[wi];
             w = a & b;
         }
         const int cnt = __popc(w);
@@ -303,20 +315,20 @@ __global__ __launch_bounds__(64 * kSmWaves) void sm_tile_kernel(SmData d, int64_
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const uint32_t *Ywb = Ycol + wb * 32 * d.npad;
-        for (int j = 0; j < tot; j += kSmChunk) {
-            uint32_t v[kSmChunk];
+        for (int j = 0; j < tot; j += CHUNK) {
+            uint32_t v[CHUNK];
 #pragma unroll
-            for (int k = 0; k < kSmChunk; k++) {
+            for (int k = 0; k < CHUNK; k++) {
                 v[k] = 0u;
                 if (j + k < tot && s_ok) v[k] = Ywb[(int64_t)L[j + k] * d.npad];
             }
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int k = 0; k < kSmChunk; k++) S[k * 64 + lane] = v[k];
+            for (int k = 0; k < CHUNK; k++) S[k * 64 + lane] = v[k];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int kn = min(kSmChunk, tot - j);
+            const int kn = min(CHUNK, tot - j);
             for (int k = 0; k < kn; k += 2) {
                 const uint4 a0 = *reinterpret_cast<const uint4 *>(S + k * 64 + ty * 4);
                 const uint4 b0 = *reinterpret_cast<const uint4 *>(S + k * 64 + 32 + tx * 4);
@@ -334,31 +346,116 @@ __global__ __launch_bounds__(64 * kSmWaves) void sm_tile_kernel(SmData d, int64_
             __builtin_amdgcn_wave_barrier();
         }
     }
-    if (d.counter && lane == 0) atomicAdd(d.counter, (unsigned long long)visited);
+    return visited;
+}
+
+// Similarity of the pair at stored positions (pa, pb) from its shared sum.
+__device__ __forceinline__ float sm_finish(const SmData &d, uint32_t acc, int64_t pa, int64_t pb) {
+    double t;
+    if (d.unweighted) {
+        // shared weight / (weight of a + weight of b - shared weight)
+        const unsigned long long den = d.colsum[pa] + d.colsum[pb] - acc;
+        t = den ? (double)acc / (double)den : __builtin_nan("");
+    } else {
+        t = (double)acc * d.inv_scale;
+    }
+    t = t > 1.0 ? 1.0 : t;
+    return (float)t;
+}
+
+// Rectangle with arbitrary (unpermuted) sample offsets: rows [r0, r0+nr) x
+// cols [c0, c0+nc), row-major output, diagonal = 1.
+__global__ __launch_bounds__(64 * kSmWaves) void sm_rect_kernel(SmData d, int64_t ntiles, int64_t nbc, int r0, int nr,
+                                                                 int c0, int nc, float *__restrict__ out) {
+    __shared__ uint16_t s_list[kSmWaves][64 * 32];
+    __shared__ uint32_t s_stage[kSmWaves][kSmChunk * 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t tile = sm_xcd_remap(blockIdx.x, gridDim.x) * kSmWaves + wave;
+    if (tile >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
+    const int64_t nblk = d.npad / kSmBlock;
+    const int64_t abase = r0 + (tile / nbc) * kSmBlock, bbase = c0 + (tile % nbc) * kSmBlock;
+    const int64_t ba = abase / kSmBlock, bb = bbase / kSmBlock;
+    const uint32_t *A0 = d.bm + ba * d.nwords, *B0 = d.bm + bb * d.nwords;
+    const uint32_t *A1 = ((abase % kSmBlock) && ba + 1 < nblk) ? d.bm + (ba + 1) * d.nwords : nullptr;
+    const uint32_t *B1 = ((bbase % kSmBlock) && bb + 1 < nblk) ? d.bm + (bb + 1) * d.nwords : nullptr;
+    uint32_t acc[4][4];
+    sm_tile_accumulate<kSmChunk>(d, A0, A1, B0, B1, abase, bbase, s_list[wave], s_stage[wave], lane, acc);
+    const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
-    for (int ii = 0; ii < 4; ii++) {
+    for (int ii = 0; ii < 4; ii++)
 #pragma unroll
         for (int jj = 0; jj < 4; jj++) {
             const int64_t row = abase + ty * 4 + ii, col = bbase + tx * 4 + jj;
-            double t = (double)acc[ii][jj] * d.inv_scale;
-            t = t > 1.0 ? 1.0 : t;
-            const float sim = (float)t;
-            if constexpr (MODE == 0) {
-                if (row < col && col < d.n) out[row * (int64_t)d.n + col - (row + 1) * (row + 2) / 2] = sim;
-            } else {
-                if (row < alim && col < blim) out[(row - r0) * (int64_t)nc + (col - c0)] = (row == col) ? 1.0f : sim;
-            }
+            if (row < r0 + nr && col < c0 + nc)
+                out[(row - r0) * (int64_t)nc + (col - c0)] =
+                    (row == col) ? 1.0f : sm_finish(d, acc[ii][jj], row, col);
         }
-    }
 }
 
-// Explicit pair list: lane = pair; 32-entry groups skipped when no lane's two
-// samples share mass in them.
+// All pairs, persistent: a fixed grid of waves pulls upper-triangle block
+// tiles [t_begin, t_begin + t_count) from a global counter (tiles are
+// row-major over blocks; with samples binned densest-first the heaviest tiles
+// come first). Every wave exits when the counter passes t_count.
+//   OUT 0: write the packed upper triangle in the caller's sample order
+//          (stored position -> sample id through d.perm, if any);
+//   OUT 1: write the packed triangle of stored positions, minus out_base
+//          (a band buffer of a streamed run).
+template <int CHUNK, int WAVES, int OUT>
+__global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int64_t nb, int64_t t_begin,
+                                                                   int64_t t_count,
+                                                                   unsigned long long *__restrict__ work,
+                                                                   float *__restrict__ out, int64_t out_base) {
+    __shared__ uint16_t s_list[WAVES][64 * 32];
+    __shared__ uint32_t s_stage[WAVES][CHUNK * 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ty = lane >> 3, tx = lane & 7;
+    uint32_t visited = 0;
+    for (;;) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(work, 1ull);
+        t = __shfl(t, 0, 64);
+        if (t >= (unsigned long long)t_count) break;
+        int64_t ra, cb;
+        sm_decode_upper(t_begin + (int64_t)t, nb, ra, cb);
+        const int64_t abase = ra * kSmBlock, bbase = cb * kSmBlock;
+        uint32_t acc[4][4];
+        visited += sm_tile_accumulate<CHUNK>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords, nullptr, abase,
+                                             bbase, s_list[wave], s_stage[wave], lane, acc);
+#pragma unroll
+        for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                const int64_t pa = abase + ty * 4 + ii, pb = bbase + tx * 4 + jj;
+                if (pa >= pb || pb >= d.n) continue;
+                const float sim = sm_finish(d, acc[ii][jj], pa, pb);
+                if constexpr (OUT == 0) {
+                    int64_t a = d.perm ? d.perm[pa] : pa, b = d.perm ? d.perm[pb] : pb;
+                    if (a > b) {
+                        const int64_t x = a;
+                        a = b;
+                        b = x;
+                    }
+                    out[a * (int64_t)d.n + b - (a + 1) * (a + 2) / 2] = sim;
+                } else {
+                    out[pa * (int64_t)d.n + pb - (pa + 1) * (pa + 2) / 2 - out_base] = sim;
+                }
+            }
+    }
+    if (d.counter && lane == 0 && visited) atomicAdd(d.counter, (unsigned long long)visited);
+}
+
+// Explicit pair list (caller's sample ids; d.pos maps them to stored
+// positions when samples are binned): lane = pair; 32-entry groups skipped
+// when no lane's two samples share mass in them.
 __global__ __launch_bounds__(256) void sm_pairs_kernel(SmData d, int64_t n_pairs, const int32_t *__restrict__ prow,
                                                        const int32_t *__restrict__ pcol, float *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = i < n_pairs;
-    const int64_t a = ok ? prow[i] : 0, b = ok ? pcol[i] : 0;
+    int64_t a = ok ? prow[i] : 0, b = ok ? pcol[i] : 0;
+    if (d.pos) {
+        a = d.pos[a];
+        b = d.pos[b];
+    }
     const uint32_t *BA = d.bm + (a / kSmBlock) * d.nwords, *BB = d.bm + (b / kSmBlock) * d.nwords;
     uint32_t acc = 0u;
     for (int64_t g = 0; g < d.nwords; g++) {
@@ -371,9 +468,8 @@ __global__ __launch_bounds__(256) void sm_pairs_kernel(SmData d, int64_t n_pairs
             m &= m - 1u;
         }
     }
-    double t = (double)acc * d.inv_scale;
-    t = t > 1.0 ? 1.0 : t;
-    if (ok) out[i] = (float)t;
+    const float sim = sm_finish(d, acc, a, b);
+    if (ok) out[i] = sim;
 }
 
 // ------------------------------------------------------------ launchers
@@ -407,32 +503,61 @@ hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const f
     return hipGetLastError();
 }
 
-hipError_t launch_embed(const EmbedData &e, hipStream_t st) {
-    if (e.n <= 0 || e.order_n <= 0) return hipSuccess;
+template <bool UNW, bool SUMS>
+static hipError_t launch_embed_t(const EmbedData &e, hipStream_t st) {
     const size_t lds = (size_t)(e.nslots > 0 ? e.nslots : 1) * 64 * sizeof(double);
     if (lds + sizeof(float) * kEmbedLeafChunk * 64 + sizeof(EmbedNode) * kEmbedNodeChunk > 65536)
         return hipErrorInvalidConfiguration;
     const unsigned waves = (unsigned)(e.npad / 64);
     if (e.nseg > 0) {
-        embed_seg_kernel<<<dim3(waves, (unsigned)e.nseg), 64, lds, st>>>(
+        embed_seg_kernel<UNW, SUMS><<<dim3(waves, (unsigned)e.nseg), 64, lds, st>>>(
             e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.nlp, e.n, e.npad, e.nwords, e.scale, e.Y, e.bm,
-            e.xroot);
+            e.xroot, e.colsum, e.colsum_f64);
         hipError_t r = hipGetLastError();
         if (r != hipSuccess) return r;
     }
     if (e.ntop > 0)
-        embed_top_kernel<<<waves, 64, lds, st>>>(e.nodes, e.top_nodes, e.top_src, e.ntop, e.ab, e.nlp, e.n, e.npad,
-                                                 e.nwords, e.scale, e.Y, e.bm, e.xroot);
+        embed_top_kernel<UNW, SUMS><<<waves, 64, lds, st>>>(e.nodes, e.top_nodes, e.top_src, e.ntop, e.ab, e.nlp,
+                                                            e.n, e.npad, e.nwords, e.scale, e.Y, e.bm, e.xroot,
+                                                            e.colsum, e.colsum_f64);
     return hipGetLastError();
 }
 
-hipError_t launch_sm_all_pairs(const SmData &d, float *out, hipStream_t st) {
-    const int64_t nb = (d.n + kSmBlock - 1) / kSmBlock;
-    const int64_t ntiles = nb * (nb + 1) / 2;
-    if (ntiles <= 0) return hipSuccess;
-    const int64_t nwg = (ntiles + kSmWaves - 1) / kSmWaves;
-    sm_tile_kernel<0><<<dim3((unsigned)nwg), 64 * kSmWaves, 0, st>>>(d, ntiles, nb, 0, 0, 0, 0, out);
+hipError_t launch_embed(const EmbedData &e, hipStream_t st) {
+    if (e.n <= 0 || e.order_n <= 0) return hipSuccess;
+    if (e.unweighted) return e.sums_only ? launch_embed_t<true, true>(e, st) : launch_embed_t<true, false>(e, st);
+    return e.sums_only ? launch_embed_t<false, true>(e, st) : launch_embed_t<false, false>(e, st);
+}
+
+// Persistent launch: variant (chunk, waves) chosen by the caller / autotuner.
+template <int OUT>
+static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
+                                      unsigned long long *work, float *out, int64_t out_base, int grid_waves,
+                                      hipStream_t st) {
+    if (t_count <= 0) return hipSuccess;
+#define CMS_PERSIST(CH, WV)                                                                                      \
+    {                                                                                                            \
+        const int64_t wg = std::max<int64_t>(1, std::min<int64_t>((t_count + WV - 1) / WV, grid_waves / WV));   \
+        sm_persistent_kernel<CH, WV, OUT><<<dim3((unsigned)wg), 64 * WV, 0, st>>>(d, nb, t_begin, t_count, work, \
+                                                                                   out, out_base);                \
+    }
+    switch (variant) {
+    case 0: CMS_PERSIST(8, 4); break;
+    case 1: CMS_PERSIST(16, 4); break;
+    case 2: CMS_PERSIST(32, 4); break;
+    case 3: CMS_PERSIST(16, 2); break;
+    case 4: CMS_PERSIST(32, 2); break;
+    default: CMS_PERSIST(16, 8); break;
+    }
+#undef CMS_PERSIST
     return hipGetLastError();
+}
+
+hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
+                                unsigned long long *work, float *out, int out_mode, int64_t out_base, int grid_waves,
+                                hipStream_t st) {
+    return out_mode == 0 ? launch_persistent_t<0>(d, variant, nb, t_begin, t_count, work, out, out_base, grid_waves, st)
+                         : launch_persistent_t<1>(d, variant, nb, t_begin, t_count, work, out, out_base, grid_waves, st);
 }
 
 hipError_t launch_sm_rect(const SmData &d, int r0, int nr, int c0, int nc, float *out, hipStream_t st) {
@@ -440,7 +565,7 @@ hipError_t launch_sm_rect(const SmData &d, int r0, int nr, int c0, int nc, float
     const int64_t ntiles = nbr * nbc;
     if (ntiles <= 0) return hipSuccess;
     const int64_t nwg = (ntiles + kSmWaves - 1) / kSmWaves;
-    sm_tile_kernel<1><<<dim3((unsigned)nwg), 64 * kSmWaves, 0, st>>>(d, ntiles, nbc, r0, nr, c0, nc, out);
+    sm_rect_kernel<<<dim3((unsigned)nwg), 64 * kSmWaves, 0, st>>>(d, ntiles, nbc, r0, nr, c0, nc, out);
     return hipGetLastError();
 }
 

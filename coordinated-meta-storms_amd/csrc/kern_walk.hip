@@ -305,10 +305,12 @@ __global__ void sample_scale_kernel(const int64_t *__restrict__ rowptr, const fl
 __global__ void walk_scatter_kernel(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                                     const float *__restrict__ val, int n, const int32_t *__restrict__ leaf_lp_ptr,
                                     const int32_t *__restrict__ leaf_lp, int64_t nlp, int storage,
-                                    const float *__restrict__ scale, void *__restrict__ ab) {
+                                    const float *__restrict__ scale, const int32_t *__restrict__ pos,
+                                    void *__restrict__ ab) {
     const int s = blockIdx.x;
     if (s >= n) return;
-    const int64_t base = (int64_t)(s >> 3) * nlp * 8 + (s & 7);
+    const int p = pos ? pos[s] : s;  // stored position of sample s
+    const int64_t base = (int64_t)(p >> 3) * nlp * 8 + (p & 7);
     const float sc = (storage == CMS_STORE_F32) ? 1.0f : scale[s];
     for (int64_t k = rowptr[s] + threadIdx.x; k < rowptr[s + 1]; k += blockDim.x) {
         const int32_t leaf = col[k];
@@ -344,9 +346,9 @@ hipError_t launch_sample_scale(const int64_t *rowptr, const float *val, int n, i
 
 hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const float *val, int n,
                                const int32_t *leaf_lp_ptr, const int32_t *leaf_lp, int64_t nlp, int storage,
-                               const float *scale, void *ab, hipStream_t st) {
+                               const float *scale, const int32_t *pos, void *ab, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    walk_scatter_kernel<<<n, 256, 0, st>>>(rowptr, col, val, n, leaf_lp_ptr, leaf_lp, nlp, storage, scale, ab);
+    walk_scatter_kernel<<<n, 256, 0, st>>>(rowptr, col, val, n, leaf_lp_ptr, leaf_lp, nlp, storage, scale, pos, ab);
     return hipGetLastError();
 }
 

@@ -44,7 +44,7 @@ constexpr int kEmbedSegMax = 512;
 //   leaf_lp_ptr/leaf_lp: leaf -> leaf-stream positions (CSR over leaves)
 hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const float *val, int n,
                                const int32_t *leaf_lp_ptr, const int32_t *leaf_lp, int64_t nlp,
-                               int storage, const float *scale, void *ab, hipStream_t st);
+                               int storage, const float *scale, const int32_t *pos, void *ab, hipStream_t st);
 // Per-sample power-of-two scale for 16-bit storage from CSR values.
 hipError_t launch_sample_scale(const int64_t *rowptr, const float *val, int n, int storage, float *scale,
                                float *inv_scale, hipStream_t st);
@@ -77,7 +77,21 @@ struct SmData {
     int n;                  // samples
     double inv_scale;       // 1 / (100 * 2^K)
     unsigned long long *counter;  // optional: entries visited (summed over tiles)
+    int unweighted;               // epilogue I / (W_a + W_b - I) instead of I / (100 2^K)
+    const unsigned long long *colsum;  // per-sample sum of Y (unweighted)
+    const int32_t *perm;          // stored position -> caller's sample id (null = identity)
+    const int32_t *pos;           // caller's sample id -> stored position (null = identity)
 };
+
+// Persistent all-pairs launch over upper-triangle block tiles
+// [t_begin, t_begin + t_count) of an nb x nb block grid. variant selects the
+// LDS chunk / waves-per-workgroup instance (0..5); out_mode 0 = packed
+// triangle in the caller's order, 1 = packed triangle of stored positions
+// minus out_base. `work` is a zeroed device counter.
+constexpr int kSmVariants = 6;
+hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
+                                unsigned long long *work, float *out, int out_mode, int64_t out_base, int grid_waves,
+                                hipStream_t st);
 
 hipError_t launch_zero_counter(unsigned long long *counter, hipStream_t st);
 hipError_t launch_calib_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
@@ -117,10 +131,13 @@ struct EmbedData {
     uint32_t *Y;
     uint32_t *bm;     // zeroed by the caller; bits are ORed in
     double *xroot;    // nseg x npad
+    int unweighted;   // presence propagation, coefficient w (EmbedNode.beta* hold w)
+    int sums_only;    // first pass: FP64 column sums of y into colsum_f64, nothing else
+    unsigned long long *colsum;  // per-sample sum of the stored fixed-point y (atomic)
+    double *colsum_f64;          // per-sample FP64 sum of y (sums_only pass)
 };
 hipError_t launch_embed(const EmbedData &e, hipStream_t st);
 // All pairs, packed upper triangle.
-hipError_t launch_sm_all_pairs(const SmData &d, float *out, hipStream_t st);
 // Rectangle over sample ranges aligned to nothing: rows [r0, r0+nr) x cols [c0, c0+nc).
 hipError_t launch_sm_rect(const SmData &d, int r0, int nr, int c0, int nc, float *out, hipStream_t st);
 // Explicit pair list.

@@ -49,6 +49,12 @@ enum {
     CMS_KERNEL_SHAREDMIN = 1 /* per-sample decayed-mass embedding + fixed-point shared-minimum tiles */
 };
 
+/* Distance metric (comp -M). */
+enum {
+    CMS_METRIC_WEIGHTED = 0,  /* -M 0, Meta-Storms (comp.h:332-375) */
+    CMS_METRIC_UNWEIGHTED = 1 /* -M 1, unweighted Meta-Storms (comp.h:378-446); shared-minimum kernel */
+};
+
 /* Abundance storage of the walk kernel (arithmetic is always FP32). */
 enum {
     CMS_STORE_F32 = 0,
@@ -63,7 +69,11 @@ typedef struct cms_opts {
     int tile_b;       /* SHAREDMIN column-block size (0 = default) */
     int chunk;        /* entries staged per LDS chunk (0 = default) */
     int use_graph;    /* capture per-band tile launches in a hipGraph (SHAREDMIN) */
-    int reserved[8];
+    int metric;       /* CMS_METRIC_* (default WEIGHTED) */
+    int bin;          /* SHAREDMIN: store samples binned by sparsity (densest first) */
+    int autotune;     /* SHAREDMIN: time the persistent-kernel variants once, keep the fastest */
+    int variant;      /* SHAREDMIN: 1..6 force a persistent-kernel variant (0 = default/tuned) */
+    int reserved[4];
 } cms_opts;
 
 typedef struct cms_ctx cms_ctx;
@@ -117,6 +127,20 @@ int cms_device_count(void);
  * (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
 int cms_calibrate_copy(cms_ctx *c, int64_t bytes, int reps, double *ms);
 
+/* Streamed all-pairs for sample sets whose triangle does not fit in host
+ * memory at once (the reference keeps it whole, cms_cuda.cu:1485-1487, and
+ * walks tiles, 1523-2196). The triangle of STORED positions (see
+ * cms_sample_order; the identity unless opts.bin) is computed in bands of
+ * about band_pairs pairs; each band is copied to pinned host memory while the
+ * next one computes, and fn(user, first, count, sims) is called on the
+ * calling thread with sims[k] = sim of stored packed index first + k. The
+ * pointer is valid during the call only. With opts.use_graph the band's
+ * launches are replayed from a hipGraph. */
+typedef void (*cms_band_fn)(void *user, int64_t first, int64_t count, const float *sims);
+int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *user);
+/* order[p] = caller's sample index stored at position p (n entries). */
+int cms_sample_order(cms_ctx *c, int32_t *order);
+
 /* Explicit pair list: sim[i] = sim(row[i], col[i]). Host arrays. The
  * reference kernels' boundary (CalcSimOfDiagonalSquare cms_cuda.cu:52-54 and
  * CalcSimOfNormalRectangle cms_cuda.cu:495-497 with d_order_row/d_order_col). */
@@ -147,6 +171,9 @@ typedef struct cms_stats {
     int nslots;                /* register slots after re-allocation */
     int scale_bits;            /* SHAREDMIN fixed-point scale (2^scale_bits per percent) */
     int entries;               /* SHAREDMIN active embedding entries */
+    int variant;               /* SHAREDMIN persistent-kernel variant in use (1..6) */
+    double tune_ms[6];         /* autotuner timings per variant (0 = not timed) */
+    int64_t bands;             /* bands of the last streamed run */
 } cms_stats;
 int cms_get_stats(cms_ctx *c, cms_stats *s);
 

@@ -254,3 +254,82 @@ void orc_pairs_f64(int leaf_n, const float *abd, int order_n, const int32_t *o1,
                              : orc_calc_sim_f64(order_n, o1, d1, o2, d2, od, A, B);
     }
 }
+
+/* comp.h:378-446, unweighted Meta-Storms in FP32, same operation order:
+ * presence propagates (a register holds 1 if either child is present);
+ * total adds w for every child edge present in both samples, total_count
+ * for every child edge present in either; sim = total / total_count. */
+float orc_calc_sim_unweight_f32(int order_n, const int32_t *o1, const float *d1, const int32_t *o2,
+                                const float *d2, const int32_t *od, const float *A, const float *B) {
+    float R1[ORC_REG_SIZE], R2[ORC_REG_SIZE];
+    float total = 0, total_count = 0;
+    memset(R1, 0, sizeof R1);
+    memset(R2, 0, sizeof R2);
+    for (int i = 0; i < order_n; i++) {
+        int c1 = o1[i], c2 = o2[i], dst = od[i] + ORC_REG_SIZE;
+        float w1 = 1 - d1[i];
+        float w2 = 1 - d2[i];
+        w1 = (w1 < 0) ? (float)ORC_MIN_DIST : w1;
+        w2 = (w2 < 0) ? (float)ORC_MIN_DIST : w2;
+        float a1, b1, a2, b2;
+        if (c1 >= 0) { a1 = (A[c1] > 0) ? 1 : 0; b1 = (B[c1] > 0) ? 1 : 0; }
+        else { a1 = R1[c1 + ORC_REG_SIZE]; b1 = R2[c1 + ORC_REG_SIZE]; }
+        total += (a1 > 0 && b1 > 0) ? w1 : 0;
+        total_count += (a1 > 0 || b1 > 0) ? w1 : 0;
+        if (c2 >= 0) { a2 = (A[c2] > 0) ? 1 : 0; b2 = (B[c2] > 0) ? 1 : 0; }
+        else { a2 = R1[c2 + ORC_REG_SIZE]; b2 = R2[c2 + ORC_REG_SIZE]; }
+        total += (a2 > 0 && b2 > 0) ? w2 : 0;
+        total_count += (a2 > 0 || b2 > 0) ? w2 : 0;
+        R1[dst] = (a1 > 0 || a2 > 0) ? 1 : 0;
+        R2[dst] = (b1 > 0 || b2 > 0) ? 1 : 0;
+    }
+    total /= total_count;
+    total = (total > 1.0) ? 1 : total;
+    total = (total < 0.0) ? 0 : total;
+    return total;
+}
+
+/* The same ratio in FP64 from presence sets: shared / (W_a + W_b - shared),
+ * W_s = sum of w over child edges whose subtree holds mass of s. */
+double orc_unweight_f64(int order_n, const int32_t *o1, const float *d1, const int32_t *o2, const float *d2,
+                        const int32_t *od, const float *A, const float *B) {
+    double R1[ORC_REG_SIZE], R2[ORC_REG_SIZE];
+    double shared = 0, wa = 0, wb = 0;
+    memset(R1, 0, sizeof R1);
+    memset(R2, 0, sizeof R2);
+    for (int i = 0; i < order_n; i++) {
+        const int32_t ch[2] = {o1[i], o2[i]};
+        const float dd[2] = {d1[i], d2[i]};
+        double pa[2], pb[2];
+        for (int j = 0; j < 2; j++) {
+            float wf = 1 - dd[j];
+            wf = (wf < 0) ? (float)ORC_MIN_DIST : wf;
+            if (ch[j] >= 0) { pa[j] = A[ch[j]] > 0; pb[j] = B[ch[j]] > 0; }
+            else { pa[j] = R1[ch[j] + ORC_REG_SIZE]; pb[j] = R2[ch[j] + ORC_REG_SIZE]; }
+            if (pa[j] > 0 && pb[j] > 0) shared += wf;
+            if (pa[j] > 0) wa += wf;
+            if (pb[j] > 0) wb += wf;
+        }
+        R1[od[i] + ORC_REG_SIZE] = (pa[0] > 0 || pa[1] > 0);
+        R2[od[i] + ORC_REG_SIZE] = (pb[0] > 0 || pb[1] > 0);
+    }
+    double t = shared / (wa + wb - shared);
+    if (t > 1.0) t = 1.0;
+    if (t < 0.0) t = 0.0;
+    return t;
+}
+
+void orc_pairs_unweight(int leaf_n, const float *abd, int order_n, const int32_t *o1, const float *d1,
+                        const int32_t *o2, const float *d2, const int32_t *od, long n_pairs, const int32_t *row,
+                        const int32_t *col, float *sim32, double *sim64, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    (void)threads;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < n_pairs; i++) {
+        const float *A = abd + (size_t)row[i] * leaf_n, *B = abd + (size_t)col[i] * leaf_n;
+        if (sim32) sim32[i] = orc_calc_sim_unweight_f32(order_n, o1, d1, o2, d2, od, A, B);
+        if (sim64) sim64[i] = orc_unweight_f64(order_n, o1, d1, o2, d2, od, A, B);
+    }
+}

@@ -48,6 +48,11 @@ _lib.orc_pairs_f64.argtypes = [C.c_int, _f32p, C.c_int, _i32p, _f32p, _i32p, _f3
                                _f64p, C.c_int, C.c_int]
 
 
+_lib.orc_pairs_unweight.restype = None
+_lib.orc_pairs_unweight.argtypes = [C.c_int, _f32p, C.c_int, _i32p, _f32p, _i32p, _f32p, _i32p, C.c_long, _i32p,
+                                    _i32p, C.c_void_p, C.c_void_p, C.c_int]
+
+
 def _tree_args(tree):
     return (tree.order_n,) + tuple(np.ascontiguousarray(a) for a in tree.arrays())
 
@@ -107,3 +112,15 @@ def ref_binary(name: str) -> str | None:
     """Path of a compiled reference tool under oracle/_ref, or None."""
     p = os.path.join(REF_DIR, name)
     return p if os.path.exists(p) else None
+
+
+def pairs_unweight(tree, abd, row, col, threads: int = 0):
+    """comp.h:378-446 in FP32 (same order) and the FP64 presence-set ratio."""
+    abd = np.ascontiguousarray(abd, np.float32)
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    s32 = np.zeros(max(len(row), 1), np.float32)
+    s64 = np.zeros(max(len(row), 1), np.float64)
+    _lib.orc_pairs_unweight(abd.shape[1], abd.reshape(-1), *_tree_args(tree), len(row), row, col,
+                            s32.ctypes.data, s64.ctypes.data, threads)
+    return s32[: len(row)], s64[: len(row)]

@@ -13,7 +13,7 @@
 // product. Needs ParallelMETA set, as the reference does at static-init time
 // (db.h:285, utility.h:27-41).
 //
-// Usage: ref_driver -D <letter> -T <table> [-r T|F] (--hex | --time P T)
+// Usage: ref_driver -D <letter> -T <table> [-r T|F] [-M metric] (--hex | --time P T)
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -27,6 +27,7 @@ int main(int argc, char **argv) {
     char db = 'G';
     std::string table_path;
     bool cp = true, hex = false;
+    int metric = 0;
     long time_pairs = 0;
     int threads = 1;
     for (int i = 1; i < argc; i++) {
@@ -35,6 +36,7 @@ int main(int argc, char **argv) {
         else if (a == "-T" && i + 1 < argc) table_path = argv[++i];
         else if (a == "-r" && i + 1 < argc) { const char *v = argv[++i]; cp = !(v[0] == 'f' || v[0] == 'F'); }
         else if (a == "--hex") hex = true;
+        else if (a == "-M" && i + 1 < argc) metric = atoi(argv[++i]);
         else if (a == "--time" && i + 2 < argc) { time_pairs = atol(argv[++i]); threads = atoi(argv[++i]); }
         else { fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
     }
@@ -53,7 +55,7 @@ int main(int argc, char **argv) {
         for (int j = i + 1; j < n; j++) { pm.push_back(i); pn.push_back(j); }
     if (hex) {
         for (size_t p = 0; p < pm.size(); p++) {
-            float s = tree.Calc_sim(abd[pm[p]], abd[pn[p]], 0);
+            float s = tree.Calc_sim(abd[pm[p]], abd[pn[p]], metric);
             printf("%d %d %a\n", pm[p], pn[p], (double)s);
         }
     }
@@ -63,7 +65,7 @@ int main(int argc, char **argv) {
         omp_set_num_threads(threads);
         auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel for schedule(dynamic, 1)
-        for (long p = 0; p < P; p++) out[p] = tree.Calc_sim(abd[pm[p]], abd[pn[p]], 0);
+        for (long p = 0; p < P; p++) out[p] = tree.Calc_sim(abd[pm[p]], abd[pn[p]], metric);
         auto t1 = std::chrono::steady_clock::now();
         double sec = std::chrono::duration<double>(t1 - t0).count();
         double chk = 0;

@@ -16,7 +16,9 @@ and stores inputs + outputs as tests/golden/<case>.tar.gz:
     dist_rT.dist             comp -r T (copy-number correction on)
     dist_rF.dist             comp -r F
     sim_rT.dist              comp -r T -d F (similarities)
+    dist_M1.dist             comp -M 1 (unweighted Meta-Storms)
     hex_rT.txt / hex_rF.txt  "i j <C99 hex float>" per pair, Calc_sim FP32
+    hex_M1.txt               the same for Calc_sim_unweight (comp.h:378-446)
 
 Usage: python tests/golden/make_golden.py [case ...]
 """
@@ -103,12 +105,15 @@ def make(case):
         comp = os.path.join(REF, "comp_O0")
         drv = os.path.join(REF, "ref_driver_O0")
         outs = {}
-        for tag, extra in [("dist_rT", []), ("dist_rF", ["-r", "F"]), ("sim_rT", ["-d", "F"])]:
+        for tag, extra in [("dist_rT", []), ("dist_rF", ["-r", "F"]), ("sim_rT", ["-d", "F"]),
+                           ("dist_M1", ["-M", "1"])]:
             o = os.path.join(tmp, tag + ".dist")
-            run_ref(d, [comp, "-T", "table.txt", "-o", o, "-D", "G", "-M", "0"] + extra)
+            args = [comp, "-T", "table.txt", "-o", o, "-D", "G"] + (extra if "-M" in extra else ["-M", "0"] + extra)
+            run_ref(d, args)
             outs[tag + ".dist"] = open(o, "rb").read()
         outs["hex_rT.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "--hex"]).encode()
         outs["hex_rF.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "-r", "F", "--hex"]).encode()
+        outs["hex_M1.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "-M", "1", "--hex"]).encode()
         path = os.path.join(HERE, case + ".tar.gz")
         with tarfile.open(path, "w:gz") as tf:
             tf.add(d, arcname="input")

@@ -1,0 +1,104 @@
+"""Batched form of the shared-minimum path: samples binned by sparsity,
+persistent-kernel variants and the autotuner, banded streaming to host memory
+(with and without hipGraph replay). All must give the same integers, hence
+bit-identical floats, as the plain all-pairs run; the rectangle and pair-list
+entry points must agree with it under binning."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def data(cms, gpu_ok):
+    # uneven sparsity (log-uniform nonzero counts) as in the batched workload
+    tree = cms.synth_tree(1003, 20000, 0.02)
+    cp = cms.synth_copy_numbers(3003, 20000, 15)
+    counts = cms.synth_counts(2003, 203, 20000, 20, 4000, 1.0, 30, 0.3, log_uniform=True)
+    return tree, cms.counts_to_abundance(counts, cp)
+
+
+def _plain(cms, tree, abd, metric=0):
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        return ctx.all_pairs()
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_binned_and_variants_identical(cms, data, metric):
+    print(f"binned variants metric={metric}", flush=True)
+    tree, abd = data
+    ref = _plain(cms, tree, abd, metric)
+    for variant in range(1, 7):
+        with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric, bin=1, variant=variant) as ctx:
+            ctx.set_tree(tree)
+            ctx.set_samples_csr(abd)
+            got = ctx.all_pairs()
+            assert ctx.stats().variant == variant
+        np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_autotune_picks_a_variant(cms, data):
+    print("autotune", flush=True)
+    tree, abd = data
+    ref = _plain(cms, tree, abd)
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, autotune=1) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        got = ctx.all_pairs()
+        st = ctx.stats()
+        assert 1 <= st.variant <= 6
+        assert all(t > 0 for t in st.tune_ms)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+@pytest.mark.parametrize("bin_", [0, 1])
+def test_stream_reassembles_triangle(cms, data, use_graph, bin_):
+    print(f"stream graph={use_graph} bin={bin_}", flush=True)
+    tree, abd = data
+    ref = _plain(cms, tree, abd)
+    n = abd.n
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=bin_, use_graph=use_graph) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        stored = np.full(n * (n - 1) // 2, np.nan, np.float32)
+        seen = []
+
+        def fn(first, count, sims):
+            stored[first:first + count] = sims
+            seen.append((first, count))
+
+        ctx.all_pairs_stream(fn, band_pairs=3000)  # many small bands
+        order = ctx.sample_order()
+        assert ctx.stats().bands == len(seen) > 3
+    assert sorted(order.tolist()) == list(range(n))
+    assert not np.isnan(stored).any()
+    # stored packed index (p, q) -> caller's pair (order[p], order[q])
+    pi, qi = np.triu_indices(n, 1)
+    a, b = order[pi], order[qi]
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    np.testing.assert_array_equal(stored.view(np.uint32), ref[cms.packed_index(lo, hi, n)].view(np.uint32))
+
+
+def test_binned_tile_and_pairs(cms, data):
+    print("binned tile/pairs", flush=True)
+    tree, abd = data
+    ref = _plain(cms, tree, abd)
+    n = abd.n
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        t = ctx.tile(5, 40, 17, 50)
+        r, c = np.meshgrid(np.arange(5, 45), np.arange(17, 67), indexing="ij")
+        exp = np.ones((40, 50), np.float32)
+        m = r != c
+        exp[m] = ref[cms.packed_index(np.minimum(r, c)[m], np.maximum(r, c)[m], n)]
+        np.testing.assert_array_equal(t.view(np.uint32), exp.view(np.uint32))
+        rows = np.array([0, 7, 150, 202, 3], np.int32)
+        cols = np.array([1, 200, 9, 0, 3], np.int32)
+        got = ctx.sim_pairs(rows, cols)
+        for k in range(4):
+            a, b = sorted((int(rows[k]), int(cols[k])))
+            assert got[k] == ref[cms.packed_index(a, b, n)]

@@ -1,0 +1,104 @@
+"""Unweighted Meta-Storms (comp -M 1, _Comp_Tree::Calc_sim_unweight,
+comp.h:378-446).
+
+CPU: the oracle's FP32 restatement is bit-exact with the compiled reference
+(golden hex fixtures) and agrees with the FP64 presence-set ratio
+shared / (W_a + W_b - shared).
+GPU: the shared-minimum kernels with presence propagation and coefficient w
+are within 1e-5 of FP64; cuda-comp -M 1 prints the reference's .dist values
+(to the 6-digit print, within one unit of the last digit).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES
+
+TOL = 1e-5
+
+
+def _inputs(cms, g, cp=True):
+    db = cms.Database(g.root, "G")
+    tab = cms.Table(g.table)
+    abd, _ = tab.abundance(db, cp)
+    return db.tree(), abd
+
+
+def _hex(g):
+    rows = [ln.split() for ln in g.read("hex_M1.txt").decode().splitlines() if ln.strip()]
+    i = np.array([int(r[0]) for r in rows], np.int32)
+    j = np.array([int(r[1]) for r in rows], np.int32)
+    s = np.array([float.fromhex(r[2]) for r in rows], np.float32)
+    return i, j, s
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_oracle_unweighted_bit_exact_vs_reference(cms, orc, golden, case):
+    g = golden(case)
+    tree, abd = _inputs(cms, g)
+    i, j, ref = _hex(g)
+    s32, s64 = orc.pairs_unweight(tree, abd, i, j)
+    np.testing.assert_array_equal(s32.view(np.uint32), ref.view(np.uint32))
+    assert np.abs(s64 - ref).max() < 1e-5
+
+
+def test_unweighted_known_answer(cms, orc, golden):
+    """Toy tree, s1 = {A, B}, s2 = {A, C}: shared edges A (0.9) + AB (0.7)
+    = 1.6; either: A, B, C, AB, CD = 0.9 + 0.8 + 0.6 + 0.7 + 0.4 = 3.4."""
+    g = golden("toy")
+    tree, abd = _inputs(cms, g)
+    _, s64 = orc.pairs_unweight(tree, abd, np.array([0], np.int32), np.array([1], np.int32))
+    assert abs(s64[0] - 1.6 / 3.4) < 1e-7
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_gpu_unweighted_vs_fp64(cms, orc, golden, gpu_ok, case):
+    print(f"unweighted golden {case}", flush=True)
+    g = golden(case)
+    tree, abd = _inputs(cms, g)
+    i, j, ref = _hex(g)
+    _, s64 = orc.pairs_unweight(tree, abd, i, j)
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=cms.METRIC_UNWEIGHTED) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples(abd)
+        got = ctx.all_pairs()
+        n = abd.shape[0]
+        p = cms.packed_index(i, j, n)
+        assert np.abs(got[p] - s64).max() <= TOL
+        np.testing.assert_array_equal(ctx.sim_pairs(i, j), got[p])
+
+
+@pytest.mark.gpu
+def test_gpu_unweighted_large_tree(cms, orc, gpu_ok):
+    print("unweighted large tree", flush=True)
+    tree, cp, counts, abd = cms.workload(1, samples=80)
+    with cms.Context(0, metric=cms.METRIC_UNWEIGHTED) as ctx:  # the walk kernel option defers to shared-minimum
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        got = ctx.all_pairs()
+    ii, jj = np.triu_indices(abd.n, 1)
+    _, s64 = orc.pairs_unweight(tree, abd.dense(tree.leaf_n), ii.astype(np.int32), jj.astype(np.int32))
+    err = np.abs(got - s64)
+    print(f"unweighted max err {err.max():.3e}", flush=True)
+    assert err.max() <= TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_cuda_comp_unweighted(cms, golden, gpu_ok, tmp_path, case):
+    g = golden(case)
+    exe = os.path.join(cms.BIN_DIR, "cuda-comp")
+    out = str(tmp_path / "o.dist")
+    r = subprocess.run([exe, "-T", g.table, "-D", "G", "-o", out, "-M", "1"],
+                       env=dict(os.environ, ParallelMETA=g.root), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = [ln.split("\t") for ln in open(out).read().splitlines()]
+    ref = [ln.split("\t") for ln in g.read("dist_M1.dist").decode().splitlines()]
+    assert got[0] == ref[0] and len(got) == len(ref)
+    for rg, rr in zip(got[1:], ref[1:]):
+        assert rg[0] == rr[0]
+        for a, b in zip(rg[1:], rr[1:]):
+            assert abs(float(a) - float(b)) <= 2e-5 + 1e-5 * abs(float(b)), (a, b)
