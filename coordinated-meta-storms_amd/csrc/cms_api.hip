@@ -12,68 +12,10 @@
 #include <string>
 #include <vector>
 
-#include "cms_internal.h"
-#include "kernels.h"
+#include "cms_ctx.h"
 
 using namespace cms;
 
-struct cms_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    cms_opts opts{};
-    std::string err;
-
-    bool has_tree = false;
-    CompiledTree tree;
-    WalkNode *d_nodes = nullptr;
-    WalkChunk *d_chunks = nullptr;
-    int32_t *d_leaf_lp_ptr = nullptr, *d_leaf_lp = nullptr, *d_leaf_uses = nullptr;
-    EmbedNode *d_embnodes = nullptr, *d_embnodes_u = nullptr;
-    int64_t nrows_u = 0;
-    int32_t *d_seg_first = nullptr, *d_seg_last = nullptr, *d_top_nodes = nullptr, *d_top_src = nullptr;
-    int64_t nrows = 0;
-
-    int n = -1;
-    int64_t nnz = 0;
-    bool has_negative = false;
-    int64_t *d_rowptr = nullptr;
-    int32_t *d_col = nullptr;
-    float *d_val = nullptr;
-
-    uint8_t *d_ab = nullptr;
-    int ab_storage = -1;  // storage of the valid walk layout, -1 = none
-    float *d_scale = nullptr, *d_inv = nullptr;
-    size_t cap_ab = 0, cap_scale = 0, cap_inv = 0;
-
-    bool sm_ready = false;
-    int sm_metric = -1;
-    int64_t sm_rows = 0;
-    bool ab_binned = false;          // walk layout holds samples at stored (binned) positions
-    bool sm_binned = false;          // the shared-minimum layout is binned
-    std::vector<int32_t> h_perm;     // stored position -> sample (binned layout)
-    int32_t *d_perm = nullptr, *d_pos = nullptr;
-    unsigned long long *d_work = nullptr;  // persistent-kernel tile counter
-    int variant = -1;                // tuned persistent-kernel variant
-    int grid_waves = 0;              // persistent grid size in waves
-    hipStream_t copy_stream = nullptr;
-    hipGraphExec_t band_exec = nullptr;
-    unsigned long long *d_colsum = nullptr;
-    size_t cap_colsum = 0;
-    uint32_t *d_Y = nullptr, *d_bm = nullptr;
-    double *d_mass = nullptr;
-    double *d_xroot = nullptr;
-    size_t cap_xroot = 0;
-    size_t cap_Y = 0, cap_bm = 0, cap_mass = 0;
-    int64_t npad = 0, nwords = 0;
-    int scale_bits = 0;
-    unsigned long long *d_counter = nullptr;  // entries visited by shared-minimum tiles
-
-    float *d_out = nullptr;
-    size_t out_cap = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
-    bool kern_pending = false, prep_pending = false;
-    cms_stats stats{};
-};
 
 namespace {
 
@@ -588,8 +530,12 @@ namespace {
 
 // Persistent-kernel variant: forced by opts.variant, else the default, or the
 // fastest of all variants timed once on the first (heaviest) tiles when
-// opts.autotune is set. Tuning writes into `out` (later overwritten).
-int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, hipStream_t st) {
+// opts.autotune is set. Tuning writes into `out` (later overwritten) with
+// out_mode as in launch_sm_persistent and out_base 0: with out_mode 1 the
+// caller passes in `ntiles` only tiles whose packed stored-position indices
+// fit `out`.
+int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out_cap, int out_mode,
+                 hipStream_t st) {
     if (!c->d_work) HIPCALL(c, dalloc(c->d_work, 1));
     if (c->grid_waves == 0) {
         int cus = 0;
@@ -613,7 +559,8 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, hipStream_t
         for (int rep = 0; rep < 2; rep++) {
             HIPCALL(c, launch_zero_counter(c->d_work, st));
             HIPCALL(c, hipEventRecord(c->evp0, st));
-            HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, 0, sample, c->d_work, out, 0, 0, c->grid_waves, st));
+            HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, 0, sample, c->d_work, out, out_cap, out_mode, 0,
+                                            c->grid_waves, st));
             HIPCALL(c, hipEventRecord(c->evp1, st));
             HIPCALL(c, hipEventSynchronize(c->evp1));
             float ms = 0;
@@ -655,12 +602,14 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
     if (use_sm(c)) {
         const int64_t nb = (c->n + kSmBlock - 1) / kSmBlock;
         const int64_t ntiles = nb * (nb + 1) / 2;
-        rc = pick_variant(c, nb, ntiles, d_sim, st);
+        const int64_t npairs = (int64_t)c->n * (c->n > 0 ? c->n - 1 : 0) / 2;  // d_sim holds the whole triangle
+        rc = pick_variant(c, nb, ntiles, d_sim, npairs, 0, st);
         if (rc) return rc;
         HIPCALL(c, hipEventRecord(c->ev0, st));  // after any tuning launches
         HIPCALL(c, launch_zero_counter(c->d_counter, st));
         HIPCALL(c, launch_zero_counter(c->d_work, st));
-        HIPCALL(c, launch_sm_persistent(sm_data(c), c->variant, nb, 0, ntiles, c->d_work, d_sim, 0, 0, c->grid_waves, st));
+        HIPCALL(c, launch_sm_persistent(sm_data(c), c->variant, nb, 0, ntiles, c->d_work, d_sim, npairs, 0, 0,
+                                        c->grid_waves, st));
         c->stats.tiles = ntiles;
     } else {
         HIPCALL(c, launch_walk_all_pairs(walk_data(c), d_sim, st));
@@ -870,7 +819,9 @@ int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *u
         STREAMCALL(hipEventCreateWithFlags(&done_c[i], hipEventDisableTiming));
     }
     if (!c->copy_stream) STREAMCALL(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
-    rc = pick_variant(c, nb, nb * (nb + 1) / 2, dbuf[0], c->stream);
+    // tune on the first band's tiles: their stored-position pairs are the
+    // first count[0] <= cap entries of the packed triangle
+    rc = pick_variant(c, nb, tile_start(cuts[1]), dbuf[0], cap, 1, c->stream);
     if (rc) {
         cleanup();
         return rc;
@@ -894,7 +845,7 @@ int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *u
                 hipGraph_t g = nullptr;
                 STREAMCALL(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 hipError_t e1 = launch_zero_counter(c->d_work, c->stream);
-                hipError_t e2 = launch_sm_persistent(d, c->variant, nb, t0, t1 - t0, c->d_work, dbuf[b], 1, first[k],
+                hipError_t e2 = launch_sm_persistent(d, c->variant, nb, t0, t1 - t0, c->d_work, dbuf[b], cap, 1, first[k],
                                                      c->grid_waves, c->stream);
                 STREAMCALL(hipStreamEndCapture(c->stream, &g));
                 STREAMCALL(e1);
@@ -915,7 +866,7 @@ int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *u
                 STREAMCALL(hipGraphLaunch(c->band_exec, c->stream));
             } else {
                 STREAMCALL(launch_zero_counter(c->d_work, c->stream));
-                STREAMCALL(launch_sm_persistent(d, c->variant, nb, t0, t1 - t0, c->d_work, dbuf[b], 1, first[k],
+                STREAMCALL(launch_sm_persistent(d, c->variant, nb, t0, t1 - t0, c->d_work, dbuf[b], cap, 1, first[k],
                                                 c->grid_waves, c->stream));
             }
             STREAMCALL(hipEventRecord(done_k[b], c->stream));

@@ -1,0 +1,83 @@
+// Context state shared by the C ABI implementation (cms_api.hip) and the
+// serving loop (cms_serve.hip). Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cms.h"
+#include "cms_internal.h"
+#include "kernels.h"
+
+using cms::CompiledTree;
+using cms::EmbedNode;
+using cms::WalkChunk;
+using cms::WalkNode;
+
+struct cms_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cms_opts opts{};
+    std::string err;
+
+    bool has_tree = false;
+    CompiledTree tree;
+    WalkNode *d_nodes = nullptr;
+    WalkChunk *d_chunks = nullptr;
+    int32_t *d_leaf_lp_ptr = nullptr, *d_leaf_lp = nullptr, *d_leaf_uses = nullptr;
+    EmbedNode *d_embnodes = nullptr, *d_embnodes_u = nullptr;
+    int64_t nrows_u = 0;
+    int32_t *d_seg_first = nullptr, *d_seg_last = nullptr, *d_top_nodes = nullptr, *d_top_src = nullptr;
+    int64_t nrows = 0;
+
+    int n = -1;
+    int64_t nnz = 0;
+    bool has_negative = false;
+    int64_t *d_rowptr = nullptr;
+    int32_t *d_col = nullptr;
+    float *d_val = nullptr;
+
+    uint8_t *d_ab = nullptr;
+    int ab_storage = -1;  // storage of the valid walk layout, -1 = none
+    float *d_scale = nullptr, *d_inv = nullptr;
+    size_t cap_ab = 0, cap_scale = 0, cap_inv = 0;
+
+    bool sm_ready = false;
+    int sm_metric = -1;
+    int64_t sm_rows = 0;
+    bool ab_binned = false;          // walk layout holds samples at stored (binned) positions
+    bool sm_binned = false;          // the shared-minimum layout is binned
+    std::vector<int32_t> h_perm;     // stored position -> sample (binned layout)
+    int32_t *d_perm = nullptr, *d_pos = nullptr;
+    unsigned long long *d_work = nullptr;  // persistent-kernel tile counter
+    int variant = -1;                // tuned persistent-kernel variant
+    int grid_waves = 0;              // persistent grid size in waves
+    hipStream_t copy_stream = nullptr;
+    hipGraphExec_t band_exec = nullptr;
+    unsigned long long *d_colsum = nullptr;
+    size_t cap_colsum = 0;
+    uint32_t *d_Y = nullptr, *d_bm = nullptr;
+    double *d_mass = nullptr;
+    double *d_xroot = nullptr;
+    size_t cap_xroot = 0;
+    size_t cap_Y = 0, cap_bm = 0, cap_mass = 0;
+    int64_t npad = 0, nwords = 0;
+    int scale_bits = 0;
+    unsigned long long *d_counter = nullptr;  // entries visited by shared-minimum tiles
+
+    float *d_out = nullptr;
+    size_t out_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+    bool kern_pending = false, prep_pending = false;
+    cms_stats stats{};
+};
+
+namespace cms {
+// Shared-minimum layout of the context's samples for its metric (built if
+// needed); the context must hold a tree and samples.
+int ctx_prepare(cms_ctx *c);
+SmData ctx_sm_data(const cms_ctx *c);
+int ctx_fail(cms_ctx *c, int code, const std::string &msg);
+}  // namespace cms

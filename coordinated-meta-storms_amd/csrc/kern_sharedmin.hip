@@ -553,9 +553,31 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
     return hipGetLastError();
 }
 
+// Host decode of upper-triangle block tile t (row-major over an nb x nb grid).
+static void host_decode_upper(int64_t t, int64_t nb, int64_t &rb, int64_t &cb) {
+    int64_t r = 0;
+    while ((r + 1) * nb - (r + 1) * r / 2 <= t) r++;
+    rb = r;
+    cb = r + (t - (r * nb - r * (r - 1) / 2));
+}
+
 hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
-                                unsigned long long *work, float *out, int out_mode, int64_t out_base, int grid_waves,
-                                hipStream_t st) {
+                                unsigned long long *work, float *out, int64_t out_cap, int out_mode,
+                                int64_t out_base, int grid_waves, hipStream_t st) {
+    if (t_count <= 0) return hipSuccess;
+    if (t_begin < 0 || t_begin + t_count > nb * (nb + 1) / 2 || nb * kSmBlock > d.npad) return hipErrorInvalidValue;
+    // every packed index the launch writes must fall inside out[0, out_cap)
+    const int64_t n = d.n;
+    auto row_start = [n](int64_t r) { return r * n - r * (r + 1) / 2; };
+    if (out_mode == 0) {
+        if (out_cap < n * (n - 1) / 2) return hipErrorInvalidValue;
+    } else {
+        int64_t rb0, cb0, rb1, cb1;
+        host_decode_upper(t_begin, nb, rb0, cb0);
+        host_decode_upper(t_begin + t_count - 1, nb, rb1, cb1);
+        const int64_t lo = row_start(std::min(rb0 * kSmBlock, n)), hi = row_start(std::min((rb1 + 1) * kSmBlock, n));
+        if (lo < out_base || hi - out_base > out_cap) return hipErrorInvalidValue;
+    }
     return out_mode == 0 ? launch_persistent_t<0>(d, variant, nb, t_begin, t_count, work, out, out_base, grid_waves, st)
                          : launch_persistent_t<1>(d, variant, nb, t_begin, t_count, work, out, out_base, grid_waves, st);
 }

@@ -87,11 +87,13 @@ struct SmData {
 // [t_begin, t_begin + t_count) of an nb x nb block grid. variant selects the
 // LDS chunk / waves-per-workgroup instance (0..5); out_mode 0 = packed
 // triangle in the caller's order, 1 = packed triangle of stored positions
-// minus out_base. `work` is a zeroed device counter.
+// minus out_base. `work` is a zeroed device counter. out holds out_cap
+// floats; a launch that could write outside them is refused on the host
+// (hipErrorInvalidValue) before anything is queued.
 constexpr int kSmVariants = 6;
 hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
-                                unsigned long long *work, float *out, int out_mode, int64_t out_base, int grid_waves,
-                                hipStream_t st);
+                                unsigned long long *work, float *out, int64_t out_cap, int out_mode,
+                                int64_t out_base, int grid_waves, hipStream_t st);
 
 hipError_t launch_zero_counter(unsigned long long *counter, hipStream_t st);
 hipError_t launch_calib_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);

@@ -53,14 +53,15 @@ def test_autotune_picks_a_variant(cms, data):
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("use_graph", [0, 1])
-@pytest.mark.parametrize("bin_", [0, 1])
-def test_stream_reassembles_triangle(cms, data, use_graph, bin_):
-    print(f"stream graph={use_graph} bin={bin_}", flush=True)
+@pytest.mark.parametrize("use_graph,bin_,autotune", [(0, 0, 0), (0, 1, 0), (1, 0, 0), (1, 1, 0), (0, 1, 1), (1, 1, 1)])
+def test_stream_reassembles_triangle(cms, data, use_graph, bin_, autotune):
+    # autotune: the tuner times variants inside the first band's buffer, which
+    # is smaller than the whole triangle here
+    print(f"stream graph={use_graph} bin={bin_} autotune={autotune}", flush=True)
     tree, abd = data
     ref = _plain(cms, tree, abd)
     n = abd.n
-    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=bin_, use_graph=use_graph) as ctx:
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=bin_, use_graph=use_graph, autotune=autotune) as ctx:
         ctx.set_tree(tree)
         ctx.set_samples_csr(abd)
         stored = np.full(n * (n - 1) // 2, np.nan, np.float32)
