@@ -230,6 +230,134 @@ def run_precision(cms, a):
     print(json.dumps(res), flush=True)
 
 
+def run_serving(cms, a):
+    """Workload 4: 200,000 resident references (workload-3 generator) on the
+    99,322-leaf tree; query sets of 1-512 samples (log-uniform sizes, drawn
+    from a pool of the same generator with another seed) arrive as a Poisson
+    process and are compared with every reference through the serving loop
+    (continuous batching over a band cycle, FIFO admission by query slots and
+    device memory). Reports latency percentiles, achieved vs offered pairs/s
+    and the cross kernel's share of VALU peak; a few pairs are checked
+    against the FP64 oracle."""
+    import threading
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    t0 = time.perf_counter()
+    tree, cp, counts, refs = cms.workload(4, samples=a.samples)
+    del counts
+    w = cms.WORKLOADS[4]
+    pool = cms.counts_to_abundance(cms.synth_counts(6004, a.query_pool, w["leaves"], w["nnz"][0], w["nnz"][1],
+                                                    w["zipf"], w["nb"][0], w["nb"][1], w["log_uniform"]), cp)
+    n_ref = refs.n
+    log(f"workload 4: {n_ref} references (nnz {refs.rowptr[-1]}), query pool {pool.n}, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    ctx = cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1)
+    ctx.set_tree(tree)
+    ctx.set_samples_csr(refs)
+    t1 = time.perf_counter()
+    srv = cms.Server(ctx, slots=a.slots, bands=a.bands, memory_budget=a.budget)
+    prep_s = time.perf_counter() - t1
+    log(f"references embedded and resident in {prep_s:.1f} s (slots {a.slots}, bands {a.bands})")
+    # warm-up set (not measured)
+    srv.submit(pool.subset(np.arange(64))).wait()
+    st0 = srv.stats()
+
+    rng = np.random.default_rng(4004)
+    n_req = max(1, int(a.rate * a.serve_seconds))
+    gaps = rng.exponential(1.0 / a.rate, n_req)
+    sizes = np.minimum(512, np.floor(np.exp(rng.uniform(0.0, np.log(513.0), n_req)))).astype(np.int64)
+    starts = rng.integers(0, pool.n, n_req)
+    subsets = [pool.subset((starts[k] + np.arange(sizes[k])) % pool.n) for k in range(n_req)]
+    reqs = [None] * n_req
+    submitted = threading.Event()
+    state = {"stopped_at": None}
+
+    def submitter():
+        t_start = time.perf_counter()
+        t_next = t_start
+        for k in range(n_req):
+            t_next += gaps[k]
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            outstanding = sum(1 for r in reqs[:k] if r is not None and r.info is None)
+            if outstanding >= a.max_outstanding:
+                state["stopped_at"] = k  # saturated: stop offering load
+                break
+            reqs[k] = srv.submit(subsets[k])
+        submitted.set()
+
+    th = threading.Thread(target=submitter)
+    t_run = time.perf_counter()
+    th.start()
+    lat, queue_s, service_s, pairs_done = [], [], [], 0
+    checked = []
+    k = 0
+    while True:
+        if k < n_req and reqs[k] is not None:
+            r = reqs[k]
+            r.wait()
+            i = r.info
+            lat.append(i.t_done - i.t_submit)
+            queue_s.append(i.t_join - i.t_submit)
+            service_s.append(i.t_done - i.t_join)
+            pairs_done += int(i.nq) * n_ref
+            if len(checked) < 3:
+                checked.append((k, r.out))
+            else:
+                r.out = None
+            k += 1
+            continue
+        if submitted.is_set() and (k >= n_req or reqs[k] is None):
+            break
+        time.sleep(0.001)
+    th.join()
+    wall = time.perf_counter() - t_run
+    st = srv.stats()
+    kernel_ms = st.kernel_ms - st0.kernel_ms
+    entries = st.pair_entries - st0.pair_entries
+    # FP64 spot check of a few pairs per checked request
+    errs = []
+    for kk, out in checked:
+        q = subsets[kk]
+        qi = rng.integers(0, q.n, 16)
+        ri = rng.integers(0, n_ref, 16)
+        both = np.vstack([q.subset(qi).dense(tree.leaf_n), refs.subset(ri).dense(tree.leaf_n)])
+        f64 = orc.pairs_f64(tree, both, np.arange(16, dtype=np.int32), np.arange(16, 32, dtype=np.int32))
+        errs.append(float(np.abs(out[qi, ri].astype(np.float64) - f64).max()))
+    srv.close()
+    ctx.close()
+    lat = np.array(lat)
+    served = len(lat)
+    mean_q = float(sizes[:served].mean()) if served else 0.0
+    valu_tops = entries * 1.5 / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    res = {"metric": "serving: p50/p99 latency and query x reference pairs/s on 1 MI355X",
+           "value": pairs_done / wall, "unit": "pairs/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "u32 fixed-point (FP64 embedding)",
+           "data": "synthetic (seeded generator; references tree 1004 / samples 2004, queries 6004)",
+           "config": {"workload": "workload 4: 200,000 resident references (log-uniform 50-20,000 OTUs) x "
+                                  "99,322-leaf tree; query sets of 1-512 (log-uniform), Poisson arrivals",
+                      "references": n_ref, "rate_per_s": a.rate, "seconds": a.serve_seconds, "slots": a.slots,
+                      "bands": a.bands, "memory_budget": st.arena_bytes},
+           "requests_offered": n_req, "requests_served": served, "saturated": state["stopped_at"] is not None,
+           "stopped_offering_at": state["stopped_at"], "mean_queries_per_set": mean_q,
+           "offered_pairs_per_s": a.rate * float(sizes.mean()) * n_ref,
+           "latency_s": {"p50": float(np.percentile(lat, 50)) if served else None,
+                         "p99": float(np.percentile(lat, 99)) if served else None,
+                         "mean": float(lat.mean()) if served else None, "max": float(lat.max()) if served else None},
+           "queue_s_p50": float(np.percentile(queue_s, 50)) if served else None,
+           "service_s_p50": float(np.percentile(service_s, 50)) if served else None,
+           "wall_s": wall, "prep_s": prep_s, "steps": st.steps - st0.steps, "kernel_ms": kernel_ms,
+           "kernel_pairs_per_s": pairs_done / (kernel_ms * 1e-3) if kernel_ms > 0 else None,
+           "arena_peak": st.arena_peak, "slots_peak": st.slots_peak,
+           "max_abs_err_vs_fp64_spot": max(errs) if errs else None,
+           "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                        "frac": valu_tops / VALU_PEAK_TOPS, "traffic": None, "kernel": "sm_cross_kernel",
+                        "pair_entries": entries}}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,9 +374,18 @@ def main():
     ap.add_argument("--bin", type=int, default=1, help="bin samples by sparsity (shared-minimum path)")
     ap.add_argument("--autotune", type=int, default=1, help="autotune the persistent-kernel variant")
     ap.add_argument("--variant", type=int, default=0, help="force a persistent-kernel variant (1..6)")
-    ap.add_argument("--workload", type=int, default=1, choices=[1, 2, 3],
+    ap.add_argument("--workload", type=int, default=1, choices=[1, 2, 3, 4],
                     help="1: headline all-pairs; 2: precision (16-bit walk storage, 50,000 samples); "
-                         "3: batched stream (100,000 samples of uneven sparsity)")
+                         "3: batched stream (100,000 samples of uneven sparsity); "
+                         "4: serving (200,000 resident references, Poisson query sets)")
+    ap.add_argument("--rate", type=float, default=20.0, help="query sets per second (workload 4)")
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="seconds of arrivals (workload 4)")
+    ap.add_argument("--slots", type=int, default=1024, help="resident query slots (workload 4)")
+    ap.add_argument("--bands", type=int, default=16, help="reference bands per cycle (workload 4)")
+    ap.add_argument("--budget", type=int, default=0, help="serving arena bytes, 0 = default (workload 4)")
+    ap.add_argument("--query-pool", type=int, default=8192, help="query samples generated (workload 4)")
+    ap.add_argument("--max-outstanding", type=int, default=400,
+                    help="stop offering load beyond this many unfinished sets (workload 4)")
     ap.add_argument("--band-pairs", type=int, default=1 << 28, help="pairs per streamed band (workload 3)")
     ap.add_argument("--graph", type=int, default=1, help="replay streamed bands from a hipGraph (workload 3)")
     ap.add_argument("--error-pairs", type=int, default=1000000, help="random pairs checked vs FP64 (workload 2)")
@@ -263,6 +400,8 @@ def main():
     cms = importlib.import_module("coordinated-meta-storms_amd")
     if cms.device_count() < 1:
         raise SystemExit("bench.py needs a HIP device: " + cms.last_error())
+    if a.workload == 4:
+        return run_serving(cms, a)
     if a.workload == 3:
         return run_batched(cms, a)
     if a.workload == 2:

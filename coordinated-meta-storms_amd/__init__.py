@@ -31,6 +31,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 
 OK = 0
 E_INVAL, E_NOMEM, E_HIP, E_SCHEDULE, E_STATE, E_IO, E_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
+E_RANGE, E_TIMEOUT = -8, -9
 KERNEL_WALK, KERNEL_SHAREDMIN = 0, 1
 METRIC_WEIGHTED, METRIC_UNWEIGHTED = 0, 1
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
@@ -57,7 +58,8 @@ _vp = C.c_void_p
 class Opts(C.Structure):
     _fields_ = [("kernel", C.c_int), ("storage", C.c_int), ("tile_a", C.c_int), ("tile_b", C.c_int),
                 ("chunk", C.c_int), ("use_graph", C.c_int), ("metric", C.c_int), ("bin", C.c_int),
-                ("autotune", C.c_int), ("variant", C.c_int), ("reserved", C.c_int * 4)]
+                ("autotune", C.c_int), ("variant", C.c_int), ("embed_slice", C.c_int),
+                ("reserved", C.c_int * 3)]
 
 
 class Stats(C.Structure):
@@ -65,6 +67,23 @@ class Stats(C.Structure):
                 ("dense_updates", C.c_double), ("valu_ops", C.c_double), ("bytes_moved", C.c_double),
                 ("tiles", C.c_int64), ("nslots", C.c_int), ("scale_bits", C.c_int), ("entries", C.c_int),
                 ("variant", C.c_int), ("tune_ms", C.c_double * 6), ("bands", C.c_int64)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_int), ("bands", C.c_int), ("memory_budget", C.c_int64), ("variant", C.c_int),
+                ("reserved", C.c_int * 3)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("rc", C.c_int), ("nq", C.c_int), ("join_step", C.c_int64), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_join", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("queued", C.c_int64),
+                ("active", C.c_int64), ("steps", C.c_int64), ("kernel_ms", C.c_double), ("pairs", C.c_double),
+                ("pair_entries", C.c_double), ("arena_bytes", C.c_int64), ("arena_peak", C.c_int64),
+                ("slots", C.c_int), ("slots_peak", C.c_int), ("bands", C.c_int), ("uptime_s", C.c_double)]
 
 
 class SynthOpts(C.Structure):
@@ -131,6 +150,13 @@ _sig("cms_write_db", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, _i32p, 
 _sig("cms_write_table", C.c_int, [C.c_char_p, C.c_int, C.c_int, _i64p, _i32p, _f32p])
 _sig("cms_free", None, [_vp])
 _sig("cms_version", C.c_char_p, [])
+_sig("cms_server_opts_default", None, [C.POINTER(ServerOpts)])
+_sig("cms_server_create", C.c_int, [_vp, C.POINTER(ServerOpts), C.POINTER(_vp)])
+_sig("cms_server_submit", C.c_int, [_vp, C.c_int, _i64p, _i32p, _f32p, _vp, C.POINTER(C.c_int64)])
+_sig("cms_server_wait", C.c_int, [_vp, C.c_int64, C.c_double, C.POINTER(RequestInfo)])
+_sig("cms_server_drain", C.c_int, [_vp, C.c_double])
+_sig("cms_server_get_stats", C.c_int, [_vp, C.POINTER(ServerStats)])
+_sig("cms_server_destroy", None, [_vp])
 
 #: every symbol declared in include/cms.h (checked by the CPU test suite)
 EXPORTED = [
@@ -143,7 +169,8 @@ EXPORTED = [
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
     "cms_table_feature_name", "cms_table_skipped", "cms_table_abundance", "cms_write_dist",
     "cms_synth_tree", "cms_synth_samples", "cms_synth_copy_numbers", "cms_counts_to_abundance_csr",
-    "cms_write_db", "cms_write_table", "cms_free", "cms_version",
+    "cms_write_db", "cms_write_table", "cms_free", "cms_version", "cms_server_opts_default", "cms_server_create",
+    "cms_server_submit", "cms_server_wait", "cms_server_drain", "cms_server_get_stats", "cms_server_destroy",
 ]
 
 
@@ -534,6 +561,86 @@ class Context:
         s = Stats()
         _check(lib.cms_get_stats(self._h, C.byref(s)), self._h)
         return s
+
+
+# ------------------------------------------------------------------ serving
+
+class Request:
+    """A submitted query set: `out` (nq x n_ref) is filled when it finishes."""
+
+    def __init__(self, server, rid: int, out: np.ndarray, keep):
+        self.server, self.id, self.out, self._keep = server, rid, out, keep
+        self.info = None
+
+    def wait(self, timeout: float | None = None) -> np.ndarray:
+        """Block until finished (timeout in seconds; None = forever); returns out."""
+        if self.info is None:
+            info = RequestInfo()
+            rc = lib.cms_server_wait(self.server._h, self.id, -1.0 if timeout is None else float(timeout),
+                                     C.byref(info))
+            if rc == E_TIMEOUT:
+                raise TimeoutError(last_error())
+            self.info = info
+            self._keep = None
+            self.server._inflight.pop(self.id, None)
+            _check(rc)
+        return self.out
+
+
+class Server:
+    """Serving loop over a context's samples as resident references
+    (cms_server_create). The context stays read-only while the server lives."""
+
+    def __init__(self, ctx: "Context", slots: int = 1024, bands: int = 16, memory_budget: int = 0, variant: int = 0):
+        o = ServerOpts()
+        lib.cms_server_opts_default(C.byref(o))
+        o.slots, o.bands, o.memory_budget, o.variant = slots, bands, memory_budget, variant
+        h = _vp()
+        _check(lib.cms_server_create(ctx._h, C.byref(o), C.byref(h)), ctx._h)
+        self._h = h
+        self.ctx = ctx
+        self.n_ref = ctx.n
+        self._inflight = {}  # id -> Request: output buffers the server still writes
+
+    def submit(self, q: "Samples") -> Request:
+        rowptr = np.ascontiguousarray(q.rowptr, np.int64)
+        col = np.ascontiguousarray(q.col, np.int32)
+        val = np.ascontiguousarray(q.val, np.float32)
+        out = np.empty((q.n, self.n_ref), np.float32)
+        rid = C.c_int64()
+        _check(lib.cms_server_submit(self._h, q.n, rowptr, col if len(col) else np.zeros(1, np.int32),
+                                     val if len(val) else np.zeros(1, np.float32),
+                                     out.ctypes.data_as(C.c_void_p), C.byref(rid)))
+        req = Request(self, rid.value, out, (rowptr, col, val))
+        self._inflight[req.id] = req
+        return req
+
+    def drain(self, timeout: float | None = None):
+        rc = lib.cms_server_drain(self._h, -1.0 if timeout is None else float(timeout))
+        if rc == E_TIMEOUT:
+            raise TimeoutError(last_error())
+        _check(rc)
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib.cms_server_get_stats(self._h, C.byref(st)))
+        return st
+
+    def close(self):
+        """Finish every submitted set (their outputs are filled), then free."""
+        if getattr(self, "_h", None):
+            lib.cms_server_destroy(self._h)
+            self._h = None
+            self._inflight.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 def packed_index(m, n, N):

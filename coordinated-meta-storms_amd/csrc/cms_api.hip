@@ -88,24 +88,32 @@ int check_ready(cms_ctx *c) {
     return CMS_OK;
 }
 
-// Block-leaf-major walk layout of the uploaded CSR samples (zeros elsewhere),
-// samples at their stored positions when `binned` (shared-minimum only).
-int ensure_walk_layout(cms_ctx *c, int storage, bool binned = false) {
-    if (c->ab_storage == storage && c->ab_binned == binned) return CMS_OK;
-    c->ab_storage = -1;
+// Block-leaf-major walk layout of stored positions [p0, p0 + cnt) in d_ab
+// (zeros elsewhere): the samples stored there are h_perm's when `binned`.
+int build_walk_slice(cms_ctx *c, int storage, bool binned, int p0, int cnt) {
     const int64_t nlp = (int64_t)c->tree.leaf_stream.size();
-    const int64_t nblk = (c->n + kWalkBlock - 1) / kWalkBlock;
+    const int64_t nblk = ((int64_t)cnt + kWalkBlock - 1) / kWalkBlock;
     const size_t elem = storage == CMS_STORE_F32 ? 4 : 2;
     const size_t bytes = (size_t)std::max<int64_t>(nblk, 1) * (size_t)std::max<int64_t>(nlp, 1) * kWalkBlock * elem;
     HIPCALL(c, dreserve(c->d_ab, c->cap_ab, bytes));
     HIPCALL(c, hipMemsetAsync(c->d_ab, 0, bytes, c->stream));
+    HIPCALL(c, launch_walk_scatter(c->d_rowptr, c->d_col, c->d_val, c->d_leaf_lp_ptr, c->d_leaf_lp, nlp, storage,
+                                   c->d_scale, binned ? c->d_perm : nullptr, p0, cnt, c->d_ab, c->stream));
+    return CMS_OK;
+}
+
+// Walk layout of all samples, at their stored positions when `binned`
+// (shared-minimum only).
+int ensure_walk_layout(cms_ctx *c, int storage, bool binned = false) {
+    if (c->ab_storage == storage && c->ab_binned == binned) return CMS_OK;
+    c->ab_storage = -1;
     if (storage != CMS_STORE_F32) {
         HIPCALL(c, dreserve(c->d_scale, c->cap_scale, (size_t)std::max(c->n, 1)));
         HIPCALL(c, dreserve(c->d_inv, c->cap_inv, (size_t)std::max(c->n, 1)));
         HIPCALL(c, launch_sample_scale(c->d_rowptr, c->d_val, c->n, storage, c->d_scale, c->d_inv, c->stream));
     }
-    HIPCALL(c, launch_walk_scatter(c->d_rowptr, c->d_col, c->d_val, c->n, c->d_leaf_lp_ptr, c->d_leaf_lp, nlp,
-                                   storage, c->d_scale, binned ? c->d_pos : nullptr, c->d_ab, c->stream));
+    int rc = build_walk_slice(c, storage, binned, 0, c->n);
+    if (rc) return rc;
     c->ab_storage = storage;
     c->ab_binned = binned;
     return CMS_OK;
@@ -173,15 +181,30 @@ int ensure_sharedmin(cms_ctx *c) {
             HIPCALL(c, hipMemcpy(c->d_pos, pos.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
         }
     }
-    rc = ensure_walk_layout(c, CMS_STORE_F32, binned);  // the embedding reads the F32 walk layout
-    if (rc) return rc;
     c->sm_binned = binned;
     const bool unw = metric == CMS_METRIC_UNWEIGHTED;
     const int64_t nrows = unw ? c->nrows_u : c->nrows;
     c->npad = ((int64_t)c->n + 63) / 64 * 64;
     c->nwords = (nrows + 31) / 32;
-    HIPCALL(c, dreserve(c->d_xroot, c->cap_xroot,
-                        (size_t)std::max<size_t>(c->tree.seg_first.size(), 1) * std::max<int64_t>(c->npad, 1)));
+    // The embedding reads the F32 walk layout. Above kEmbedSliceBytes it is
+    // built and embedded in slices of stored positions (multiples of 64
+    // samples), so the transient layout stays bounded beside Y.
+    const int64_t nlp = (int64_t)c->tree.leaf_stream.size();
+    const size_t per_sample = (size_t)std::max<int64_t>(nlp, 1) * sizeof(float);
+    int64_t slice = c->npad;
+    if (c->opts.embed_slice > 0)
+        slice = std::max<int64_t>(64, (int64_t)c->opts.embed_slice / 64 * 64);
+    else if ((size_t)c->npad * per_sample > kEmbedSliceBytes)
+        slice = std::max<int64_t>(64, (int64_t)(kEmbedSliceBytes / per_sample) / 64 * 64);
+    const int64_t nslices = c->n > 0 ? (c->n + slice - 1) / slice : 1;
+    if (nslices == 1) {
+        rc = ensure_walk_layout(c, CMS_STORE_F32, binned);
+        if (rc) return rc;
+    } else {
+        c->ab_storage = -1;  // the layout will hold the last slice only
+    }
+    const int64_t spad = std::min<int64_t>(slice, c->npad);
+    HIPCALL(c, dreserve(c->d_xroot, c->cap_xroot, (size_t)std::max<size_t>(c->tree.seg_first.size(), 1) * spad));
     HIPCALL(c, dreserve(c->d_colsum, c->cap_colsum, (size_t)std::max<int64_t>(c->npad, 1)));
     HIPCALL(c, dreserve(c->d_mass, c->cap_mass, (size_t)std::max<int64_t>(c->npad, 1)));
     EmbedData ed{};
@@ -195,20 +218,37 @@ int ensure_sharedmin(cms_ctx *c) {
     ed.top_src = c->d_top_src;
     ed.ntop = (int)c->tree.top_nodes.size();
     ed.ab = (const float *)c->d_ab;
-    ed.nlp = (int64_t)c->tree.leaf_stream.size();
-    ed.n = c->n;
-    ed.npad = c->npad;
+    ed.nlp = nlp;
     ed.nwords = c->nwords;
+    ed.ystride = c->npad;
     ed.xroot = c->d_xroot;
     ed.unweighted = unw ? 1 : 0;
-    ed.colsum = c->d_colsum;
-    ed.colsum_f64 = c->d_mass;
+    // one embedding pass over all slices (first pass: column sums only)
+    auto embed_all = [&](bool sums_only) -> int {
+        for (int64_t k = 0; k < nslices; k++) {
+            const int64_t p0 = k * slice, cnt = std::min<int64_t>(slice, c->n - p0);
+            if (nslices > 1) {
+                int r = build_walk_slice(c, CMS_STORE_F32, binned, (int)p0, (int)cnt);
+                if (r) return r;
+            }
+            EmbedData e = ed;
+            e.ab = (const float *)c->d_ab;
+            e.n = (int)cnt;
+            e.npad = (cnt + 63) / 64 * 64;
+            e.sums_only = sums_only ? 1 : 0;
+            e.Y = c->d_Y ? c->d_Y + p0 : nullptr;
+            e.bm = c->d_bm ? c->d_bm + (p0 / kSmBlock) * c->nwords : nullptr;
+            e.colsum = c->d_colsum + p0;
+            e.colsum_f64 = c->d_mass + p0;
+            HIPCALL(c, launch_embed(e, c->stream));
+        }
+        return CMS_OK;
+    };
     if (unw) {
         // column sums of y (shared weight bound) from a first, write-free pass
         HIPCALL(c, hipMemsetAsync(c->d_mass, 0, sizeof(double) * std::max<int64_t>(c->npad, 1), c->stream));
-        ed.sums_only = 1;
-        HIPCALL(c, launch_embed(ed, c->stream));
-        ed.sums_only = 0;
+        rc = embed_all(true);
+        if (rc) return rc;
     } else {
         // a column sum of y equals the sample's consumed leaf mass
         HIPCALL(c, launch_sample_mass(c->d_rowptr, c->d_col, c->d_val, c->n, c->d_leaf_uses, c->d_mass, c->stream));
@@ -219,6 +259,7 @@ int ensure_sharedmin(cms_ctx *c) {
     HIPCALL(c, hipStreamSynchronize(c->stream));
     double maxm = 0.0;
     for (int s = 0; s < c->n; s++) maxm = std::max(maxm, mass[s]);
+    c->max_mass = maxm;
     const int K = choose_scale_bits(maxm, nrows);
     c->scale_bits = K;
     const int64_t nblk = c->npad / kSmBlock;
@@ -228,9 +269,8 @@ int ensure_sharedmin(cms_ctx *c) {
     HIPCALL(c, hipMemsetAsync(c->d_bm, 0, sizeof(uint32_t) * nbm, c->stream));
     HIPCALL(c, hipMemsetAsync(c->d_colsum, 0, sizeof(unsigned long long) * std::max<int64_t>(c->npad, 1), c->stream));
     ed.scale = std::ldexp(1.0, K);
-    ed.Y = c->d_Y;
-    ed.bm = c->d_bm;
-    HIPCALL(c, launch_embed(ed, c->stream));
+    rc = embed_all(false);
+    if (rc) return rc;
     c->sm_rows = nrows;
     c->sm_metric = metric;
     c->sm_ready = true;
@@ -374,9 +414,11 @@ const char *cms_last_error(const cms_ctx *c) { return c ? c->err.c_str() : globa
 
 int cms_set_opts(cms_ctx *c, const cms_opts *o) {
     if (!c || !o) return fail(c, CMS_E_INVAL, "cms_set_opts: null argument");
+    if (c->servers > 0) return fail(c, CMS_E_STATE, "the context is serving (destroy its server first)");
     if (o->kernel != CMS_KERNEL_WALK && o->kernel != CMS_KERNEL_SHAREDMIN) return fail(c, CMS_E_INVAL, "unknown kernel");
     if (o->metric != CMS_METRIC_WEIGHTED && o->metric != CMS_METRIC_UNWEIGHTED) return fail(c, CMS_E_INVAL, "unknown metric");
     if (o->storage < CMS_STORE_F32 || o->storage > CMS_STORE_F16) return fail(c, CMS_E_INVAL, "unknown storage");
+    if (o->embed_slice < 0) return fail(c, CMS_E_INVAL, "embed_slice must be >= 0");
     c->opts = *o;
     return CMS_OK;
 }
@@ -384,6 +426,7 @@ int cms_set_opts(cms_ctx *c, const cms_opts *o) {
 int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const float *d1, const int32_t *o2,
                  const float *d2, const int32_t *od) {
     if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (c->servers > 0) return fail(c, CMS_E_STATE, "the context is serving (destroy its server first)");
     std::string err;
     CompiledTree t;
     int rc = compile_tree(leaf_n, order_n, o1, d1, o2, d2, od, kWalkLeafChunk, t, err);
@@ -463,6 +506,7 @@ int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const f
     if (!lp.empty()) HIPCALL(c, hipMemcpy(c->d_leaf_lp, lp.data(), sizeof(int32_t) * lp.size(), hipMemcpyHostToDevice));
     HIPCALL(c, hipMemcpy(c->d_leaf_uses, uses.data(), sizeof(int32_t) * uses.size(), hipMemcpyHostToDevice));
     c->tree = std::move(t);
+    c->h_leaf_uses = std::move(uses);
     c->nrows = nrows;
     c->nrows_u = nrows_u;
     c->has_tree = true;
@@ -473,6 +517,7 @@ int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const f
 
 int cms_set_samples_csr(cms_ctx *c, int n, const int64_t *rowptr, const int32_t *leaf, const float *val) {
     if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (c->servers > 0) return fail(c, CMS_E_STATE, "the context is serving (destroy its server first)");
     if (!c->has_tree) return fail(c, CMS_E_STATE, "cms_set_samples before cms_set_tree");
     if (n < 0 || !rowptr || (rowptr[n] > 0 && (!leaf || !val))) return fail(c, CMS_E_INVAL, "cms_set_samples_csr: bad arrays");
     if (rowptr[0] != 0) return fail(c, CMS_E_INVAL, "rowptr[0] must be 0");
@@ -497,6 +542,7 @@ int cms_set_samples_csr(cms_ctx *c, int n, const int64_t *rowptr, const int32_t 
 
 int cms_set_samples(cms_ctx *c, int n, const float *abd) {
     if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (c->servers > 0) return fail(c, CMS_E_STATE, "the context is serving (destroy its server first)");
     if (!c->has_tree) return fail(c, CMS_E_STATE, "cms_set_samples before cms_set_tree");
     if (n < 0 || (n > 0 && c->tree.leaf_n > 0 && !abd)) return fail(c, CMS_E_INVAL, "cms_set_samples: bad arrays");
     const int L = c->tree.leaf_n;
@@ -682,6 +728,7 @@ int cms_calibrate_copy(cms_ctx *c, int64_t bytes, int reps, double *ms) {
 // bitmaps), reusing allocations.
 int cms_prepare(cms_ctx *c, int force) {
     if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (c->servers > 0) return fail(c, CMS_E_STATE, "the context is serving (destroy its server first)");
     if (force) {
         c->ab_storage = -1;
         c->sm_ready = false;
@@ -924,3 +971,25 @@ int cms_get_stats(cms_ctx *c, cms_stats *s) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- internal (serving loop)
+
+namespace cms {
+
+int ctx_prepare(cms_ctx *c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    return ensure_sharedmin(c);
+}
+
+SmData ctx_sm_data(const cms_ctx *c) { return sm_data(c); }
+
+int ctx_fail(cms_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
+
+void ctx_release_walk_layout(cms_ctx *c) {
+    dfree(c->d_ab);
+    c->cap_ab = 0;
+    c->ab_storage = -1;
+}
+
+}  // namespace cms

@@ -65,6 +65,9 @@ struct cms_ctx {
     size_t cap_Y = 0, cap_bm = 0, cap_mass = 0;
     int64_t npad = 0, nwords = 0;
     int scale_bits = 0;
+    double max_mass = 0.0;  // largest sample column sum of y (FP64) that chose scale_bits
+    int servers = 0;        // serving loops reading this context's layout
+    std::vector<int32_t> h_leaf_uses;  // per leaf: leaf-stream positions consuming it
     unsigned long long *d_counter = nullptr;  // entries visited by shared-minimum tiles
 
     float *d_out = nullptr;
@@ -80,4 +83,6 @@ namespace cms {
 int ctx_prepare(cms_ctx *c);
 SmData ctx_sm_data(const cms_ctx *c);
 int ctx_fail(cms_ctx *c, int code, const std::string &msg);
+// Free the walk layout (rebuilt on demand by later calls).
+void ctx_release_walk_layout(cms_ctx *c);
 }  // namespace cms

@@ -30,6 +30,7 @@
 #include <algorithm>
 
 #include "kernels.h"
+#include "sm_tile.h"
 
 namespace cms {
 
@@ -111,7 +112,8 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
                                                        const int32_t *__restrict__ seg_first,
                                                        const int32_t *__restrict__ seg_last, int order_n,
                                                        const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
-                                                       int64_t nwords, double scale, uint32_t *__restrict__ Y,
+                                                       int64_t nwords, int64_t ystride, double scale,
+                                                       uint32_t *__restrict__ Y,
                                                        uint32_t *__restrict__ bm, double *__restrict__ xroot,
                                                        unsigned long long *__restrict__ colsum,
                                                        double *__restrict__ colsum_f64) {
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
                     continue;
                 }
                 const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
-                Y[(int64_t)rows[j] * npad + s] = q;
+                Y[(int64_t)rows[j] * ystride + s] = q;
                 qsum += q;
                 bw.add(rows[j], q, bm, blk0, nwords, lane);
             }
@@ -201,7 +203,8 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
                                                        const int32_t *__restrict__ top_nodes,
                                                        const int32_t *__restrict__ top_src, int ntop,
                                                        const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
-                                                       int64_t nwords, double scale, uint32_t *__restrict__ Y,
+                                                       int64_t nwords, int64_t ystride, double scale,
+                                                       uint32_t *__restrict__ Y,
                                                        uint32_t *__restrict__ bm, const double *__restrict__ xroot,
                                                        unsigned long long *__restrict__ colsum,
                                                        double *__restrict__ colsum_f64) {
@@ -244,7 +247,7 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
                 continue;
             }
             const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
-            Y[(int64_t)rows[j] * npad + s] = q;
+            Y[(int64_t)rows[j] * ystride + s] = q;
             qsum += q;
             bw.add(rows[j], q, bm, blk0, nwords, lane);
         }
@@ -263,104 +266,23 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
 
 // ------------------------------------------------------------ pair tiles
 
-// One wave = 32 x 32 pairs, lane (ty, tx) = 4 rows x 4 columns.
-//
-// Visit the entries set in both blocks' bitmaps (OR of two blocks when a
-// side is not block-aligned): per 64 bitmap words, compact the set bits into
-// an LDS list, stage their 2 x 32 Y values in LDS CHUNK rows at a time, and
-// accumulate min/add into the lane's 4 x 4 registers. Returns the number of
-// entries visited.
+// Square tile of the all-pairs matrix: lanes 0..31 stage the row block's
+// columns of Y, lanes 32..63 the column block's.
 template <int CHUNK>
-__device__ __forceinline__ uint32_t sm_tile_accumulate(const SmData &d, const uint32_t *A0, const uint32_t *A1,
-                                                       const uint32_t *B0, const uint32_t *B1, int64_t abase,
-                                                       int64_t bbase, uint16_t *L, uint32_t *S, int lane,
-                                                       uint32_t (&acc)[4][4]) {
-    const int ty = lane >> 3, tx = lane & 7;
-    // Column this lane stages: lanes 0..31 the row block, 32..63 the column block.
+__device__ __forceinline__ uint32_t sm_square_tile(const SmData &d, const uint32_t *A0, const uint32_t *A1,
+                                                   const uint32_t *B0, const uint32_t *B1, int64_t abase,
+                                                   int64_t bbase, uint16_t *L, uint32_t *S, int lane,
+                                                   uint32_t (&acc)[4][4]) {
     const int64_t my_s = (lane < 32) ? abase + lane : bbase + (lane - 32);
     const bool s_ok = my_s < d.npad;
-    const uint32_t *Ycol = d.Y + (s_ok ? my_s : 0);
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0u;
-    uint32_t visited = 0;
-    for (int64_t wb = 0; wb < d.nwords; wb += 64) {
-        const int64_t wi = wb + lane;
-        uint32_t w = 0u;
-        if (wi < d.nwords) {
-            uint32_t a = A0[wi], b = B0[wi];
-            if (A1) a |= A1[wi];
-            if (B1) b |= B1[wi];
-            w = a & b;
-        }
-        const int cnt = __popc(w);
-        int incl = cnt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            int v = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += v;
-        }
-        const int tot = __shfl(incl, 63, 64);
-        if (tot == 0) continue;
-        visited += (uint32_t)tot;
-        int pos = incl - cnt;
-        __builtin_amdgcn_wave_barrier();
-        while (w) {
-            const int b = __ffs(w) - 1;
-            L[pos++] = (uint16_t)(lane * 32 + b);
-            w &= w - 1u;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t *Ywb = Ycol + wb * 32 * d.npad;
-        for (int j = 0; j < tot; j += CHUNK) {
-            uint32_t v[CHUNK];
-#pragma unroll
-            for (int k = 0; k < CHUNK; k++) {
-                v[k] = 0u;
-                if (j + k < tot && s_ok) v[k] = Ywb[(int64_t)L[j + k] * d.npad];
-            }
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int k = 0; k < CHUNK; k++) S[k * 64 + lane] = v[k];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int kn = min(CHUNK, tot - j);
-            for (int k = 0; k < kn; k += 2) {
-                const uint4 a0 = *reinterpret_cast<const uint4 *>(S + k * 64 + ty * 4);
-                const uint4 b0 = *reinterpret_cast<const uint4 *>(S + k * 64 + 32 + tx * 4);
-                const uint4 a1 = *reinterpret_cast<const uint4 *>(S + (k + 1) * 64 + ty * 4);
-                const uint4 b1 = *reinterpret_cast<const uint4 *>(S + (k + 1) * 64 + 32 + tx * 4);
-                const uint32_t av0[4] = {a0.x, a0.y, a0.z, a0.w}, bv0[4] = {b0.x, b0.y, b0.z, b0.w};
-                const uint32_t av1[4] = {a1.x, a1.y, a1.z, a1.w}, bv1[4] = {b1.x, b1.y, b1.z, b1.w};
-#pragma unroll
-                for (int ii = 0; ii < 4; ii++)
-#pragma unroll
-                    for (int jj = 0; jj < 4; jj++)
-                        acc[ii][jj] = acc[ii][jj] + min(av0[ii], bv0[jj]) + min(av1[ii], bv1[jj]);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-        }
-    }
-    return visited;
+    return sm_tile_accumulate<CHUNK>(A0, A1, B0, B1, 0, d.nwords, d.Y + (s_ok ? my_s : 0), d.npad, s_ok, L, S,
+                                     lane, acc);
 }
 
 // Similarity of the pair at stored positions (pa, pb) from its shared sum.
 __device__ __forceinline__ float sm_finish(const SmData &d, uint32_t acc, int64_t pa, int64_t pb) {
-    double t;
-    if (d.unweighted) {
-        // shared weight / (weight of a + weight of b - shared weight)
-        const unsigned long long den = d.colsum[pa] + d.colsum[pb] - acc;
-        t = den ? (double)acc / (double)den : __builtin_nan("");
-    } else {
-        t = (double)acc * d.inv_scale;
-    }
-    t = t > 1.0 ? 1.0 : t;
-    return (float)t;
+    return d.unweighted ? sm_similarity(true, d.inv_scale, acc, d.colsum[pa], d.colsum[pb])
+                        : sm_similarity(false, d.inv_scale, acc, 0ull, 0ull);
 }
 
 // Rectangle with arbitrary (unpermuted) sample offsets: rows [r0, r0+nr) x
@@ -379,7 +301,7 @@ __global__ __launch_bounds__(64 * kSmWaves) void sm_rect_kernel(SmData d, int64_
     const uint32_t *A1 = ((abase % kSmBlock) && ba + 1 < nblk) ? d.bm + (ba + 1) * d.nwords : nullptr;
     const uint32_t *B1 = ((bbase % kSmBlock) && bb + 1 < nblk) ? d.bm + (bb + 1) * d.nwords : nullptr;
     uint32_t acc[4][4];
-    sm_tile_accumulate<kSmChunk>(d, A0, A1, B0, B1, abase, bbase, s_list[wave], s_stage[wave], lane, acc);
+    sm_square_tile<kSmChunk>(d, A0, A1, B0, B1, abase, bbase, s_list[wave], s_stage[wave], lane, acc);
     const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
     for (int ii = 0; ii < 4; ii++)
@@ -419,7 +341,7 @@ __global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int
         sm_decode_upper(t_begin + (int64_t)t, nb, ra, cb);
         const int64_t abase = ra * kSmBlock, bbase = cb * kSmBlock;
         uint32_t acc[4][4];
-        visited += sm_tile_accumulate<CHUNK>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords, nullptr, abase,
+        visited += sm_square_tile<CHUNK>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords, nullptr, abase,
                                              bbase, s_list[wave], s_stage[wave], lane, acc);
 #pragma unroll
         for (int ii = 0; ii < 4; ii++)
@@ -509,17 +431,18 @@ static hipError_t launch_embed_t(const EmbedData &e, hipStream_t st) {
     if (lds + sizeof(float) * kEmbedLeafChunk * 64 + sizeof(EmbedNode) * kEmbedNodeChunk > 65536)
         return hipErrorInvalidConfiguration;
     const unsigned waves = (unsigned)(e.npad / 64);
+    const int64_t ystride = e.ystride ? e.ystride : e.npad;
     if (e.nseg > 0) {
         embed_seg_kernel<UNW, SUMS><<<dim3(waves, (unsigned)e.nseg), 64, lds, st>>>(
-            e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.nlp, e.n, e.npad, e.nwords, e.scale, e.Y, e.bm,
-            e.xroot, e.colsum, e.colsum_f64);
+            e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.nlp, e.n, e.npad, e.nwords, ystride, e.scale, e.Y,
+            e.bm, e.xroot, e.colsum, e.colsum_f64);
         hipError_t r = hipGetLastError();
         if (r != hipSuccess) return r;
     }
     if (e.ntop > 0)
         embed_top_kernel<UNW, SUMS><<<waves, 64, lds, st>>>(e.nodes, e.top_nodes, e.top_src, e.ntop, e.ab, e.nlp,
-                                                            e.n, e.npad, e.nwords, e.scale, e.Y, e.bm, e.xroot,
-                                                            e.colsum, e.colsum_f64);
+                                                            e.n, e.npad, e.nwords, ystride, e.scale, e.Y, e.bm,
+                                                            e.xroot, e.colsum, e.colsum_f64);
     return hipGetLastError();
 }
 

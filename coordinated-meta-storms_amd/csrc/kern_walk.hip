@@ -303,13 +303,13 @@ __global__ void sample_scale_kernel(const int64_t *__restrict__ rowptr, const fl
 }
 
 __global__ void walk_scatter_kernel(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                                    const float *__restrict__ val, int n, const int32_t *__restrict__ leaf_lp_ptr,
+                                    const float *__restrict__ val, const int32_t *__restrict__ leaf_lp_ptr,
                                     const int32_t *__restrict__ leaf_lp, int64_t nlp, int storage,
-                                    const float *__restrict__ scale, const int32_t *__restrict__ pos,
-                                    void *__restrict__ ab) {
-    const int s = blockIdx.x;
-    if (s >= n) return;
-    const int p = pos ? pos[s] : s;  // stored position of sample s
+                                    const float *__restrict__ scale, const int32_t *__restrict__ perm, int p0,
+                                    int cnt, void *__restrict__ ab) {
+    const int p = blockIdx.x;  // position in the layout
+    if (p >= cnt) return;
+    const int s = perm ? perm[p0 + p] : p0 + p;  // sample stored there
     const int64_t base = (int64_t)(p >> 3) * nlp * 8 + (p & 7);
     const float sc = (storage == CMS_STORE_F32) ? 1.0f : scale[s];
     for (int64_t k = rowptr[s] + threadIdx.x; k < rowptr[s + 1]; k += blockDim.x) {
@@ -344,11 +344,12 @@ hipError_t launch_sample_scale(const int64_t *rowptr, const float *val, int n, i
     return hipGetLastError();
 }
 
-hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const float *val, int n,
+hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const float *val,
                                const int32_t *leaf_lp_ptr, const int32_t *leaf_lp, int64_t nlp, int storage,
-                               const float *scale, const int32_t *pos, void *ab, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    walk_scatter_kernel<<<n, 256, 0, st>>>(rowptr, col, val, n, leaf_lp_ptr, leaf_lp, nlp, storage, scale, pos, ab);
+                               const float *scale, const int32_t *perm, int p0, int cnt, void *ab, hipStream_t st) {
+    if (cnt <= 0) return hipSuccess;
+    walk_scatter_kernel<<<cnt, 256, 0, st>>>(rowptr, col, val, leaf_lp_ptr, leaf_lp, nlp, storage, scale, perm, p0, cnt,
+                                             ab);
     return hipGetLastError();
 }
 

@@ -39,12 +39,17 @@ int walk_max_slots();
 constexpr int kEmbedMaxSlots = 56;
 // Largest subtree segment (nodes) one embedding wave walks.
 constexpr int kEmbedSegMax = 512;
+// Largest transient F32 walk layout built for the embedding; larger sample
+// sets are embedded in slices of stored positions.
+constexpr size_t kEmbedSliceBytes = (size_t)16 << 30;
 
-// Scatter CSR samples into the walk layout (buffer zeroed by the caller).
+// Scatter CSR samples into the walk layout (buffer zeroed by the caller):
+// layout position p in [0, cnt) receives sample perm[p0 + p] (p0 + p when
+// perm is null), so a slice of stored positions can be laid out alone.
 //   leaf_lp_ptr/leaf_lp: leaf -> leaf-stream positions (CSR over leaves)
-hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const float *val, int n,
-                               const int32_t *leaf_lp_ptr, const int32_t *leaf_lp, int64_t nlp,
-                               int storage, const float *scale, const int32_t *pos, void *ab, hipStream_t st);
+hipError_t launch_walk_scatter(const int64_t *rowptr, const int32_t *col, const float *val,
+                               const int32_t *leaf_lp_ptr, const int32_t *leaf_lp, int64_t nlp, int storage,
+                               const float *scale, const int32_t *perm, int p0, int cnt, void *ab, hipStream_t st);
 // Per-sample power-of-two scale for 16-bit storage from CSR values.
 hipError_t launch_sample_scale(const int64_t *rowptr, const float *val, int n, int storage, float *scale,
                                float *inv_scale, hipStream_t st);
@@ -127,8 +132,9 @@ struct EmbedData {
     const float *ab;  // F32 walk layout
     int64_t nlp;
     int n;
-    int64_t npad;
+    int64_t npad;     // samples rounded up to 64 (one wave each); xroot stride
     int64_t nwords;
+    int64_t ystride;  // row stride of Y (0 = npad)
     double scale;     // 2^K
     uint32_t *Y;
     uint32_t *bm;     // zeroed by the caller; bits are ORed in
@@ -145,5 +151,44 @@ hipError_t launch_sm_rect(const SmData &d, int r0, int nr, int c0, int nc, float
 // Explicit pair list.
 hipError_t launch_sm_pairs(const SmData &d, int64_t n_pairs, const int32_t *row, const int32_t *col, float *out,
                            hipStream_t st);
+
+// ------------------------------------------------------------------ serving
+
+// Query slots (columns of QY, same rows as the references' Y) against the
+// resident references.
+struct CrossData {
+    const uint32_t *QY;                 // nrows x qstride, slot = column
+    int64_t qstride;                    // slots (multiple of 32)
+    const uint32_t *qbm;                // (qstride / 32) x nwords query-block bitmaps
+    const unsigned long long *qcolsum;  // per slot (unweighted)
+    const uint32_t *Y;                  // references, nrows x npad
+    const uint32_t *bm;                 // reference-block bitmaps
+    const unsigned long long *colsum;   // per stored reference (unweighted)
+    const int32_t *perm;                // stored reference -> caller's reference id (null = identity)
+    int64_t npad;
+    int64_t nwords;
+    int n_ref;
+    int unweighted;
+    double inv_scale;                   // 1 / (100 * 2^K)
+    unsigned long long *counter;        // optional: entries visited
+};
+
+// Step: tiles (active query block qblocks[k]) x (reference blocks
+// [rb0, rb0 + nrb)); slot_out[slot] = the output row of the query in that slot
+// (indexed by caller's reference id), null for an inactive slot. `work` is a
+// zeroed device counter. Variants as launch_sm_persistent. When the step has
+// too few tiles to fill the device, tile rows are split over several waves
+// whose sums meet in part_acc (part_tiles x 1024 zeroed uint32, left zeroed)
+// and a second kernel writes the similarities.
+hipError_t launch_sm_cross(const CrossData &x, int variant, const int32_t *qblocks, int nqa, int64_t rb0, int64_t nrb,
+                           float *const *slot_out, uint32_t *part_acc, int64_t part_tiles, unsigned long long *work,
+                           int grid_waves, hipStream_t st);
+// Bitmaps of the listed query blocks from their active slots (masks[k] bit j
+// = slot 32 blocks[k] + j is active).
+hipError_t launch_qbm_build(const uint32_t *QY, int64_t qstride, int64_t nrows, int64_t nwords, const int32_t *blocks,
+                            const uint32_t *masks, int nblocks, uint32_t *qbm, hipStream_t st);
+// QY[:, slots[q]] = Ys[:, q] and qcs[slots[q]] = cs[q] for q < nq.
+hipError_t launch_pack(const uint32_t *Ys, int64_t sstride, int nq, const int32_t *slots, int64_t nrows, uint32_t *QY,
+                       int64_t qstride, const unsigned long long *cs, unsigned long long *qcs, hipStream_t st);
 
 }  // namespace cms

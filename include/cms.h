@@ -40,7 +40,9 @@ enum {
     CMS_E_SCHEDULE = -4,   /* order.txt schedule invalid (slot out of range, read before write, leaf out of range) */
     CMS_E_STATE = -5,      /* call out of order (no tree / no samples loaded) */
     CMS_E_IO = -6,         /* file could not be opened or parsed */
-    CMS_E_UNSUPPORTED = -7 /* valid input that the selected kernel does not handle */
+    CMS_E_UNSUPPORTED = -7, /* valid input that the selected kernel does not handle */
+    CMS_E_RANGE = -8,       /* sample mass outside the fixed-point range of the resident scale */
+    CMS_E_TIMEOUT = -9      /* cms_server_wait: the request has not finished yet */
 };
 
 /* Kernel families. */
@@ -73,7 +75,8 @@ typedef struct cms_opts {
     int bin;          /* SHAREDMIN: store samples binned by sparsity (densest first) */
     int autotune;     /* SHAREDMIN: time the persistent-kernel variants once, keep the fastest */
     int variant;      /* SHAREDMIN: 1..6 force a persistent-kernel variant (0 = default/tuned) */
-    int reserved[4];
+    int embed_slice;  /* SHAREDMIN: samples per embedding slice, multiple of 64 (0 = by a 16 GiB layout) */
+    int reserved[3];
 } cms_opts;
 
 typedef struct cms_ctx cms_ctx;
@@ -176,6 +179,79 @@ typedef struct cms_stats {
     int64_t bands;             /* bands of the last streamed run */
 } cms_stats;
 int cms_get_stats(cms_ctx *c, cms_stats *s);
+
+/* ------------------------------------------------------------- serving
+ *
+ * No counterpart in the reference (its only mode is one all-pairs run per
+ * process, cms_cuda.cu:1470-2235). A server keeps the shared-minimum layout
+ * of a context's samples (the references) resident in HBM and compares
+ * query sets against all of them:
+ *   - query sets are queued FIFO and admitted while query slots and a
+ *     device-memory budget (an arena allocated at creation) have room;
+ *   - admitted sets are embedded on their own stream while the device runs
+ *     steps; a step compares every active query with one band of reference
+ *     blocks; a set joins at the next step boundary, is compared with each
+ *     band exactly once over `bands` consecutive steps, and leaves (its slots
+ *     and memory are reused) — continuous batching over the band cycle;
+ *   - results are exact integer sums of the fixed-point embedding with the
+ *     references' scale, so a query's output does not depend on what else
+ *     is in flight.
+ * The context must outlive the server and is read-only while it exists
+ * (cms_set_* and cms_prepare return CMS_E_STATE). */
+typedef struct cms_server cms_server;
+
+typedef struct cms_server_opts {
+    int slots;             /* query slots resident at once, multiple of 32 (default 1024) */
+    int bands;             /* reference bands per cycle (default 16) */
+    int64_t memory_budget; /* bytes of the per-request arena (0 = 8 GiB or less if the device has less free) */
+    int variant;           /* cross-kernel variant 1..6 as cms_opts.variant (0 = default) */
+    int reserved[3];
+} cms_server_opts;
+
+typedef struct cms_request_info {
+    int rc;                /* CMS_OK or the request's error */
+    int nq;                /* queries */
+    int64_t join_step;     /* step at which it joined the cycle (-1 = never) */
+    double t_submit;       /* seconds on the server's clock */
+    double t_admit;        /* slots and memory granted, embedding queued */
+    double t_join;         /* first step launched */
+    double t_done;         /* output delivered to the caller's buffer */
+} cms_request_info;
+
+typedef struct cms_server_stats {
+    int64_t submitted, completed, failed;
+    int64_t queued, active;        /* now */
+    int64_t steps;                 /* steps launched */
+    double kernel_ms;              /* device time of the cross kernels (finished steps) */
+    double pairs;                  /* query x reference pairs of finished requests */
+    double pair_entries;           /* entries visited by the cross tiles (x 1024 pairs) */
+    int64_t arena_bytes, arena_peak;
+    int slots, slots_peak, bands;
+    double uptime_s;
+} cms_server_stats;
+
+void cms_server_opts_default(cms_server_opts *o);
+/* Build (if needed) the context's shared-minimum layout for its metric and
+ * start the serving threads. The walk layout of the context is released. */
+int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out);
+/* Queue nq query samples (CSR over leaves, abundances as for
+ * cms_set_samples_csr; copied before return). When the request finishes,
+ * out (host, nq x n_ref floats) holds row q = similarity of query q to every
+ * reference in the context's sample order. Fails at once (and queues
+ * nothing) on bad input, on a set that can never fit (nq > slots or memory
+ * beyond the budget), on negative values, or on a query whose mass exceeds
+ * the references' fixed-point range (CMS_E_RANGE). */
+int cms_server_submit(cms_server *s, int nq, const int64_t *rowptr, const int32_t *leaf, const float *val,
+                      float *out, int64_t *id);
+/* Wait up to timeout_s (< 0: forever) for request id; CMS_E_TIMEOUT if it
+ * is still running, else its rc. info may be NULL. A finished request is
+ * forgotten after the first successful wait. */
+int cms_server_wait(cms_server *s, int64_t id, double timeout_s, cms_request_info *info);
+/* Wait until every submitted request has finished. */
+int cms_server_drain(cms_server *s, double timeout_s);
+int cms_server_get_stats(cms_server *s, cms_server_stats *st);
+/* Stops accepting requests, finishes the ones in flight, frees the arena. */
+void cms_server_destroy(cms_server *s);
 
 /* ------------------------------------------------- host loader (no GPU) */
 

@@ -103,3 +103,18 @@ def test_binned_tile_and_pairs(cms, data):
         for k in range(4):
             a, b = sorted((int(rows[k]), int(cols[k])))
             assert got[k] == ref[cms.packed_index(a, b, n)]
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_sliced_embedding_identical(cms, data, metric):
+    # the embedding built in slices of 64 stored positions (as for sample
+    # sets whose walk layout exceeds the slice budget) gives the same integers
+    print(f"sliced embedding metric={metric}", flush=True)
+    tree, abd = data
+    ref = _plain(cms, tree, abd, metric)
+    for bin_ in (0, 1):
+        with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric, bin=bin_, embed_slice=64) as ctx:
+            ctx.set_tree(tree)
+            ctx.set_samples_csr(abd)
+            got = ctx.all_pairs()
+        np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))

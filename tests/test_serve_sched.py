@@ -1,0 +1,163 @@
+"""Serving scheduler policy on the CPU (no GPU): the arena allocator, FIFO
+admission by query slots and device memory, and the band cycle of
+continuous batching. The scheduler source (csrc/serve_sched.cpp) is the one
+linked into libcms; tests/sched_harness.cpp drives it tick by tick."""
+import os
+import random
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "coordinated-meta-storms_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("sched") / "sched_harness")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", CSRC, os.path.join(ROOT, "tests", "sched_harness.cpp"),
+                    os.path.join(CSRC, "serve_sched.cpp"), "-o", exe], check=True)
+    return exe
+
+
+def run_sim(exe, slots, arena, bands, embed_ticks, deliver_ticks, reqs):
+    text = f"{slots} {arena} {bands} {embed_ticks} {deliver_ticks}\n" + "".join(
+        f"{t} {nq} {sc} {out}\n" for t, nq, sc, out in reqs)
+    r = subprocess.run([exe, "sim"], input=text, capture_output=True, text=True, check=True)
+    ev = {"submit": [], "admit": [], "join": [], "step": [], "finish": [], "deliver": [], "arena": [], "blocks": []}
+    end = None
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if f[0] == "end":
+            end = (int(f[1]), int(f[2]))
+        else:
+            ev[f[0]].append(f[1:])
+    return ev, end
+
+
+def check_cycle(ev, slots, arena, bands):
+    """Invariants every run must satisfy."""
+    joined = {int(f[1]): int(f[2]) for f in ev["join"]}
+    finished = {int(f[1]): int(f[2]) for f in ev["finish"]}
+    admitted = [int(f[1]) for f in ev["admit"]]
+    accepted = [int(f[1]) for f in ev["submit"] if f[2] == "ok"]
+    # FIFO admission
+    assert admitted == sorted(admitted) == accepted
+    # every set runs exactly `bands` consecutive steps and sees each band once
+    steps = {int(f[1]): (int(f[2]), {int(a.split(":")[0]): int(a.split(":")[1]) for a in f[3:]}) for f in ev["step"]}
+    slot_n = {int(f[1]): len(f) - 2 for f in ev["admit"]}
+    for rid, j in joined.items():
+        assert finished[rid] == j + bands - 1
+        seen = []
+        for k in range(j, j + bands):
+            band, per = steps[k]
+            assert per.get(rid) == slot_n[rid]  # all its queries active in each of its steps
+            seen.append(band)
+        assert sorted(seen) == list(range(bands))
+    # no set outside its window
+    for k, (band, per) in steps.items():
+        assert band == k % bands
+        assert sum(per.values()) <= slots
+        for rid in per:
+            assert joined[rid] <= k <= finished[rid]
+    # slots are never shared: a set admitted before another finished (within a
+    # tick, admission runs before the step that finishes sets) holds others
+    adm = {int(f[1]): (int(f[0]), {int(x) for x in f[2:]}) for f in ev["admit"]}
+    fin_tick = {int(f[1]): int(f[0]) for f in ev["finish"]}
+    ids = sorted(adm)
+    for i, a in enumerate(ids):
+        for b in ids[i + 1:]:
+            if adm[b][0] <= fin_tick[a]:
+                assert not (adm[a][1] & adm[b][1]), (a, b)
+    # arena consistent and within capacity at every tick; empty at the end
+    for f in ev["arena"]:
+        assert f[2] == "1"
+        assert 0 <= int(f[1]) <= arena
+    assert int(ev["arena"][-1][1]) == 0
+    assert int(ev["arena"][-1][3]) == slots
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_arena_matches_brute_force(harness, seed):
+    r = subprocess.run([harness, "arena", str(seed), "3000"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout
+
+
+def test_single_set_runs_one_cycle(harness):
+    ev, end = run_sim(harness, 64, 1 << 20, 4, 0, 0, [(0, 10, 1000, 2000)])
+    assert end == (1, 1)
+    check_cycle(ev, 64, 1 << 20, 4)
+    assert [f[2] for f in ev["step"]] == ["0", "1", "2", "3"]
+
+
+def test_sets_join_mid_cycle(harness):
+    # continuous batching: a set arriving while another is mid-cycle joins at
+    # the next step, not after the first finishes
+    ev, end = run_sim(harness, 64, 1 << 20, 8, 1, 0, [(0, 20, 100, 100), (3, 5, 100, 100), (4, 30, 100, 100)])
+    assert end == (3, 3)
+    check_cycle(ev, 64, 1 << 20, 8)
+    joins = {int(f[1]): int(f[2]) for f in ev["join"]}
+    fin = {int(f[1]): int(f[2]) for f in ev["finish"]}
+    assert joins[2] < fin[1] and joins[3] < fin[1]
+    # the later set starts at a band other than 0 and still covers all bands
+    assert int([f for f in ev["step"] if int(f[1]) == joins[2]][0][2]) != 0
+
+
+def test_slot_limit_blocks_admission_fifo(harness):
+    # 64 slots: the second set (40) waits for the first (40) to finish; the
+    # third (10) fits but waits behind it (FIFO, no overtaking)
+    ev, end = run_sim(harness, 64, 1 << 20, 4, 0, 1, [(0, 40, 10, 10), (0, 40, 10, 10), (0, 10, 10, 10)])
+    assert end == (3, 3)
+    check_cycle(ev, 64, 1 << 20, 4)
+    adm = {int(f[1]): int(f[0]) for f in ev["admit"]}
+    fin = {int(f[1]): int(f[0]) for f in ev["finish"]}
+    assert adm[2] > fin[1]
+    assert adm[3] >= adm[2]
+
+
+def test_memory_budget_limits_concurrency(harness):
+    # each set needs 300 + 300 bytes of a 1,300-byte arena: at most two at once
+    reqs = [(0, 4, 300, 300) for _ in range(6)]
+    ev, end = run_sim(harness, 256, 1300, 3, 0, 2, reqs)
+    assert end == (6, 6)
+    check_cycle(ev, 256, 1300, 3)
+    for f in ev["step"]:
+        assert len(f) - 3 <= 2
+    assert max(int(f[1]) for f in ev["arena"]) <= 1300
+
+
+def test_never_fitting_sets_are_rejected(harness):
+    ev, end = run_sim(harness, 32, 1000, 2, 0, 0,
+                      [(0, 33, 10, 10), (0, 4, 900, 200), (0, 4, 100, 100)])
+    assert [f[2] for f in ev["submit"]] == ["reject", "reject", "ok"]
+    assert end == (1, 1)
+
+
+def test_slots_reused_and_blocks_marked_dirty(harness):
+    ev, end = run_sim(harness, 64, 1 << 20, 2, 0, 0, [(0, 32, 10, 10), (1, 32, 10, 10), (4, 32, 10, 10)])
+    assert end == (3, 3)
+    check_cycle(ev, 64, 1 << 20, 2)
+    slots = {int(f[1]): [int(x) for x in f[2:]] for f in ev["admit"]}
+    assert slots[1] == list(range(32)) and slots[2] == list(range(32, 64))
+    assert slots[3] == list(range(32))  # reused after set 1 finished
+    # a block whose membership changes is re-summarised before the next step
+    for f in ev["blocks"]:
+        tick = int(f[0])
+        dirty = [int(x) for x in f[f.index("|") + 1:]]
+        joins_now = [int(j[1]) for j in ev["join"] if int(j[0]) == tick]
+        for rid in joins_now:
+            for b in {s // 32 for s in slots[rid]}:
+                assert b in dirty
+
+
+def test_random_load(harness):
+    rng = random.Random(7)
+    reqs = []
+    t = 0
+    for _ in range(200):
+        t += rng.randint(0, 3)
+        nq = rng.randint(1, 96)
+        reqs.append((t, nq, rng.randint(1, 5000), nq * 400))
+    ev, end = run_sim(harness, 256, 120_000, 5, rng.randint(0, 2), 1, reqs)
+    assert end == (200, 200)
+    check_cycle(ev, 256, 120_000, 5)
