@@ -150,6 +150,15 @@ _sig("cms_write_db", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, _i32p, 
 _sig("cms_write_table", C.c_int, [C.c_char_p, C.c_int, C.c_int, _i64p, _i32p, _f32p])
 _sig("cms_free", None, [_vp])
 _sig("cms_version", C.c_char_p, [])
+_sig("cms_newick_parse", C.c_int, [C.c_char_p, C.c_int64, C.POINTER(_vp)])
+_sig("cms_newick_load", C.c_int, [C.c_char_p, C.POINTER(_vp)])
+_sig("cms_newick_free", None, [_vp])
+_sig("cms_newick_leaf_n", C.c_int, [_vp])
+_sig("cms_newick_order_n", C.c_int, [_vp])
+_sig("cms_newick_leaf_name", C.c_char_p, [_vp, C.c_int])
+_sig("cms_newick_order", C.c_int, [_vp] + [C.POINTER(_vp)] * 5)
+_sig("cms_newick_info", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)])
+_sig("cms_newick_write_db", C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_char_p])
 _sig("cms_server_opts_default", None, [C.POINTER(ServerOpts)])
 _sig("cms_server_create", C.c_int, [_vp, C.POINTER(ServerOpts), C.POINTER(_vp)])
 _sig("cms_server_submit", C.c_int, [_vp, C.c_int, _i64p, _i32p, _f32p, _vp, C.POINTER(C.c_int64)])
@@ -169,7 +178,9 @@ EXPORTED = [
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
     "cms_table_feature_name", "cms_table_skipped", "cms_table_abundance", "cms_write_dist",
     "cms_synth_tree", "cms_synth_samples", "cms_synth_copy_numbers", "cms_counts_to_abundance_csr",
-    "cms_write_db", "cms_write_table", "cms_free", "cms_version", "cms_server_opts_default", "cms_server_create",
+    "cms_write_db", "cms_write_table", "cms_free", "cms_version", "cms_newick_parse", "cms_newick_load",
+    "cms_newick_free", "cms_newick_leaf_n", "cms_newick_order_n", "cms_newick_leaf_name", "cms_newick_order",
+    "cms_newick_info", "cms_newick_write_db", "cms_server_opts_default", "cms_server_create",
     "cms_server_submit", "cms_server_wait", "cms_server_drain", "cms_server_get_stats", "cms_server_destroy",
 ]
 
@@ -275,6 +286,65 @@ class Database:
 
         return Tree(self.leaf_n, arr(ptrs[0], np.int32), arr(ptrs[1], np.float32), arr(ptrs[2], np.int32),
                     arr(ptrs[3], np.float32), arr(ptrs[4], np.int32))
+
+
+class Newick:
+    """A Newick tree as a reference schedule (cms_newick_*): leaves in order of
+    appearance, binarised, Sethi-Ullman order with lowest-free registers."""
+
+    def __init__(self, path: str | None = None, text: str | None = None):
+        h = _vp()
+        if (path is None) == (text is None):
+            raise ValueError("give exactly one of path or text")
+        if path is not None:
+            _check(lib.cms_newick_load(path.encode(), C.byref(h)))
+        else:
+            b = text.encode()
+            _check(lib.cms_newick_parse(b, len(b), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.cms_newick_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def leaf_n(self) -> int:
+        return lib.cms_newick_leaf_n(self._h)
+
+    @property
+    def order_n(self) -> int:
+        return lib.cms_newick_order_n(self._h)
+
+    @property
+    def names(self) -> list:
+        return [lib.cms_newick_leaf_name(self._h, i).decode() for i in range(self.leaf_n)]
+
+    def info(self) -> dict:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib.cms_newick_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"nslots": a.value, "multifurcations": b.value, "unary_merged": c.value}
+
+    def tree(self) -> Tree:
+        ptrs = [_vp() for _ in range(5)]
+        _check(lib.cms_newick_order(self._h, *[C.byref(p) for p in ptrs]))
+        n = self.order_n
+
+        def arr(p, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            ct = C.c_int32 if dt == np.int32 else C.c_float
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,)).astype(dt, copy=True)
+
+        return Tree(self.leaf_n, arr(ptrs[0], np.int32), arr(ptrs[1], np.float32), arr(ptrs[2], np.int32),
+                    arr(ptrs[3], np.float32), arr(ptrs[4], np.int32))
+
+    def write_db(self, root: str, name: str, cp_path: str | None = None):
+        _check(lib.cms_newick_write_db(self._h, root.encode(), name.encode(),
+                                       cp_path.encode() if cp_path else None))
 
 
 class Table:

@@ -704,9 +704,17 @@ int cms_server_submit(cms_server *s, int nq, const int64_t *rowptr, const int32_
         destroy_events();
         return sfail(CMS_E_STATE, "cms_server_submit: the server is shutting down");
     }
+    // densest queries first into the set's slots: 32-slot blocks then hold
+    // queries of similar sparsity, whose root-path union (the rows a tile
+    // visits) stays small for the sparse blocks
+    std::vector<int32_t> order((size_t)nq);
+    for (int q = 0; q < nq; q++) order[(size_t)q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return rowptr[a + 1] - rowptr[a] > rowptr[b + 1] - rowptr[b];
+    });
     const int64_t rid = s->next_id++;
     std::string err;
-    if (!s->sched->submit(rid, nq, SL.total, out_bytes, err)) {
+    if (!s->sched->submit(rid, nq, SL.total, out_bytes, err, order)) {
         destroy_events();
         return sfail(nq > s->opts.slots ? CMS_E_INVAL : CMS_E_NOMEM, "cms_server_submit: " + err);
     }

@@ -57,11 +57,16 @@ __global__ __launch_bounds__(64 * WAVES) void sm_cross_kernel(CrossData x, const
         if (w0 >= w1) continue;
         const int64_t rb = rb0 + t / nqa;
         const int64_t qb = qblocks[t % nqa];
-        const uint32_t *col = (lane < 32) ? x.QY + qb * kSmBlock + lane : x.Y + rb * kSmBlock + (lane - 32);
-        const int64_t stride = (lane < 32) ? x.qstride : x.npad;
+        CrossRows rows;
+        rows.QY = x.QY + qb * kSmBlock;
+        rows.qs = x.qstride;
+        rows.Y = x.Y + rb * kSmBlock;
+        rows.ys = x.npad;
+        rows.col4 = (uint32_t)(lane & 31) * 4u;
+        rows.lo = lane < 32;
         uint32_t acc[4][4];
         visited += sm_tile_accumulate<CHUNK>(x.qbm + qb * x.nwords, nullptr, x.bm + rb * x.nwords, nullptr, w0, w1,
-                                             col, stride, true, s_list[wave], s_stage[wave], lane, acc);
+                                             rows, s_list[wave], s_stage[wave], lane, acc);
         if constexpr (SPLIT) {
             uint32_t *pa = part_acc + t * 1024 + lane;
 #pragma unroll

@@ -267,16 +267,20 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
 // ------------------------------------------------------------ pair tiles
 
 // Square tile of the all-pairs matrix: lanes 0..31 stage the row block's
-// columns of Y, lanes 32..63 the column block's.
-template <int CHUNK>
+// columns of Y, lanes 32..63 the column block's. CHECK when a block may
+// extend past the padded sample count (rectangles at arbitrary offsets).
+template <int CHUNK, bool CHECK>
 __device__ __forceinline__ uint32_t sm_square_tile(const SmData &d, const uint32_t *A0, const uint32_t *A1,
                                                    const uint32_t *B0, const uint32_t *B1, int64_t abase,
                                                    int64_t bbase, uint16_t *L, uint32_t *S, int lane,
                                                    uint32_t (&acc)[4][4]) {
     const int64_t my_s = (lane < 32) ? abase + lane : bbase + (lane - 32);
-    const bool s_ok = my_s < d.npad;
-    return sm_tile_accumulate<CHUNK>(A0, A1, B0, B1, 0, d.nwords, d.Y + (s_ok ? my_s : 0), d.npad, s_ok, L, S,
-                                     lane, acc);
+    SquareRows<CHECK> rows;
+    rows.Y = d.Y;
+    rows.stride = d.npad;
+    rows.ok = my_s < d.npad;
+    rows.col4 = (uint32_t)((!CHECK || rows.ok) ? my_s : 0) * 4u;
+    return sm_tile_accumulate<CHUNK>(A0, A1, B0, B1, 0, d.nwords, rows, L, S, lane, acc);
 }
 
 // Similarity of the pair at stored positions (pa, pb) from its shared sum.
@@ -301,7 +305,7 @@ __global__ __launch_bounds__(64 * kSmWaves) void sm_rect_kernel(SmData d, int64_
     const uint32_t *A1 = ((abase % kSmBlock) && ba + 1 < nblk) ? d.bm + (ba + 1) * d.nwords : nullptr;
     const uint32_t *B1 = ((bbase % kSmBlock) && bb + 1 < nblk) ? d.bm + (bb + 1) * d.nwords : nullptr;
     uint32_t acc[4][4];
-    sm_square_tile<kSmChunk>(d, A0, A1, B0, B1, abase, bbase, s_list[wave], s_stage[wave], lane, acc);
+    sm_square_tile<kSmChunk, true>(d, A0, A1, B0, B1, abase, bbase, s_list[wave], s_stage[wave], lane, acc);
     const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
     for (int ii = 0; ii < 4; ii++)
@@ -341,7 +345,7 @@ __global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int
         sm_decode_upper(t_begin + (int64_t)t, nb, ra, cb);
         const int64_t abase = ra * kSmBlock, bbase = cb * kSmBlock;
         uint32_t acc[4][4];
-        visited += sm_square_tile<CHUNK>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords, nullptr, abase,
+        visited += sm_square_tile<CHUNK, false>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords, nullptr, abase,
                                              bbase, s_list[wave], s_stage[wave], lane, acc);
 #pragma unroll
         for (int ii = 0; ii < 4; ii++)

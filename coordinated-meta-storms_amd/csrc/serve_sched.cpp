@@ -79,10 +79,23 @@ Scheduler::Scheduler(int slots, int64_t arena_bytes, int bands)
     : nslots_(slots), bands_(std::max(bands, 1)), arena_(arena_bytes), owner_((size_t)std::max(slots, 0), -1),
       free_count_(std::max(slots, 0)), dirty_((size_t)(std::max(slots, 0) + 31) / 32, 0) {}
 
-bool Scheduler::submit(int64_t id, int nq, int64_t scratch_bytes, int64_t out_bytes, std::string &err) {
+bool Scheduler::submit(int64_t id, int nq, int64_t scratch_bytes, int64_t out_bytes, std::string &err,
+                       const std::vector<int32_t> &order) {
     if (nq <= 0 || nq > nslots_) {
         err = "query set of " + std::to_string(nq) + " samples: the server holds 1.." + std::to_string(nslots_);
         return false;
+    }
+    if (!order.empty()) {
+        std::vector<char> seen((size_t)nq, 0);
+        bool ok = (int)order.size() == nq;
+        for (size_t i = 0; ok && i < order.size(); i++) {
+            ok = order[i] >= 0 && order[i] < nq && !seen[(size_t)order[i]];
+            if (ok) seen[(size_t)order[i]] = 1;
+        }
+        if (!ok) {
+            err = "query placement order is not a permutation of the set";
+            return false;
+        }
     }
     Arena probe(arena_.capacity());
     if (probe.allocate(scratch_bytes) < 0 || probe.allocate(out_bytes) < 0) {
@@ -95,6 +108,7 @@ bool Scheduler::submit(int64_t id, int nq, int64_t scratch_bytes, int64_t out_by
     r.nq = nq;
     r.scratch_bytes = scratch_bytes;
     r.out_bytes = out_bytes;
+    r.order = order;
     reqs_[id] = r;
     queue_.push_back(id);
     return true;
@@ -114,11 +128,14 @@ std::vector<int64_t> Scheduler::admit() {
         }
         r.scratch_off = s;
         r.out_off = o;
-        r.slots.clear();
-        for (int k = 0; k < nslots_ && (int)r.slots.size() < r.nq; k++)
+        r.slots.assign((size_t)r.nq, -1);
+        int placed = 0;
+        for (int k = 0; k < nslots_ && placed < r.nq; k++)
             if (owner_[k] < 0) {
                 owner_[k] = r.id;
-                r.slots.push_back(k);
+                const int q = r.order.empty() ? placed : r.order[(size_t)placed];
+                r.slots[(size_t)q] = k;
+                placed++;
             }
         free_count_ -= r.nq;
         r.state = ReqState::Embedding;

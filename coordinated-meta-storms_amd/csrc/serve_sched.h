@@ -50,6 +50,7 @@ struct SchedRequest {
     int64_t scratch_bytes = 0, out_bytes = 0;
     int64_t scratch_off = -1, out_off = -1;
     std::vector<int32_t> slots;  // query q -> slot
+    std::vector<int32_t> order;  // placement order of the queries (empty = 0..nq-1)
     ReqState state = ReqState::Queued;
     int64_t join_step = -1;      // step at which it joined
     int steps_done = 0;
@@ -73,7 +74,11 @@ public:
 
     // Queue a query set (FIFO). Returns false with `err` when it can never be
     // admitted: more queries than slots or more memory than the arena.
-    bool submit(int64_t id, int nq, int64_t scratch_bytes, int64_t out_bytes, std::string &err);
+    // `order` (a permutation of 0..nq-1, or empty) is the order in which the
+    // queries take the set's slots, lowest slot first: queries adjacent in it
+    // share 32-slot blocks.
+    bool submit(int64_t id, int nq, int64_t scratch_bytes, int64_t out_bytes, std::string &err,
+                const std::vector<int32_t> &order = {});
     // Admit queued sets in FIFO order while the head fits (free slots and
     // both arena blocks); returns the ids admitted now (state Embedding).
     std::vector<int64_t> admit();

@@ -8,26 +8,76 @@
 
 namespace cms {
 
+// Row loaders: value of this lane's sample at row rbase + r of Y, with
+// window(rbase) called once per 64-word bitmap window and r < 2048. Row
+// addresses come from the wave-uniform row index in scalar registers (32-bit
+// r * stride: strides below 2^21 samples, checked by the launchers), and the
+// lane's column is a 32-bit byte offset, so a staged row costs one vector
+// load and no vector address arithmetic.
+//
+// One matrix (all-pairs tiles): lane column `col` (row block sample for lanes
+// 0..31, column block sample for lanes 32..63) of Y with row stride `stride`.
+// CHECK: a lane whose column lies outside the matrix (ok false) loads column
+// 0 and stages zero.
+template <bool CHECK>
+struct SquareRows {
+    const uint32_t *Y;
+    int64_t stride;
+    uint32_t col4;  // column * 4 (bytes)
+    bool ok;
+    const uint32_t *win = nullptr;
+    __device__ __forceinline__ void window(int64_t rbase) { win = Y + rbase * stride; }
+    __device__ __forceinline__ uint32_t load(uint32_t r) const {
+        const char *row = reinterpret_cast<const char *>(win + r * (uint32_t)stride);
+        const uint32_t v = *reinterpret_cast<const uint32_t *>(row + col4);
+        return (!CHECK || ok) ? v : 0u;
+    }
+};
+
+// Two matrices (serving): lanes 0..31 read QY (query slots of one block,
+// row stride qs), lanes 32..63 read Y (references of one block, stride ys).
+struct CrossRows {
+    const uint32_t *QY;
+    int64_t qs;
+    const uint32_t *Y;
+    int64_t ys;
+    uint32_t col4;  // (lane & 31) * 4
+    bool lo;        // lane < 32
+    const uint32_t *qwin = nullptr, *ywin = nullptr;
+    __device__ __forceinline__ void window(int64_t rbase) {
+        qwin = QY + rbase * qs;
+        ywin = Y + rbase * ys;
+    }
+    __device__ __forceinline__ uint32_t load(uint32_t r) const {
+        uint32_t v;
+        if (lo) {
+            const char *row = reinterpret_cast<const char *>(qwin + r * (uint32_t)qs);
+            v = *reinterpret_cast<const uint32_t *>(row + col4);
+        } else {
+            const char *row = reinterpret_cast<const char *>(ywin + r * (uint32_t)ys);
+            v = *reinterpret_cast<const uint32_t *>(row + col4);
+        }
+        return v;
+    }
+};
+
 // One wave = 32 x 32 pairs, lane (ty, tx) = 4 rows x 4 columns.
 //
 // Visit the entries set in both blocks' bitmaps (OR of two blocks when a
 // side is not block-aligned): per 64 bitmap words, compact the set bits into
-// an LDS list, stage their 2 x 32 Y values in LDS CHUNK rows at a time, and
-// accumulate min/add into the lane's 4 x 4 registers. Returns the number of
-// entries visited.
-//
-// Ycol/ystride: the Y column this lane stages (lanes 0..31 the 32 row
-// samples, lanes 32..63 the 32 column samples) and its row stride; s_ok
-// false for a lane whose column lies outside the matrix (it stages zeros).
+// an LDS list, stage their 2 x 32 Y values in LDS CHUNK rows at a time
+// (through `rows`, see the loaders above), and accumulate min/add into the
+// lane's 4 x 4 registers. Returns the number of entries visited.
 //
 // Only bitmap words [w_begin, w_end) are visited (w_begin a multiple of 64):
 // a tile's rows can be split over several waves whose integer partial sums
 // add up exactly.
-template <int CHUNK>
+template <int CHUNK, class Rows>
 __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const uint32_t *A1, const uint32_t *B0,
                                                        const uint32_t *B1, int64_t w_begin, int64_t w_end,
-                                                       const uint32_t *Ycol, int64_t ystride, bool s_ok,
-                                                       uint16_t *L, uint32_t *S, int lane, uint32_t (&acc)[4][4]) {
+                                                       const Rows &rows, uint16_t *L, uint32_t *S, int lane,
+                                                       uint32_t (&acc)[4][4]) {
+    static_assert(CHUNK % 2 == 0, "rows are staged in pairs");
     const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -50,7 +100,7 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
             int v = __shfl_up(incl, off, 64);
             if (lane >= off) incl += v;
         }
-        const int tot = __shfl(incl, 63, 64);
+        const int tot = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
         if (tot == 0) continue;
         visited += (uint32_t)tot;
         int pos = incl - cnt;
@@ -63,21 +113,32 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t *Ywb = Ycol + wb * 32 * ystride;
+        Rows win = rows;
+        win.window(wb * 32);
         for (int j = 0; j < tot; j += CHUNK) {
+            // row ids are wave-uniform (every lane reads the same LDS word);
+            // slots past the list end load the chunk's first row (a valid
+            // address) and are never accumulated
+            const uint32_t first =
+                (uint32_t)__builtin_amdgcn_readfirstlane((int)*reinterpret_cast<const uint32_t *>(L + j)) & 0xffffu;
             uint32_t v[CHUNK];
 #pragma unroll
-            for (int k = 0; k < CHUNK; k++) {
-                v[k] = 0u;
-                if (j + k < tot && s_ok) v[k] = Ywb[(int64_t)L[j + k] * ystride];
+            for (int k = 0; k < CHUNK; k += 2) {
+                const uint32_t pr =
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)*reinterpret_cast<const uint32_t *>(L + j + k));
+                const uint32_t r0 = (j + k < tot) ? (pr & 0xffffu) : first;
+                const uint32_t r1 = (j + k + 1 < tot) ? (pr >> 16) : first;
+                v[k] = win.load(r0);
+                v[k + 1] = win.load(r1);
             }
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int k = 0; k < CHUNK; k++) S[k * 64 + lane] = v[k];
+            const int kn = min(CHUNK, tot - j);
+            if (kn & 1) S[kn * 64 + lane] = 0u;  // rows are accumulated in pairs
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int kn = min(CHUNK, tot - j);
             for (int k = 0; k < kn; k += 2) {
                 const uint4 a0 = *reinterpret_cast<const uint4 *>(S + k * 64 + ty * 4);
                 const uint4 b0 = *reinterpret_cast<const uint4 *>(S + k * 64 + 32 + tx * 4);

@@ -311,6 +311,29 @@ int cms_table_abundance(const cms_db *db, const cms_table *t, int cp_correct,
 int cms_write_dist(const char *path, int n, const char *const *names,
                    const float *sim_packed, int is_sim);
 
+/* ---------------------------------------- database builder (no GPU)
+ *
+ * The reference only reads tree/id.txt and tree/order.txt
+ * (_Comp_Tree::Load_id / Load_order, comp.h:139-192); these build them from
+ * a Newick tree. Leaves are numbered in order of appearance; a node with one
+ * child is merged into its child's edge; a node with k > 2 children becomes
+ * a chain of binary nodes joined by zero-length edges; nodes are evaluated in
+ * Sethi-Ullman order with lowest-free registers written as -(slot + 1). */
+typedef struct cms_newick cms_newick;
+int cms_newick_parse(const char *text, int64_t len, cms_newick **out);
+int cms_newick_load(const char *path, cms_newick **out);
+void cms_newick_free(cms_newick *t);
+int cms_newick_leaf_n(const cms_newick *t);
+int cms_newick_order_n(const cms_newick *t);
+const char *cms_newick_leaf_name(const cms_newick *t, int i);
+/* Schedule arrays (valid until cms_newick_free), as cms_set_tree takes them. */
+int cms_newick_order(const cms_newick *t, const int32_t **order_1, const float **dist_1,
+                     const int32_t **order_2, const float **dist_2, const int32_t **order_d);
+int cms_newick_info(const cms_newick *t, int *nslots, int *multifurcations, int *unary_merged);
+/* root/databases/db.config (entry 'G' -> name), root/databases/name/tree/
+ * {id,order}.txt, and name/copy_number.txt copied from cp_path if not NULL. */
+int cms_newick_write_db(const cms_newick *t, const char *root, const char *name, const char *cp_path);
+
 /* ------------------------------------------- synthetic inputs (no GPU) */
 
 /* Seeded synthetic tree: random binary tree (Yule process), branch lengths

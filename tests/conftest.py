@@ -68,7 +68,7 @@ class Golden:
         return i, j, s
 
 
-GOLDEN_CASES = ["toy", "quirks", "config0"]
+GOLDEN_CASES = ["toy", "quirks", "config0", "core"]
 
 
 @pytest.fixture(scope="session")

@@ -88,7 +88,28 @@ def config0_case(d):
     cms.write_table(os.path.join(d, "table.txt"), counts, tree.leaf_n)
 
 
-CASES = {"toy": toy_case, "quirks": quirks_case, "config0": config0_case}
+def core_case(d):
+    """The reference's only real tree, databases/oral_core/tree/core.tre
+    (1,045 leaves; copied to tests/golden/core.tre), turned into a database
+    by this repository's Newick builder; synthetic copy numbers and 40
+    samples over its leaf ids (seeds 4342/4343)."""
+    nw = cms.Newick(os.path.join(HERE, "core.tre"))
+    names = nw.names
+    cp = cms.synth_copy_numbers(4342, len(names), 15)
+    cp_path = os.path.join(d, "cp_in.txt")
+    with open(cp_path, "w") as f:
+        f.write("ID\tcopy_number\n" + "".join(f"{n}\t{int(c)}\n" for n, c in zip(names, cp)))
+    nw.write_db(d, "oral_core", cp_path)
+    os.remove(cp_path)
+    counts = cms.synth_counts(4343, 40, len(names), 20, 300, 1.1, 50, 0.5)
+    dense = counts.dense(len(names))
+    with open(os.path.join(d, "table.txt"), "w") as f:
+        f.write("SampleID\t" + "\t".join(names) + "\n")
+        for i in range(dense.shape[0]):
+            f.write(f"C{i}\t" + "\t".join("%g" % v for v in dense[i]) + "\n")
+
+
+CASES = {"toy": toy_case, "quirks": quirks_case, "config0": config0_case, "core": core_case}
 
 
 def run_ref(d, args, out=None):

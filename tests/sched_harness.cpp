@@ -9,7 +9,8 @@
 //                                     trace of events for the test to check
 //
 // Scenario: first line "slots arena bands embed_ticks deliver_ticks", then
-// one line per query set "arrive_tick nq scratch_bytes out_bytes".
+// one line per query set "arrive_tick nq scratch_bytes out_bytes reverse"
+// (reverse 1: the set's queries take its slots last query first).
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -78,10 +79,11 @@ static int sim() {
     struct Arrival {
         int tick, nq;
         long long scratch, out;
+        int reverse;
     };
     std::vector<Arrival> arr;
     Arrival x;
-    while (scanf("%d %d %lld %lld", &x.tick, &x.nq, &x.scratch, &x.out) == 4) arr.push_back(x);
+    while (scanf("%d %d %lld %lld %d", &x.tick, &x.nq, &x.scratch, &x.out, &x.reverse) == 5) arr.push_back(x);
     Scheduler s(slots, arena, bands);
     std::map<int64_t, int> embed_due, deliver_due, scratch_due;
     size_t next_arrival = 0;
@@ -91,7 +93,10 @@ static int sim() {
             const Arrival &a = arr[next_arrival];
             const int64_t id = (int64_t)next_arrival + 1;
             std::string err;
-            if (s.submit(id, a.nq, a.scratch, a.out, err)) {
+            std::vector<int32_t> order;
+            if (a.reverse)
+                for (int q = a.nq - 1; q >= 0; q--) order.push_back(q);
+            if (s.submit(id, a.nq, a.scratch, a.out, err, order)) {
                 printf("submit %d %lld ok\n", t, (long long)id);
                 accepted++;
             } else {

@@ -22,7 +22,7 @@ def harness(tmp_path_factory):
 
 def run_sim(exe, slots, arena, bands, embed_ticks, deliver_ticks, reqs):
     text = f"{slots} {arena} {bands} {embed_ticks} {deliver_ticks}\n" + "".join(
-        f"{t} {nq} {sc} {out}\n" for t, nq, sc, out in reqs)
+        f"{r[0]} {r[1]} {r[2]} {r[3]} {r[4] if len(r) > 4 else 0}\n" for r in reqs)
     r = subprocess.run([exe, "sim"], input=text, capture_output=True, text=True, check=True)
     ev = {"submit": [], "admit": [], "join": [], "step": [], "finish": [], "deliver": [], "arena": [], "blocks": []}
     end = None
@@ -161,3 +161,14 @@ def test_random_load(harness):
     ev, end = run_sim(harness, 256, 120_000, 5, rng.randint(0, 2), 1, reqs)
     assert end == (200, 200)
     check_cycle(ev, 256, 120_000, 5)
+
+
+def test_placement_order(harness):
+    # a set's placement order decides which query takes which slot: reversed,
+    # the last query gets the lowest free slot
+    ev, end = run_sim(harness, 64, 1 << 20, 2, 0, 0, [(0, 8, 10, 10, 0), (0, 8, 10, 10, 1)])
+    assert end == (2, 2)
+    check_cycle(ev, 64, 1 << 20, 2)
+    slots = {int(f[1]): [int(x) for x in f[2:]] for f in ev["admit"]}
+    assert slots[1] == list(range(8))
+    assert slots[2] == list(range(15, 7, -1))
