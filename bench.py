@@ -107,6 +107,25 @@ def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, s
     return res, acc
 
 
+def roofline(st, kernel, traffic=None):
+    """Roofline entry of a shared-minimum pair kernel from its work counters:
+    VALU (algorithmic lane-ops: 1.5 per visited pair-entry, i.e. per pair of
+    a visited 32 x 32 tile-row) and HBM (algorithmic bytes: 2 x 32 x 4 B of Y
+    per visited tile-row), each over the kernel time from HIP events; `bound`
+    is the one closer to its peak."""
+    t = st.kernel_ms * 1e-3
+    valu = st.valu_ops / t / 1e12 if t > 0 else 0.0
+    hbm = st.bytes_moved / t / 1e9 if t > 0 else 0.0
+    fv, fh = valu / VALU_PEAK_TOPS, hbm / HBM_PEAK_GBS
+    main = ({"bound": "valu", "achieved": valu, "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": fv} if fv >= fh else
+            {"bound": "hbm", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fh})
+    main.update({"traffic": traffic, "kernel": kernel, "kernel_ms": st.kernel_ms,
+                 "valu": {"achieved_tops": valu, "frac": fv, "ops": st.valu_ops},
+                 "hbm": {"achieved_gbs": hbm, "frac": fh, "algorithmic_bytes": st.bytes_moved},
+                 "pair_entries": st.pair_node_updates, "ops_per_pair_entry": 1.5, "tune_ms": list(st.tune_ms)})
+    return main
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the pair kernel, as measured by the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE runs (profiles/traffic_workload1.txt:
@@ -155,7 +174,6 @@ def run_batched(cms, a):
     ctx.all_pairs_stream(fn, a.band_pairs)
     stream_s = time.perf_counter() - t2
     st = ctx.stats()
-    valu_tops = st.valu_ops / (st.kernel_ms * 1e-3) / 1e12 if st.kernel_ms > 0 else 0.0
     res = {"metric": METRIC, "value": npairs / (stream_s + prep_s), "unit": "sample-pairs/s", "n_gpus": 1,
            "steps": 1, "warmup": 0, "ms_per_step": (stream_s + prep_s) * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "u32 fixed-point (FP64 embedding)",
@@ -165,10 +183,7 @@ def run_batched(cms, a):
                       "variant": st.variant, "bands": st.bands, "band_pairs": a.band_pairs,
                       "hipgraph": bool(a.graph)},
            "prep_s": prep_s, "stream_s": stream_s, "pairs_streamed": got["pairs"],
-           "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                        "frac": valu_tops / VALU_PEAK_TOPS, "traffic": None, "kernel": "sm_persistent_kernel",
-                        "kernel_ms": st.kernel_ms, "pair_entries": st.pair_node_updates,
-                        "tune_ms": list(st.tune_ms)}}
+           "roofline": roofline(st, "sm_persistent_kernel")}
     ctx.close()
     print(json.dumps(res), flush=True)
 
@@ -331,7 +346,9 @@ def run_serving(cms, a):
     lat = np.array(lat)
     served = len(lat)
     mean_q = float(sizes[:served].mean()) if served else 0.0
-    valu_tops = entries * 1.5 / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    valu_ops = entries * 1.5
+    valu_tops = valu_ops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    hbm_gbs = entries / 1024 * 256 / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     res = {"metric": "serving: p50/p99 latency and query x reference pairs/s on 1 MI355X",
            "value": pairs_done / wall, "unit": "pairs/s", "n_gpus": 1, "higher_is_better": True,
            "dtype": "u32 fixed-point (FP64 embedding)",
@@ -354,7 +371,11 @@ def run_serving(cms, a):
            "max_abs_err_vs_fp64_spot": max(errs) if errs else None,
            "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
                         "frac": valu_tops / VALU_PEAK_TOPS, "traffic": None, "kernel": "sm_cross_kernel",
-                        "pair_entries": entries}}
+                        "pair_entries": entries, "valu_ops": valu_ops,
+                        "hbm": {"achieved_gbs": hbm_gbs, "frac": hbm_gbs / HBM_PEAK_GBS}}}
+    if hbm_gbs / HBM_PEAK_GBS > valu_tops / VALU_PEAK_TOPS:
+        res["roofline"].update({"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": hbm_gbs / HBM_PEAK_GBS})
     print(json.dumps(res), flush=True)
 
 
@@ -442,7 +463,6 @@ def main():
                              "ops_per_pair_node": 14}}
         log(f"walk: {wsec * 1e3:.1f} ms/step, kernel {wst.kernel_ms:.1f} ms, {w_ops:.2f} TOP/s")
 
-    valu_tops = st.valu_ops / (st.kernel_ms * 1e-3) / 1e12
     res = {
         "metric": METRIC,
         "value": npairs / sec,
@@ -460,11 +480,7 @@ def main():
                    "samples": n, "leaves": tree.leaf_n, "pairs": npairs, "kernel": "sharedmin",
                    "binned": bool(a.bin), "variant": st.variant},
         "pair_node_updates_per_s": npairs * tree.order_n / sec,
-        "roofline": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                     "frac": valu_tops / VALU_PEAK_TOPS, "traffic": traffic_from_profiles(),
-                     "kernel": "sm_persistent_kernel", "kernel_ms": st.kernel_ms,
-                     "ops_per_pair_entry": 1.5, "pair_entries": st.pair_node_updates,
-                     "tune_ms": list(st.tune_ms)},
+        "roofline": roofline(st, "sm_persistent_kernel", traffic_from_profiles()),
         "prep_ms": st.prep_ms,
         "plain_fp32_walk": walk,
     }

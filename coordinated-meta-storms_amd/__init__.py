@@ -140,6 +140,7 @@ _sig("cms_table_sample_name", C.c_char_p, [_vp, C.c_int])
 _sig("cms_table_feature_name", C.c_char_p, [_vp, C.c_int])
 _sig("cms_table_skipped", C.c_int, [_vp])
 _sig("cms_table_abundance", C.c_int, [_vp, _vp, C.c_int, _f32p, _vp])
+_sig("cms_table_abundance_csr", C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)])
 _sig("cms_write_dist", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), _f32p, C.c_int])
 _sig("cms_synth_tree", C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_double, _i32p, _f32p, _i32p, _f32p, _i32p,
                                  C.POINTER(C.c_int)])
@@ -176,7 +177,8 @@ EXPORTED = [
     "cms_tile", "cms_get_stats", "cms_prepare", "cms_validate_tree", "cms_db_open", "cms_db_close", "cms_db_leaf_n", "cms_db_order_n",
     "cms_db_has_tree", "cms_db_has_cp", "cms_db_order", "cms_db_leaf_name", "cms_table_load",
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
-    "cms_table_feature_name", "cms_table_skipped", "cms_table_abundance", "cms_write_dist",
+    "cms_table_feature_name", "cms_table_skipped", "cms_table_abundance", "cms_table_abundance_csr",
+    "cms_write_dist",
     "cms_synth_tree", "cms_synth_samples", "cms_synth_copy_numbers", "cms_counts_to_abundance_csr",
     "cms_write_db", "cms_write_table", "cms_free", "cms_version", "cms_newick_parse", "cms_newick_load",
     "cms_newick_free", "cms_newick_leaf_n", "cms_newick_order_n", "cms_newick_leaf_name", "cms_newick_order",
@@ -395,6 +397,21 @@ class Table:
         _check(lib.cms_table_abundance(db._h, self._h, int(cp_correct), out.reshape(-1),
                                        mapped.ctypes.data_as(_vp)))
         return out, mapped
+
+    def abundance_csr(self, db: Database, cp_correct: bool = True) -> "Samples":
+        """The same abundances as CSR samples (cms_table_abundance_csr)."""
+        rp, col, val = _vp(), _vp(), _vp()
+        _check(lib.cms_table_abundance_csr(db._h, self._h, int(cp_correct), C.byref(rp), C.byref(col), C.byref(val)))
+        try:
+            n = self.n
+            rowptr = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int64)), shape=(n + 1,)).copy()
+            nnz = int(rowptr[-1])
+            cols = np.ctypeslib.as_array(C.cast(col, C.POINTER(C.c_int32)), shape=(max(nnz, 1),))[:nnz].copy()
+            vals = np.ctypeslib.as_array(C.cast(val, C.POINTER(C.c_float)), shape=(max(nnz, 1),))[:nnz].copy()
+        finally:
+            for p in (rp, col, val):
+                lib.cms_free(p)
+        return Samples(rowptr, cols, vals)
 
 
 def write_dist(path: str, names, sim_packed: np.ndarray, is_sim: bool = False):

@@ -2,8 +2,15 @@
 // (Parse_Para / Output_Matrix / main of cms_cuda.cu:1222-1342, 2237-2261),
 // computing -M 0 (weighted Meta-Storms) on the GPU through libcms.
 //
-//   cuda-comp -T table -D G -o out.dist [-r T|F] [-d T|F] [-M 0] [-t n]
+//   cuda-comp -T table -D G -o out.dist [-r T|F] [-d T|F] [-M 0|1] [-t n]
 //             [--kernel walk|sharedmin] [--storage f32|bf16|f16] [--device k]
+//             [--format dist|binary]
+//
+// --format binary writes the packed upper triangle (pair (i, j), i < j, at
+// i*n + j - (i+1)(i+2)/2; the values .dist would print: 1 - sim, or sim with
+// -d F) as little-endian float32 to the -o file and the sample names, one per
+// line, to <file>.names. With the shared-minimum kernel it is streamed to the
+// file in bands as the GPU computes them (no n x n buffer in host memory).
 //
 // The database is read from $ParallelMETA/databases (db.config, tree/id.txt,
 // tree/order.txt, copy_number.txt). Errors exit with status 1 (the reference
@@ -36,6 +43,7 @@ namespace {
     printf("\t  --kernel walk|sharedmin  GPU kernel (walk is bit-identical to comp -M 0)\n");
     printf("\t  --storage f32|bf16|f16   walk abundance storage, default f32\n");
     printf("\t  --device k               GPU index, default 0\n");
+    printf("\t  --format dist|binary     text .dist (default) or packed float32 triangle + .names\n");
     printf("\t  -h Help\n");
     exit(0);
 }
@@ -45,6 +53,28 @@ namespace {
     exit(1);
 }
 
+struct BinaryOut {
+    FILE *f = nullptr;
+    bool is_sim = false;
+    int64_t written = 0;
+    bool ok = true;
+    std::vector<float> buf;
+};
+
+// Band callback of cms_all_pairs_stream: values as .dist prints them.
+void write_band(void *user, int64_t first, int64_t count, const float *sims) {
+    BinaryOut *o = static_cast<BinaryOut *>(user);
+    if (!o->ok) return;
+    if (first != o->written) {  // bands arrive in packed order when samples are not binned
+        o->ok = false;
+        return;
+    }
+    o->buf.resize((size_t)count);
+    for (int64_t k = 0; k < count; k++) o->buf[(size_t)k] = o->is_sim ? sims[k] : (float)(1.0 - (double)sims[k]);
+    if (fwrite(o->buf.data(), sizeof(float), (size_t)count, o->f) != (size_t)count) o->ok = false;
+    o->written += count;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -52,6 +82,7 @@ int main(int argc, char **argv) {
     std::string table_path, out_path;
     bool cp_correct = true, is_sim = false;
     int metric = 0, device = 0;
+    bool binary = false;
     cms_opts opts;
     cms_opts_default(&opts);
     if (argc == 1) printhelp();
@@ -79,6 +110,11 @@ int main(int argc, char **argv) {
             else die("unknown storage " + v);
         } else if (a == "--device") {
             device = atoi(need(1));
+        } else if (a == "--format") {
+            std::string v = need(1);
+            if (v == "dist") binary = false;
+            else if (v == "binary") binary = true;
+            else die("unknown format " + v);
         } else {
             switch (a[1]) {
             case 'D': ref_db = need(1)[0]; break;
@@ -111,35 +147,65 @@ int main(int argc, char **argv) {
     cms_table *table = nullptr;
     if (cms_table_load(table_path.c_str(), &table) != CMS_OK) die(cms_last_error(nullptr));
     const int n = cms_table_n(table), L = cms_db_leaf_n(db), on = cms_db_order_n(db);
-    std::vector<float> abd((size_t)n * L);
-    if (cms_table_abundance(db, table, cp_correct ? 1 : 0, abd.data(), nullptr) != CMS_OK) die(cms_last_error(nullptr));
+    int64_t *rowptr = nullptr;
+    int32_t *leaf = nullptr;
+    float *val = nullptr;
+    if (cms_table_abundance_csr(db, table, cp_correct ? 1 : 0, &rowptr, &leaf, &val) != CMS_OK)
+        die(cms_last_error(nullptr));
     printf("%d files loaded\n", n);
     auto t1 = std::chrono::steady_clock::now();
+    if (out_path.empty()) die("Cannot open output file : ");
 
     const int32_t *o1, *o2, *od;
     const float *d1, *d2;
     cms_db_order(db, &o1, &d1, &o2, &d2, &od);
+    const bool stream = binary && (opts.kernel == CMS_KERNEL_SHAREDMIN || metric == 1);
+    if (stream) opts.bin = 0;  // bands then arrive in the packed order of the table's samples
     cms_ctx *ctx = nullptr;
     if (cms_create(device, &opts, &ctx) != CMS_OK) die(cms_last_error(nullptr));
     if (cms_set_tree(ctx, L, on, o1, d1, o2, d2, od) != CMS_OK) die(cms_last_error(ctx));
-    if (cms_set_samples(ctx, n, abd.data()) != CMS_OK) die(cms_last_error(ctx));
-    abd.clear();
-    abd.shrink_to_fit();
-    std::vector<float> sim((size_t)n * (n > 0 ? n - 1 : 0) / 2);
-    if (cms_all_pairs(ctx, sim.data()) != CMS_OK) die(cms_last_error(ctx));
-    cms_stats st;
-    cms_get_stats(ctx, &st);
-    auto t2 = std::chrono::steady_clock::now();
-
-    if (out_path.empty()) die("Cannot open output file : ");
+    if (cms_set_samples_csr(ctx, n, rowptr, leaf, val) != CMS_OK) die(cms_last_error(ctx));
+    cms_free(rowptr);
+    cms_free(leaf);
+    cms_free(val);
     std::vector<const char *> names(n);
     for (int k = 0; k < n; k++) names[k] = cms_table_sample_name(table, k);
-    if (cms_write_dist(out_path.c_str(), n, names.data(), sim.data(), is_sim ? 1 : 0) != CMS_OK)
-        die(cms_last_error(nullptr));
-    auto t3 = std::chrono::steady_clock::now();
     auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
-    printf("*** load %.3f s, gpu %.3f s (pair kernel %.3f ms), write %.3f s\n", sec(t0, t1), sec(t1, t2), st.kernel_ms,
-           sec(t2, t3));
+    if (binary) {
+        const std::string names_path = out_path + ".names";
+        FILE *nf = fopen(names_path.c_str(), "w");
+        if (!nf) die("Cannot open output file : " + names_path);
+        for (int k = 0; k < n; k++) fprintf(nf, "%s\n", names[k]);
+        fclose(nf);
+        BinaryOut bo;
+        bo.f = fopen(out_path.c_str(), "wb");
+        bo.is_sim = is_sim;
+        if (!bo.f) die("Cannot open output file : " + out_path);
+        if (stream) {
+            if (cms_all_pairs_stream(ctx, (int64_t)1 << 26, write_band, &bo) != CMS_OK) die(cms_last_error(ctx));
+        } else {
+            std::vector<float> sim((size_t)n * (n > 0 ? n - 1 : 0) / 2);
+            if (cms_all_pairs(ctx, sim.data()) != CMS_OK) die(cms_last_error(ctx));
+            write_band(&bo, 0, (int64_t)sim.size(), sim.data());
+        }
+        if (fclose(bo.f) != 0) bo.ok = false;
+        if (!bo.ok || bo.written != (int64_t)n * (n > 0 ? n - 1 : 0) / 2) die("writing " + out_path + " failed");
+        cms_stats st;
+        cms_get_stats(ctx, &st);
+        auto t2 = std::chrono::steady_clock::now();
+        printf("*** load %.3f s, gpu + write %.3f s (pair kernel %.3f ms)\n", sec(t0, t1), sec(t1, t2), st.kernel_ms);
+    } else {
+        std::vector<float> sim((size_t)n * (n > 0 ? n - 1 : 0) / 2);
+        if (cms_all_pairs(ctx, sim.data()) != CMS_OK) die(cms_last_error(ctx));
+        cms_stats st;
+        cms_get_stats(ctx, &st);
+        auto t2 = std::chrono::steady_clock::now();
+        if (cms_write_dist(out_path.c_str(), n, names.data(), sim.data(), is_sim ? 1 : 0) != CMS_OK)
+            die(cms_last_error(nullptr));
+        auto t3 = std::chrono::steady_clock::now();
+        printf("*** load %.3f s, gpu %.3f s (pair kernel %.3f ms), write %.3f s\n", sec(t0, t1), sec(t1, t2),
+               st.kernel_ms, sec(t2, t3));
+    }
     cms_destroy(ctx);
     cms_table_free(table);
     cms_db_close(db);

@@ -5,6 +5,7 @@
 // fresh implementation with a tokenizer instead of iostreams so that tables of
 // 10^4 x 10^5 entries load in seconds, and with errors returned instead of
 // exit().
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -415,6 +416,82 @@ int cms_table_abundance(const cms_db *db, const cms_table *t, int cp_correct, fl
         }
         if (mapped) mapped[s] = m;
     }
+    return CMS_OK;
+}
+
+// The same abundances as cms_table_abundance, as CSR over leaves (leaf ids
+// ascending per sample; a leaf named by two columns keeps the later column's
+// value, as the dense form does). Arrays are malloc'ed; free with cms_free.
+int cms_table_abundance_csr(const cms_db *db, const cms_table *t, int cp_correct, int64_t **rowptr_out,
+                            int32_t **col_out, float **val_out) {
+    if (!db || !t || !rowptr_out || !col_out || !val_out) {
+        set_global_error("cms_table_abundance_csr: null argument");
+        return CMS_E_INVAL;
+    }
+    const size_t L = db->ids.size(), F = t->features.size(), N = t->samples.size();
+    std::vector<int32_t> leaf_of(F, -1);
+    std::vector<float> cp_of(F, 1.0f);
+    for (size_t j = 0; j < F; j++) {
+        std::string a = check_otu(t->features[j]);
+        if (cp_correct) {
+            auto it = db->cp.find(a);
+            if (it != db->cp.end()) cp_of[j] = it->second;
+        }
+        auto it = db->id_hash.find(a);
+        if (it != db->id_hash.end()) leaf_of[j] = it->second;
+    }
+    std::vector<std::vector<std::pair<int32_t, float>>> rows(N);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long s = 0; s < (long)N; s++) {
+        const float *row = t->abd.data() + (size_t)s * F;
+        std::vector<std::pair<int32_t, float>> &r = rows[(size_t)s];
+        float total = 0;
+        for (size_t j = 0; j < F; j++) {
+            if (row[j] > 0 && leaf_of[j] >= 0) {
+                const float x = row[j] / cp_of[j];
+                r.push_back({leaf_of[j], x});
+                total += x;  // a duplicated leaf counts twice, as in Load_abd
+            }
+        }
+        total /= 100.0;
+        // last column wins for a duplicated leaf: stable sort by leaf, keep the last
+        std::stable_sort(r.begin(), r.end(), [](const std::pair<int32_t, float> &a, const std::pair<int32_t, float> &b) {
+            return a.first < b.first;
+        });
+        size_t k = 0;
+        for (size_t i = 0; i < r.size(); i++) {
+            if (k > 0 && r[k - 1].first == r[i].first) k--;
+            r[k++] = r[i];
+        }
+        r.resize(k);
+        for (auto &e : r) e.second = e.second / total;
+    }
+    size_t nnz = 0;
+    for (auto &r : rows) nnz += r.size();
+    int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * (N + 1));
+    int32_t *col = (int32_t *)malloc(sizeof(int32_t) * std::max<size_t>(nnz, 1));
+    float *val = (float *)malloc(sizeof(float) * std::max<size_t>(nnz, 1));
+    if (!rp || !col || !val) {
+        free(rp);
+        free(col);
+        free(val);
+        set_global_error("cms_table_abundance_csr: out of host memory");
+        return CMS_E_NOMEM;
+    }
+    rp[0] = 0;
+    for (size_t s = 0; s < N; s++) {
+        size_t at = (size_t)rp[s];
+        for (auto &e : rows[s]) {
+            col[at] = e.first;
+            val[at] = e.second;
+            at++;
+        }
+        rp[s + 1] = (int64_t)at;
+    }
+    (void)L;
+    *rowptr_out = rp;
+    *col_out = col;
+    *val_out = val;
     return CMS_OK;
 }
 

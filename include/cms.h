@@ -303,6 +303,11 @@ int cms_table_skipped(const cms_table *t); /* rows rejected by the width check *
  * correction (db.h:251-280). mapped[i] (optional) = mapped OTU count. */
 int cms_table_abundance(const cms_db *db, const cms_table *t, int cp_correct,
                         float *abd, int *mapped);
+/* The same abundances as CSR over leaves (ascending leaf ids per sample;
+ * arrays allocated by the library, free each with cms_free): the input of
+ * cms_set_samples_csr without the n x leaf_n dense matrix. */
+int cms_table_abundance_csr(const cms_db *db, const cms_table *t, int cp_correct,
+                            int64_t **rowptr, int32_t **leaf, float **val);
 
 /* .dist writer, Output_Matrix (cms_cuda.cu:1305-1342 = comp_sam.cpp:184-219):
  * header "\t"+names; row i = name then "\t"+value per column, value

@@ -51,3 +51,32 @@ def test_cuda_comp_errors(cms, tmp_path):
     assert r.returncode == 1 and "Error" in r.stderr
     r = subprocess.run([exe, "-Z", "1"], capture_output=True, text=True)
     assert "Unrec argument" in r.stderr
+
+
+@pytest.mark.parametrize("case", ["config0", "core"])
+@pytest.mark.parametrize("kernel", ["walk", "sharedmin"])
+def test_cuda_comp_binary_output(cms, gpu_ok, golden, tmp_path, case, kernel):
+    """--format binary: packed float32 triangle of the values .dist prints.
+    Walk: exactly (float)(1 - (double)sim) of the reference's Calc_sim; the
+    shared-minimum kernel streams bands to the file (<= 1e-5)."""
+    import numpy as np
+    print(f"cuda-comp binary {case} {kernel}", flush=True)
+    g = golden(case)
+    exe = os.path.join(cms.BIN_DIR, "cuda-comp")
+    out = str(tmp_path / "o.bin")
+    env = dict(os.environ, ParallelMETA=g.root)
+    r = subprocess.run([exe, "-T", g.table, "-D", "G", "-o", out, "--format", "binary", "--kernel", kernel], env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    names = open(out + ".names").read().split("\n")[:-1]
+    assert names == [ln.split("\t")[0] for ln in g.read("dist_rT.dist").decode().splitlines()[1:]]
+    got = np.fromfile(out, dtype="<f4")
+    i, j, s = g.hex_sims(True)
+    n = len(names)
+    assert got.size == n * (n - 1) // 2
+    exp = (1.0 - s.astype(np.float64)).astype(np.float32)
+    p = cms.packed_index(i, j, n)
+    if kernel == "walk":
+        np.testing.assert_array_equal(got[p].view(np.uint32), exp.view(np.uint32))
+    else:
+        assert np.abs(got[p].astype(np.float64) - exp).max() <= 1e-5

@@ -212,3 +212,20 @@ def test_package_refuses_without_library(tmp_path):
     r = subprocess.run([sys.executable, "-c", "import importlib; importlib.import_module('coordinated-meta-storms_amd')"],
                        cwd=str(tmp_path), capture_output=True, text=True)
     assert r.returncode != 0 and "libcms.so not built" in r.stderr
+
+
+@pytest.mark.parametrize("case", ["quirks", "config0", "core"])
+def test_abundance_csr_equals_dense(cms, golden, case):
+    # the CSR loader gives exactly the dense loader's values (duplicated OTU
+    # columns, prefixes and unmapped columns of the quirks table included)
+    g = golden(case)
+    db = cms.Database(g.root, "G")
+    t = cms.Table(g.table)
+    for cp in (True, False):
+        dense, mapped = t.abundance(db, cp)
+        csr = t.abundance_csr(db, cp)
+        np.testing.assert_array_equal(csr.dense(db.leaf_n), dense)
+        np.testing.assert_array_equal(np.diff(csr.rowptr), mapped)
+        for s in range(csr.n):
+            c = csr.col[csr.rowptr[s]:csr.rowptr[s + 1]]
+            assert np.all(np.diff(c) > 0)
