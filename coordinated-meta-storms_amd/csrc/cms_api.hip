@@ -575,8 +575,8 @@ int cms_set_samples(cms_ctx *c, int n, const float *abd) {
 namespace {
 
 // Persistent-kernel variant: forced by opts.variant, else the default, or the
-// fastest of all variants timed once on the first (heaviest) tiles when
-// opts.autotune is set. Tuning writes into `out` (later overwritten) with
+// fastest of all variants timed on a sample of tiles spread over the range
+// when opts.autotune is set. Tuning writes into `out` (later overwritten) with
 // out_mode as in launch_sm_persistent and out_base 0: with out_mode 1 the
 // caller passes in `ntiles` only tiles whose packed stored-position indices
 // fit `out`.
@@ -597,16 +597,24 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
         c->variant = 1;  // 16-entry LDS chunk, 4 waves per workgroup
         return CMS_OK;
     }
-    const int64_t sample = std::min<int64_t>(ntiles, 8192);
+    // three segments of 8,192 tiles (about one per resident wave, so each
+    // launch runs at full occupancy) at the start, middle and end of the
+    // tile range: with binned samples the first tiles are the densest, the
+    // last the sparsest
+    const int nseg = ntiles >= 3 * 8192 ? 3 : 1;
+    const int64_t seg_len = std::min<int64_t>(ntiles, 8192);
     float best = 1e30f;
     int best_v = 1;
     for (int v = 0; v < kSmVariants; v++) {
         float ms_min = 1e30f;
         for (int rep = 0; rep < 2; rep++) {
-            HIPCALL(c, launch_zero_counter(c->d_work, st));
             HIPCALL(c, hipEventRecord(c->evp0, st));
-            HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, 0, sample, c->d_work, out, out_cap, out_mode, 0,
-                                            c->grid_waves, st));
+            for (int k = 0; k < nseg; k++) {
+                const int64_t t0 = nseg == 1 ? 0 : (ntiles - seg_len) * k / (nseg - 1);
+                HIPCALL(c, launch_zero_counter(c->d_work, st));
+                HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, t0, seg_len, c->d_work, out, out_cap, out_mode, 0,
+                                                c->grid_waves, st));
+            }
             HIPCALL(c, hipEventRecord(c->evp1, st));
             HIPCALL(c, hipEventSynchronize(c->evp1));
             float ms = 0;

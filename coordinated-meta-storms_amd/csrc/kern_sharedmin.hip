@@ -77,7 +77,10 @@ __global__ void sample_mass_kernel(const int64_t *__restrict__ rowptr, const int
 //                     from xroot[segment][sample].
 // Each node writes Y[row][s] for its active child edges and ORs the
 // per-32-sample-block bits of those rows into the bitmaps.
-constexpr int kEmbedNodeChunk = 64;
+#ifndef CMS_EMBED_NODE_CHUNK
+#define CMS_EMBED_NODE_CHUNK 8  // node records (and 2x leaf positions) staged in LDS per step
+#endif
+constexpr int kEmbedNodeChunk = CMS_EMBED_NODE_CHUNK;
 constexpr int kEmbedLeafChunk = 2 * kEmbedNodeChunk;
 
 struct BitmapWriter {
