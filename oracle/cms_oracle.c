@@ -20,6 +20,10 @@
  *                      with the column->leaf map and copy numbers resolved.
  *   orc_all_pairs_f32  Multi_Comp_Table's pair loop (comp_sam.cpp:321-349),
  *                      OpenMP dynamic,1, packed upper triangle.
+ *   orc_leaf_sim_f32   comp -M 2..5 (cosine, Euclidean, JSD, Bray-Curtis;
+ *                      dist.h:13-159 via comp.h:296-330), FP32 as the
+ *                      reference; orc_leaf_sim_f64 the same from their
+ *                      definitions in FP64.
  *
  * Pinning: orc_calc_sim_f32 is checked bit-exactly against the compiled
  * reference (golden fixtures); orc_calc_sim_f64 and orc_sharedmin_f64 are two
@@ -331,5 +335,142 @@ void orc_pairs_unweight(int leaf_n, const float *abd, int order_n, const int32_t
         const float *A = abd + (size_t)row[i] * leaf_n, *B = abd + (size_t)col[i] * leaf_n;
         if (sim32) sim32[i] = orc_calc_sim_unweight_f32(order_n, o1, d1, o2, d2, od, A, B);
         if (sim64) sim64[i] = orc_unweight_f64(order_n, o1, d1, o2, d2, od, A, B);
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * Non-tree metrics of comp -M 2..5: dist.h:13-159 (Calc_Dist_Cos,
+ * Calc_Dist_E, Calc_Dist_JSD, Calc_Dist_Bray_Curtis) as dispatched by
+ * _Comp_Tree::Calc_sim(a, b, mode) (comp.h:296-330): similarity =
+ * (float)(1.0 - dist). FP32 with the reference's operation order; where the
+ * reference calls pow(float, 2) the square is taken in double and added to a
+ * float (C++ promotes pow(float, int) to double); sqrt and log of a float are
+ * the float functions (the C++ overloads), the JSD's final sqrt is double.
+ */
+static float orc_dist_cos(const float *m, const float *n, int dim) {
+    float f = 0.0f, fm_sum = 0.0f, fn_sum = 0.0f;
+    for (int i = 0; i < dim; i++) {
+        const float fm = m[i], fn = n[i];
+        f += fm * fn;
+        fm_sum += fm * fm;
+        fn_sum += fn * fn;
+    }
+    if (fm_sum * fn_sum == 0.0f) return 1.0f;
+    f = f / sqrtf(fm_sum * fn_sum);
+    return 1.0f - f;
+}
+
+static float orc_dist_e(const float *m, const float *n, int dim) {
+    float sm = 0.0f, sn = 0.0f;
+    for (int i = 0; i < dim; i++) {
+        sm += m[i];
+        sn += n[i];
+    }
+    if (sm <= 0.0f) return 1.0f;
+    if (sn <= 0.0f) return 1.0f;
+    float f = 0.0f;
+    for (int i = 0; i < dim; i++) {
+        const float d = m[i] / sm - n[i] / sn;
+        f = (float)((double)f + (double)d * (double)d);
+    }
+    return sqrtf(f);
+}
+
+static float orc_dist_jsd(const float *m, const float *n, int dim) {
+    float sm = 0.0f, sn = 0.0f;
+    for (int i = 0; i < dim; i++) {
+        sm += m[i];
+        sn += n[i];
+    }
+    float dkl_m = 0.0f, dkl_n = 0.0f;
+    for (int i = 0; i < dim; i++) {
+        const float p = m[i] / sm, q = n[i] / sn;
+        if ((p == 0.0f) && (q == 0.0f)) continue;
+        const float h = (p + q) / 2;
+        if (p > 0.0f) dkl_m += p * logf(p / h);
+        if (q > 0.0f) dkl_n += q * logf(q / h);
+    }
+    return (float)sqrt((double)(dkl_m + dkl_n) / 2.0);
+}
+
+static float orc_dist_bc(const float *m, const float *n, int dim) {
+    float sm = 0.0f, sn = 0.0f;
+    for (int i = 0; i < dim; i++) {
+        sm += m[i];
+        sn += n[i];
+    }
+    if (sm <= 0.0f) return 1.0f;
+    if (sn <= 0.0f) return 1.0f;
+    float sum = 0.0f, diff = 0.0f;
+    for (int i = 0; i < dim; i++) {
+        const float p = m[i] / sm, q = n[i] / sn;
+        sum += (p + q);
+        float d = p - q;
+        if (d < 0.0f) d = d * (-1.0f);
+        diff += d;
+    }
+    if (sum <= 0.0f) return 1.0f;
+    return diff / sum;
+}
+
+float orc_leaf_sim_f32(int mode, int dim, const float *a, const float *b) {
+    float d;
+    switch (mode) {
+    case 2: d = orc_dist_cos(a, b, dim); break;
+    case 3: d = orc_dist_e(a, b, dim); break;
+    case 5: d = orc_dist_bc(a, b, dim); break;
+    default: d = orc_dist_jsd(a, b, dim); break;
+    }
+    return (float)(1.0 - (double)d);
+}
+
+/* The same four similarities from their definitions in FP64 (normalised
+ * p = a / sum a in double). Degenerate samples follow the reference: an
+ * all-zero sample gives similarity 0 for Cos, Euclidean and Bray-Curtis; for
+ * JSD, 1 when both are all-zero and NaN when exactly one is. */
+double orc_leaf_sim_f64(int mode, int dim, const float *a, const float *b) {
+    double sa = 0.0, sb = 0.0, dot = 0.0, na = 0.0, nb = 0.0;
+    for (int i = 0; i < dim; i++) {
+        sa += a[i];
+        sb += b[i];
+        dot += (double)a[i] * b[i];
+        na += (double)a[i] * a[i];
+        nb += (double)b[i] * b[i];
+    }
+    if (mode == 2) return (na * nb == 0.0) ? 0.0 : dot / sqrt(na * nb);
+    if (mode == 3 || mode == 5) {
+        if (sa <= 0.0 || sb <= 0.0) return 0.0;
+        double f = 0.0, sum = 0.0;
+        for (int i = 0; i < dim; i++) {
+            const double p = a[i] / sa, q = b[i] / sb;
+            f += mode == 3 ? (p - q) * (p - q) : fabs(p - q);
+            sum += p + q;
+        }
+        return mode == 3 ? 1.0 - sqrt(f) : 1.0 - f / sum;
+    }
+    if (sa == 0.0 && sb == 0.0) return 1.0;
+    if (sa == 0.0 || sb == 0.0) return NAN;
+    double dk = 0.0;
+    for (int i = 0; i < dim; i++) {
+        const double p = a[i] / sa, q = b[i] / sb;
+        if (p == 0.0 && q == 0.0) continue;
+        const double h = (p + q) / 2.0;
+        if (p > 0.0) dk += p * log(p / h);
+        if (q > 0.0) dk += q * log(q / h);
+    }
+    return 1.0 - sqrt(dk / 2.0);
+}
+
+void orc_pairs_leaf(int mode, int leaf_n, const float *abd, long n_pairs, const int32_t *row, const int32_t *col,
+                    float *sim32, double *sim64, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    (void)threads;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < n_pairs; i++) {
+        const float *A = abd + (size_t)row[i] * leaf_n, *B = abd + (size_t)col[i] * leaf_n;
+        if (sim32) sim32[i] = orc_leaf_sim_f32(mode, leaf_n, A, B);
+        if (sim64) sim64[i] = orc_leaf_sim_f64(mode, leaf_n, A, B);
     }
 }

@@ -51,6 +51,10 @@ _lib.orc_pairs_f64.argtypes = [C.c_int, _f32p, C.c_int, _i32p, _f32p, _i32p, _f3
 _lib.orc_pairs_unweight.restype = None
 _lib.orc_pairs_unweight.argtypes = [C.c_int, _f32p, C.c_int, _i32p, _f32p, _i32p, _f32p, _i32p, C.c_long, _i32p,
                                     _i32p, C.c_void_p, C.c_void_p, C.c_int]
+_lib.orc_pairs_leaf.restype = None
+_lib.orc_pairs_leaf.argtypes = [C.c_int, C.c_int, _f32p, C.c_long, _i32p, _i32p, C.c_void_p, C.c_void_p, C.c_int]
+_lib.orc_leaf_sim_f32.restype = C.c_float
+_lib.orc_leaf_sim_f32.argtypes = [C.c_int, C.c_int, _f32p, _f32p]
 
 
 def _tree_args(tree):
@@ -123,4 +127,17 @@ def pairs_unweight(tree, abd, row, col, threads: int = 0):
     s64 = np.zeros(max(len(row), 1), np.float64)
     _lib.orc_pairs_unweight(abd.shape[1], abd.reshape(-1), *_tree_args(tree), len(row), row, col,
                             s32.ctypes.data, s64.ctypes.data, threads)
+    return s32[: len(row)], s64[: len(row)]
+
+
+def pairs_leaf(mode, abd, row, col, threads: int = 0):
+    """comp -M 2..5 (dist.h:13-159 via comp.h:296-330): FP32 as the reference
+    and FP64 from the definitions."""
+    abd = np.ascontiguousarray(abd, np.float32)
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    s32 = np.zeros(max(len(row), 1), np.float32)
+    s64 = np.zeros(max(len(row), 1), np.float64)
+    _lib.orc_pairs_leaf(int(mode), abd.shape[1], abd.reshape(-1), len(row), row, col, s32.ctypes.data,
+                        s64.ctypes.data, threads)
     return s32[: len(row)], s64[: len(row)]

@@ -19,6 +19,8 @@ and stores inputs + outputs as tests/golden/<case>.tar.gz:
     dist_M1.dist             comp -M 1 (unweighted Meta-Storms)
     hex_rT.txt / hex_rF.txt  "i j <C99 hex float>" per pair, Calc_sim FP32
     hex_M1.txt               the same for Calc_sim_unweight (comp.h:378-446)
+    hex_M2..M5.txt           the same for Calc_sim(a, b, mode) modes 2-5
+                             (comp.h:296-330: cosine, Euclidean, JSD, Bray-Curtis)
 
 Usage: python tests/golden/make_golden.py [case ...]
 """
@@ -135,6 +137,8 @@ def make(case):
         outs["hex_rT.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "--hex"]).encode()
         outs["hex_rF.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "-r", "F", "--hex"]).encode()
         outs["hex_M1.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "-M", "1", "--hex"]).encode()
+        for m in (2, 3, 4, 5):  # cosine, Euclidean, JSD, Bray-Curtis (dist.h)
+            outs[f"hex_M{m}.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "-M", str(m), "--hex"]).encode()
         path = os.path.join(HERE, case + ".tar.gz")
         with tarfile.open(path, "w:gz") as tf:
             tf.add(d, arcname="input")
