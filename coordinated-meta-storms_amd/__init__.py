@@ -34,6 +34,8 @@ E_INVAL, E_NOMEM, E_HIP, E_SCHEDULE, E_STATE, E_IO, E_UNSUPPORTED = -1, -2, -3, 
 E_RANGE, E_TIMEOUT = -8, -9
 KERNEL_WALK, KERNEL_SHAREDMIN = 0, 1
 METRIC_WEIGHTED, METRIC_UNWEIGHTED = 0, 1
+# comp -M 2..5 (dist.h): leaf-abundance metrics, no tree needed
+METRIC_COS, METRIC_EU, METRIC_JSD, METRIC_BC = 2, 3, 4, 5
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 REG_SIZE = 150
 

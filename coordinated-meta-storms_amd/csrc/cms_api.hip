@@ -74,11 +74,21 @@ void drop_layouts(cms_ctx *c) {
     c->cap_xroot = c->cap_colsum = 0;
     c->cap_Y = c->cap_bm = c->cap_mass = 0;
     c->sm_ready = false;
+    dfree(c->d_leafP);
+    dfree(c->d_leafbm);
+    dfree(c->d_leafstats);
+    c->cap_leafP = c->cap_leafbm = c->cap_leafstats = 0;
+    c->leaf_ready = false;
 }
+
+// -M 2..5 run on the leaf-metric kernels.
+bool is_leaf_metric(const cms_ctx *c) { return c->opts.metric >= CMS_METRIC_COS; }
 
 // The shared-minimum kernels serve CMS_KERNEL_SHAREDMIN and every
 // unweighted request (the walk computes the weighted metric only).
-bool use_sm(const cms_ctx *c) { return c->opts.kernel == CMS_KERNEL_SHAREDMIN || c->opts.metric == CMS_METRIC_UNWEIGHTED; }
+bool use_sm(const cms_ctx *c) {
+    return !is_leaf_metric(c) && (c->opts.kernel == CMS_KERNEL_SHAREDMIN || c->opts.metric == CMS_METRIC_UNWEIGHTED);
+}
 
 int check_ready(cms_ctx *c) {
     if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
@@ -294,10 +304,55 @@ SmData sm_data(const cms_ctx *c) {
     return d;
 }
 
+// Normalised leaf abundances, block bitmaps over leaves and per-sample
+// totals for the leaf metrics.
+int ensure_leaf_layout(cms_ctx *c) {
+    if (c->leaf_ready) return CMS_OK;
+    const int64_t npad = ((int64_t)c->n + 63) / 64 * 64;
+    const int L = c->tree.leaf_n;
+    const int64_t nwords = ((int64_t)L + 31) / 32;
+    const size_t nP = (size_t)std::max<int64_t>((int64_t)L * npad, 1);
+    const size_t nbm = (size_t)std::max<int64_t>(npad / kSmBlock * nwords, 1);
+    HIPCALL(c, dreserve(c->d_leafP, c->cap_leafP, nP));
+    HIPCALL(c, dreserve(c->d_leafbm, c->cap_leafbm, nbm));
+    HIPCALL(c, dreserve(c->d_leafstats, c->cap_leafstats, (size_t)std::max<int64_t>(3 * npad, 1)));
+    HIPCALL(c, hipMemsetAsync(c->d_leafP, 0, sizeof(float) * nP, c->stream));
+    HIPCALL(c, hipMemsetAsync(c->d_leafbm, 0, sizeof(uint32_t) * nbm, c->stream));
+    HIPCALL(c, hipMemsetAsync(c->d_leafstats, 0, sizeof(double) * std::max<int64_t>(3 * npad, 1), c->stream));
+    LeafData d{};
+    d.P = c->d_leafP;
+    d.bm = c->d_leafbm;
+    d.stats = c->d_leafstats;
+    d.npad = npad;
+    d.nwords = nwords;
+    d.n = c->n;
+    d.leaf_n = L;
+    HIPCALL(c, launch_leaf_prepare(c->d_rowptr, c->d_col, c->d_val, c->n, d, c->stream));
+    c->leaf_ready = true;
+    return CMS_OK;
+}
+
+LeafData leaf_data(const cms_ctx *c) {
+    LeafData d{};
+    d.P = c->d_leafP;
+    d.bm = c->d_leafbm;
+    d.stats = c->d_leafstats;
+    d.npad = ((int64_t)c->n + 63) / 64 * 64;
+    d.nwords = ((int64_t)c->tree.leaf_n + 31) / 32;
+    d.n = c->n;
+    d.leaf_n = c->tree.leaf_n;
+    return d;
+}
+
 int prepare(cms_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
-    if (use_sm(c)) {
+    if (is_leaf_metric(c)) {
+        if (c->leaf_ready) return CMS_OK;
+        if (c->has_negative) return fail(c, CMS_E_UNSUPPORTED, "-M 2..5 on the GPU need non-negative abundances");
+        HIPCALL(c, hipEventRecord(c->evp0, c->stream));
+        rc = ensure_leaf_layout(c);
+    } else if (use_sm(c)) {
         if (c->sm_ready && c->sm_metric == c->opts.metric && c->sm_binned == (c->opts.bin != 0)) return CMS_OK;
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
         rc = ensure_sharedmin(c);
@@ -353,9 +408,20 @@ void cms_opts_default(cms_opts *o) {
     o->storage = CMS_STORE_F32;
 }
 
+namespace {
+const char *opts_error(const cms_opts *o) {
+    if (o->kernel != CMS_KERNEL_WALK && o->kernel != CMS_KERNEL_SHAREDMIN) return "unknown kernel";
+    if (o->metric < CMS_METRIC_WEIGHTED || o->metric > CMS_METRIC_BC) return "unknown metric";
+    if (o->storage < CMS_STORE_F32 || o->storage > CMS_STORE_F16) return "unknown storage";
+    if (o->embed_slice < 0) return "embed_slice must be >= 0";
+    return nullptr;
+}
+}  // namespace
+
 int cms_create(int device, const cms_opts *o, cms_ctx **out) {
     if (!out) return fail(nullptr, CMS_E_INVAL, "cms_create: out is NULL");
     *out = nullptr;
+    if (o && opts_error(o)) return fail(nullptr, CMS_E_INVAL, std::string("cms_create: ") + opts_error(o));
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
     if (e != hipSuccess || count <= 0)
@@ -415,10 +481,7 @@ const char *cms_last_error(const cms_ctx *c) { return c ? c->err.c_str() : globa
 int cms_set_opts(cms_ctx *c, const cms_opts *o) {
     if (!c || !o) return fail(c, CMS_E_INVAL, "cms_set_opts: null argument");
     if (c->servers > 0) return fail(c, CMS_E_STATE, "the context is serving (destroy its server first)");
-    if (o->kernel != CMS_KERNEL_WALK && o->kernel != CMS_KERNEL_SHAREDMIN) return fail(c, CMS_E_INVAL, "unknown kernel");
-    if (o->metric != CMS_METRIC_WEIGHTED && o->metric != CMS_METRIC_UNWEIGHTED) return fail(c, CMS_E_INVAL, "unknown metric");
-    if (o->storage < CMS_STORE_F32 || o->storage > CMS_STORE_F16) return fail(c, CMS_E_INVAL, "unknown storage");
-    if (o->embed_slice < 0) return fail(c, CMS_E_INVAL, "embed_slice must be >= 0");
+    if (opts_error(o)) return fail(c, CMS_E_INVAL, opts_error(o));
     c->opts = *o;
     return CMS_OK;
 }
@@ -652,6 +715,22 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
     }
     const double pairs = (double)c->n * (c->n - 1) / 2.0;
     c->stats.dense_updates = pairs * c->tree.order_n;
+    if (is_leaf_metric(c)) {
+        if (!c->d_work) HIPCALL(c, dalloc(c->d_work, 1));
+        if (c->grid_waves == 0) {
+            int cus = 0;
+            HIPCALL(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+            c->grid_waves = std::max(cus, 1) * 32;
+        }
+        const int64_t npairs = (int64_t)c->n * (c->n > 0 ? c->n - 1 : 0) / 2;
+        HIPCALL(c, hipEventRecord(c->ev0, st));
+        HIPCALL(c, launch_zero_counter(c->d_work, st));
+        HIPCALL(c, launch_leaf_all_pairs(leaf_data(c), c->opts.metric, c->d_work, d_sim, npairs, c->grid_waves, st));
+        HIPCALL(c, hipEventRecord(c->ev1, st));
+        c->stats.pair_node_updates = c->stats.valu_ops = c->stats.bytes_moved = 0.0;
+        c->kern_pending = true;
+        return CMS_OK;
+    }
     if (!use_sm(c)) HIPCALL(c, hipEventRecord(c->ev0, st));
     if (use_sm(c)) {
         const int64_t nb = (c->n + kSmBlock - 1) / kSmBlock;
@@ -740,6 +819,7 @@ int cms_prepare(cms_ctx *c, int force) {
     if (force) {
         c->ab_storage = -1;
         c->sm_ready = false;
+        c->leaf_ready = false;
     }
     return prepare(c);
 }
@@ -762,6 +842,7 @@ int cms_all_pairs(cms_ctx *c, float *sim) {
 int cms_sim_pairs(cms_ctx *c, int64_t n_pairs, const int32_t *row, const int32_t *col, float *sim) {
     int rc = prepare(c);
     if (rc) return rc;
+    if (is_leaf_metric(c)) return fail(c, CMS_E_UNSUPPORTED, "-M 2..5 run through cms_all_pairs only");
     if (n_pairs < 0 || (n_pairs > 0 && (!row || !col || !sim))) return fail(c, CMS_E_INVAL, "cms_sim_pairs: bad arrays");
     if (n_pairs == 0) return CMS_OK;
     for (int64_t i = 0; i < n_pairs; i++)
@@ -788,6 +869,7 @@ int cms_sim_pairs(cms_ctx *c, int64_t n_pairs, const int32_t *row, const int32_t
 int cms_tile(cms_ctx *c, int r0, int nr, int c0, int nc, float *out) {
     int rc = prepare(c);
     if (rc) return rc;
+    if (is_leaf_metric(c)) return fail(c, CMS_E_UNSUPPORTED, "-M 2..5 run through cms_all_pairs only");
     if (r0 < 0 || c0 < 0 || nr < 0 || nc < 0 || (int64_t)r0 + nr > c->n || (int64_t)c0 + nc > c->n)
         return fail(c, CMS_E_INVAL, "cms_tile: range outside the sample set");
     const size_t cnt = (size_t)nr * nc;
@@ -830,6 +912,7 @@ int cms_sample_order(cms_ctx *c, int32_t *order) {
 int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *user) {
     int rc = prepare(c);
     if (rc) return rc;
+    if (is_leaf_metric(c)) return fail(c, CMS_E_UNSUPPORTED, "-M 2..5 run through cms_all_pairs only");
     if (!use_sm(c)) return fail(c, CMS_E_UNSUPPORTED, "streamed all-pairs runs on the shared-minimum kernel");
     if (!fn) return fail(c, CMS_E_INVAL, "cms_all_pairs_stream: null callback");
     const int64_t n = c->n, nb = (n + kSmBlock - 1) / kSmBlock;
@@ -987,6 +1070,7 @@ namespace cms {
 int ctx_prepare(cms_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (is_leaf_metric(c)) return fail(c, CMS_E_UNSUPPORTED, "serving runs -M 0 and -M 1 only");
     return ensure_sharedmin(c);
 }
 

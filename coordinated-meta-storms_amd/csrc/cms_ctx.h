@@ -67,6 +67,12 @@ struct cms_ctx {
     int scale_bits = 0;
     double max_mass = 0.0;  // largest sample column sum of y (FP64) that chose scale_bits
     int servers = 0;        // serving loops reading this context's layout
+    // leaf metrics (-M 2..5)
+    bool leaf_ready = false;
+    float *d_leafP = nullptr;
+    uint32_t *d_leafbm = nullptr;
+    double *d_leafstats = nullptr;
+    size_t cap_leafP = 0, cap_leafbm = 0, cap_leafstats = 0;
     std::vector<int32_t> h_leaf_uses;  // per leaf: leaf-stream positions consuming it
     unsigned long long *d_counter = nullptr;  // entries visited by shared-minimum tiles
 

@@ -1,8 +1,9 @@
 // cuda-comp — command-line driver with the reference's flags and .dist output
 // (Parse_Para / Output_Matrix / main of cms_cuda.cu:1222-1342, 2237-2261),
-// computing -M 0 (weighted Meta-Storms) on the GPU through libcms.
+// computing every -M metric (0 Meta-Storms, 1 unweighted, 2..5 cosine,
+// Euclidean, JSD, Bray-Curtis) on the GPU through libcms.
 //
-//   cuda-comp -T table -D G -o out.dist [-r T|F] [-d T|F] [-M 0|1] [-t n]
+//   cuda-comp -T table -D G -o out.dist [-r T|F] [-d T|F] [-M 0..5] [-t n]
 //             [--kernel walk|sharedmin] [--storage f32|bf16|f16] [--device k]
 //             [--format dist|binary]
 //
@@ -37,7 +38,8 @@ namespace {
     printf("\t  -o Output file\n");
     printf("\t  -d Output format, distance (T) or similarity (F), default is T\n");
     printf("\t[Other options]\n");
-    printf("\t  -M (upper) Distance Metric, 0: Meta-Storms; 1: Meta-Storms-unweighted, default is 0\n");
+    printf("\t  -M (upper) Distance Metric, 0: Meta-Storms; 1: Meta-Storms-unweighted; 2: Cosine; 3: Euclidean;\n");
+    printf("\t     4: Jensen-Shannon; 5: Bray-Curtis; default is 0\n");
     printf("\t  -r rRNA copy number correction, T(rue) or F(alse), default is T\n");
     printf("\t  -t Number of host threads for loading, default is auto\n");
     printf("\t  --kernel walk|sharedmin  GPU kernel (walk is bit-identical to comp -M 0)\n");
@@ -136,14 +138,15 @@ int main(int argc, char **argv) {
         i += 2;
     }
     if (table_path.empty()) die("an OTU table (-T) is required");
-    if (metric != 0 && metric != 1)
-        die("only -M 0 (Meta-Storms) and -M 1 (unweighted Meta-Storms) run on the GPU in this build");
-    opts.metric = metric;  // -M 1 runs on the shared-minimum kernels
+    if (metric < 0 || metric > 5) metric = 4;  // Calc_sim's default case (comp.h:322-325)
 
     auto t0 = std::chrono::steady_clock::now();
     cms_db *db = nullptr;
     if (cms_db_open(nullptr, ref_db, &db) != CMS_OK) die(cms_last_error(nullptr));
-    if (!cms_db_has_tree(db)) die("the selected database has no phylogeny tree");
+    // without a tree, -M 0 and -M 1 fall back to JSD as in Calc_sim (comp.h:299-302)
+    const bool has_tree = cms_db_has_tree(db) && cms_db_order_n(db) > 0;
+    if (!has_tree && metric < 2) metric = 4;
+    opts.metric = metric;  // -M 1 runs on the shared-minimum kernels, -M 2..5 on the leaf kernels
     cms_table *table = nullptr;
     if (cms_table_load(table_path.c_str(), &table) != CMS_OK) die(cms_last_error(nullptr));
     const int n = cms_table_n(table), L = cms_db_leaf_n(db), on = cms_db_order_n(db);
@@ -156,14 +159,17 @@ int main(int argc, char **argv) {
     auto t1 = std::chrono::steady_clock::now();
     if (out_path.empty()) die("Cannot open output file : ");
 
-    const int32_t *o1, *o2, *od;
-    const float *d1, *d2;
-    cms_db_order(db, &o1, &d1, &o2, &d2, &od);
-    const bool stream = binary && (opts.kernel == CMS_KERNEL_SHAREDMIN || metric == 1);
+    const int32_t *o1 = nullptr, *o2 = nullptr, *od = nullptr;
+    const float *d1 = nullptr, *d2 = nullptr;
+    const int32_t none_i = 0;
+    const float none_f = 0.0f;
+    if (on > 0) cms_db_order(db, &o1, &d1, &o2, &d2, &od);
+    if (!o1) { o1 = o2 = od = &none_i; d1 = d2 = &none_f; }
+    const bool stream = binary && metric < 2 && (opts.kernel == CMS_KERNEL_SHAREDMIN || metric == 1);
     if (stream) opts.bin = 0;  // bands then arrive in the packed order of the table's samples
     cms_ctx *ctx = nullptr;
     if (cms_create(device, &opts, &ctx) != CMS_OK) die(cms_last_error(nullptr));
-    if (cms_set_tree(ctx, L, on, o1, d1, o2, d2, od) != CMS_OK) die(cms_last_error(ctx));
+    if (cms_set_tree(ctx, L, has_tree ? on : 0, o1, d1, o2, d2, od) != CMS_OK) die(cms_last_error(ctx));
     if (cms_set_samples_csr(ctx, n, rowptr, leaf, val) != CMS_OK) die(cms_last_error(ctx));
     cms_free(rowptr);
     cms_free(leaf);

@@ -152,6 +152,26 @@ hipError_t launch_sm_rect(const SmData &d, int r0, int nr, int c0, int nc, float
 hipError_t launch_sm_pairs(const SmData &d, int64_t n_pairs, const int32_t *row, const int32_t *col, float *out,
                            hipStream_t st);
 
+// --------------------------------------------------- leaf metrics (-M 2..5)
+
+// Normalised leaf abundances p = a / sum(a) as float, P[leaf][sample]
+// (leaf_n x npad), the per-32-sample-block bitmaps of leaves with mass and
+// per-sample totals stats[3 s + {0, 1, 2}] = sum a, sum p, sum p^2 (FP64).
+struct LeafData {
+    const float *P;
+    const uint32_t *bm;
+    const double *stats;
+    int64_t npad, nwords;  // nwords = ceil(leaf_n / 32)
+    int n, leaf_n;
+};
+// Fill P (zeroed by the caller), bm (zeroed) and stats from CSR samples.
+hipError_t launch_leaf_prepare(const int64_t *rowptr, const int32_t *col, const float *val, int n, const LeafData &d,
+                               hipStream_t st);
+// All pairs of metric `mode` (CMS_METRIC_COS..BC), packed upper triangle of
+// out_cap >= n (n - 1) / 2 floats; `work` is a zeroed device counter.
+hipError_t launch_leaf_all_pairs(const LeafData &d, int mode, unsigned long long *work, float *out, int64_t out_cap,
+                                 int grid_waves, hipStream_t st);
+
 // ------------------------------------------------------------------ serving
 
 // Query slots (columns of QY, same rows as the references' Y) against the

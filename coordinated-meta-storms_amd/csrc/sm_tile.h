@@ -61,6 +61,16 @@ struct CrossRows {
     }
 };
 
+// Per-entry pair operation of the tiles: acc += f(a0, b0) + f(a1, b1) for
+// two staged rows. The shared minimum adds integer minima (one v_min_u32 per
+// pair-entry and one v_add3_u32 per two).
+struct MinAddOp {
+    typedef uint32_t Acc;
+    __device__ __forceinline__ static void add2(Acc &acc, uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1) {
+        acc = acc + min(a0, b0) + min(a1, b1);
+    }
+};
+
 // One wave = 32 x 32 pairs, lane (ty, tx) = 4 rows x 4 columns.
 //
 // Visit the entries set in both blocks' bitmaps (OR of two blocks when a
@@ -72,17 +82,17 @@ struct CrossRows {
 // Only bitmap words [w_begin, w_end) are visited (w_begin a multiple of 64):
 // a tile's rows can be split over several waves whose integer partial sums
 // add up exactly.
-template <int CHUNK, class Rows>
+template <int CHUNK, class Rows, class Op = MinAddOp>
 __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const uint32_t *A1, const uint32_t *B0,
                                                        const uint32_t *B1, int64_t w_begin, int64_t w_end,
                                                        const Rows &rows, uint16_t *L, uint32_t *S, int lane,
-                                                       uint32_t (&acc)[4][4]) {
+                                                       typename Op::Acc (&acc)[4][4]) {
     static_assert(CHUNK % 2 == 0, "rows are staged in pairs");
     const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0u;
+        for (int j = 0; j < 4; j++) acc[i][j] = 0;
     uint32_t visited = 0;
     for (int64_t wb = w_begin; wb < w_end; wb += 64) {
         const int64_t wi = wb + lane;
@@ -150,7 +160,7 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
                 for (int ii = 0; ii < 4; ii++)
 #pragma unroll
                     for (int jj = 0; jj < 4; jj++)
-                        acc[ii][jj] = acc[ii][jj] + min(av0[ii], bv0[jj]) + min(av1[ii], bv1[jj]);
+                        Op::add2(acc[ii][jj], av0[ii], bv0[jj], av1[ii], bv1[jj]);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

@@ -90,3 +90,163 @@ def test_planted_mistake_moves_jsd(orc):
         if y > 0:
             wrong += y * math.log(y / z)
     assert abs((1 - math.sqrt(max(wrong, 0) / 2)) - s64[0]) > 1e-3
+
+
+# ------------------------------------------------------------------ GPU
+
+def _leaf_tree(cms, leaf_n):
+    z = np.zeros(0, np.int32)
+    f = np.zeros(0, np.float32)
+    return cms.Tree(leaf_n, z, f, z, f, z)
+
+
+def _same(got, exp, tol=TOL):
+    """|got - exp| <= tol where exp is finite; NaN exactly where exp is NaN."""
+    got = np.asarray(got, np.float64)
+    nan = np.isnan(exp)
+    np.testing.assert_array_equal(np.isnan(got), nan)
+    err = np.abs(got[~nan] - exp[~nan])
+    return float(err.max()) if err.size else 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("mode", MODES)
+def test_gpu_golden(cms, orc, gpu_ok, golden, case, mode):
+    """Golden inputs through cms_all_pairs: within 1e-5 of the reference's
+    FP32 values and of the FP64 definitions."""
+    print(f"leaf metric -M {mode} golden {case}", flush=True)
+    g = golden(case)
+    db = cms.Database(g.root, "G")
+    abd, _ = cms.Table(g.table).abundance(db, True)
+    i, j, ref = _hex(g, mode)
+    with cms.Context(0, metric=mode) as c:
+        c.set_tree(db.tree())
+        c.set_samples(abd)
+        got = c.all_pairs()
+    n = abd.shape[0]
+    p = cms.packed_index(i, j, n)
+    _, s64 = orc.pairs_leaf(mode, abd, i, j)
+    assert _same(got[p], ref.astype(np.float64)) <= TOL
+    assert _same(got[p], s64) <= TOL
+
+
+def _synthetic(seed, n, leaf_n):
+    rng = np.random.default_rng(seed)
+    a = (rng.gamma(0.3, 10.0, (n, leaf_n)) * (rng.random((n, leaf_n)) < 0.05)).astype(np.float32)
+    a[3] = 0.0                      # an all-zero sample
+    a[7] = 0.0                      # and another
+    a[11] = a[12] * 3.0             # a multiple of another sample
+    a[20] = 0.0
+    a[20, :5] = [1e-30, 1.0, 2.0, 0.0, 5.0]   # a mass far below 2^-24 of the rest
+    a[21] = 0.0
+    a[21, :5] = [4.0, 1e-30, 0.0, 7.0, 0.0]
+    a[30] = 0.0
+    a[30, leaf_n - 1] = 2.5         # one leaf, the last one
+    return a
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,leaf_n", [(2, 40), (45, 33), (203, 3000), (513, 20000)])
+def test_gpu_synthetic_vs_fp64(cms, orc, gpu_ok, mode, n, leaf_n):
+    """Ragged sample counts (tiles past n), leaf counts across bitmap windows,
+    degenerate samples; no tree (order_n 0) is accepted for these metrics."""
+    print(f"leaf metric -M {mode} synthetic n={n} leaves={leaf_n}", flush=True)
+    a = _synthetic(5 + n, max(n, 32), leaf_n)[:n]
+    with cms.Context(0, metric=mode) as c:
+        c.set_tree(_leaf_tree(cms, leaf_n))
+        c.set_samples(a)
+        got = c.all_pairs()
+        got2 = c.all_pairs()        # a second call reuses the prepared layout
+    i, j = np.triu_indices(n, 1)
+    _, s64 = orc.pairs_leaf(mode, a, i.astype(np.int32), j.astype(np.int32))
+    assert _same(got, s64) <= TOL
+    np.testing.assert_array_equal(got.view(np.uint32), got2.view(np.uint32))
+    if mode == 4 and n > 7:
+        # the reference's NaN for one empty sample, bit for bit (0/0 on the host)
+        k = cms.packed_index(np.array([0]), np.array([3]), n)[0]
+        assert got.view(np.uint32)[k] == 0xFFC00000
+        k = cms.packed_index(np.array([3]), np.array([7]), n)[0]
+        assert got[k] == 1.0
+
+
+@pytest.mark.gpu
+def test_gpu_leaf_metrics_csr_and_guards(cms, orc, gpu_ok):
+    """CSR input equals dense input bit for bit; entry points other than
+    cms_all_pairs(_device) reject -M 2..5 with an error; negative abundances
+    are rejected."""
+    a = _synthetic(9, 96, 500)
+    rowptr = np.zeros(a.shape[0] + 1, np.int64)
+    nz = [np.nonzero(r)[0] for r in a]
+    rowptr[1:] = np.cumsum([len(x) for x in nz])
+    col = np.concatenate(nz).astype(np.int32)
+    val = np.concatenate([a[k, x] for k, x in enumerate(nz)]).astype(np.float32)
+    with cms.Context(0, metric=cms.METRIC_JSD) as c:
+        c.set_tree(_leaf_tree(cms, 500))
+        c.set_samples(a)
+        dense = c.all_pairs()
+        c.set_samples_csr(cms.Samples(rowptr, col, val))
+        csr = c.all_pairs()
+        np.testing.assert_array_equal(dense.view(np.uint32), csr.view(np.uint32))
+        with pytest.raises(cms.CmsError):
+            c.sim_pairs(np.array([0], np.int32), np.array([1], np.int32))
+        with pytest.raises(cms.CmsError):
+            c.tile(0, 4, 0, 4)
+        b = a.copy()
+        b[5, 3] = -1.0
+        c.set_samples(b)
+        with pytest.raises(cms.CmsError):
+            c.all_pairs()
+
+
+def _dist_values(path):
+    rows = [ln.split("\t") for ln in open(path).read().splitlines()]
+    return rows[0], [r[0] for r in rows[1:]], np.array([[float(x) for x in r[1:]] for r in rows[1:]])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", MODES)
+def test_cuda_comp_leaf_metric(cms, gpu_ok, golden, tmp_path, mode):
+    """cuda-comp -M m prints 1 - sim of the reference's values (6 digits)."""
+    print(f"cuda-comp -M {mode}", flush=True)
+    g = golden("core")
+    exe = os.path.join(cms.BIN_DIR, "cuda-comp")
+    out = str(tmp_path / "o.dist")
+    env = dict(os.environ, ParallelMETA=g.root)
+    r = subprocess.run([exe, "-T", g.table, "-D", "G", "-o", out, "-M", str(mode)], env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    hdr, names, d = _dist_values(out)
+    ref_hdr = g.read("dist_rT.dist").decode().splitlines()[0].split("\t")
+    assert hdr == ref_hdr
+    i, j, s = _hex(g, mode)
+    exp = 1.0 - s.astype(np.float64)
+    assert _same(d[i, j], exp, 1e-5 + 5e-6) <= 1.5e-5
+    assert _same(d[j, i], exp, 1e-5 + 5e-6) <= 1.5e-5
+    assert np.all(np.diag(d) == 0.0)
+
+
+@pytest.mark.gpu
+def test_cuda_comp_no_tree_falls_back_to_jsd(cms, gpu_ok, golden, tmp_path):
+    """A database without a tree runs -M 0 as -M 4 (comp.h:299-302)."""
+    import shutil
+    g = golden("config0")
+    root = str(tmp_path / "root")
+    shutil.copytree(g.root, root)
+    cfg = os.path.join(root, "databases", "db.config")
+    lines = open(cfg).read().splitlines()
+    k = lines.index("#tree")
+    assert lines[k + 1] == "Y"
+    lines[k + 1] = "N"
+    open(cfg, "w").write("\n".join(lines) + "\n")
+    exe = os.path.join(cms.BIN_DIR, "cuda-comp")
+    env = dict(os.environ, ParallelMETA=root)
+    outs = []
+    for m in ("0", "4"):
+        out = str(tmp_path / f"o{m}.dist")
+        r = subprocess.run([exe, "-T", os.path.join(root, "table.txt"), "-D", "G", "-o", out, "-M", m], env=env,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(open(out, "rb").read())
+    assert outs[0] == outs[1]
