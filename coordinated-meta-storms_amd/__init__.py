@@ -144,6 +144,14 @@ _sig("cms_table_skipped", C.c_int, [_vp])
 _sig("cms_table_abundance", C.c_int, [_vp, _vp, C.c_int, _f32p, _vp])
 _sig("cms_table_abundance_csr", C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)])
 _sig("cms_write_dist", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), _f32p, C.c_int])
+_sig("cms_sample_file_abundance", C.c_int, [_vp, C.c_char_p, C.c_int, _f32p, C.POINTER(C.c_int)])
+_sig("cms_sample_files_csr", C.c_int, [_vp, C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(_vp),
+                                       C.POINTER(_vp), C.POINTER(_vp), _vp])
+_sig("cms_list_load", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(_vp)])
+_sig("cms_list_n", C.c_int, [_vp])
+_sig("cms_list_path", C.c_char_p, [_vp, C.c_int])
+_sig("cms_list_name", C.c_char_p, [_vp, C.c_int])
+_sig("cms_list_free", None, [_vp])
 _sig("cms_synth_tree", C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_double, _i32p, _f32p, _i32p, _f32p, _i32p,
                                  C.POINTER(C.c_int)])
 _sig("cms_synth_samples", C.c_int, [C.POINTER(SynthOpts), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)])
@@ -180,7 +188,8 @@ EXPORTED = [
     "cms_db_has_tree", "cms_db_has_cp", "cms_db_order", "cms_db_leaf_name", "cms_table_load",
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
     "cms_table_feature_name", "cms_table_skipped", "cms_table_abundance", "cms_table_abundance_csr",
-    "cms_write_dist",
+    "cms_write_dist", "cms_sample_file_abundance", "cms_sample_files_csr", "cms_list_load", "cms_list_n",
+    "cms_list_path", "cms_list_name", "cms_list_free",
     "cms_synth_tree", "cms_synth_samples", "cms_synth_copy_numbers", "cms_counts_to_abundance_csr",
     "cms_write_db", "cms_write_table", "cms_free", "cms_version", "cms_newick_parse", "cms_newick_load",
     "cms_newick_free", "cms_newick_leaf_n", "cms_newick_order_n", "cms_newick_leaf_name", "cms_newick_order",
@@ -414,6 +423,53 @@ class Table:
             for p in (rp, col, val):
                 lib.cms_free(p)
         return Samples(rowptr, cols, vals)
+
+
+def sample_file_abundance(db: Database, path: str, cp_correct: bool = True):
+    """One classification file's leaf abundances and its number of OTUs in
+    the tree (_Comp_Tree::Load_abd(const char *, float *, bool), comp.h:194-236)."""
+    out = np.zeros(max(db.leaf_n, 1), np.float32)
+    m = C.c_int(0)
+    _check(lib.cms_sample_file_abundance(db._h, path.encode(), int(cp_correct), out, C.byref(m)))
+    return out[: db.leaf_n], m.value
+
+
+def sample_files_csr(db: Database, paths, cp_correct: bool = True):
+    """Several files as CSR samples (zero abundances dropped) and the mapped
+    OTU count of each (cms_sample_files_csr)."""
+    n = len(paths)
+    arr = (C.c_char_p * max(n, 1))(*[p.encode() for p in paths])
+    mapped = np.zeros(max(n, 1), np.int32)
+    rp, col, val = _vp(), _vp(), _vp()
+    _check(lib.cms_sample_files_csr(db._h, n, arr, int(cp_correct), C.byref(rp), C.byref(col), C.byref(val),
+                                    mapped.ctypes.data_as(_vp)))
+    try:
+        rowptr = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int64)), shape=(n + 1,)).copy()
+        nnz = int(rowptr[-1])
+        cols = np.ctypeslib.as_array(C.cast(col, C.POINTER(C.c_int32)), shape=(max(nnz, 1),))[:nnz].copy()
+        vals = np.ctypeslib.as_array(C.cast(val, C.POINTER(C.c_float)), shape=(max(nnz, 1),))[:nnz].copy()
+    finally:
+        for p in (rp, col, val):
+            lib.cms_free(p)
+    return Samples(rowptr, cols, vals), mapped[:n]
+
+
+class SampleList:
+    """Sample list of `-l list [-p prefix]` (Load_List with ids and prefix,
+    utility.h:344-391): .paths (files to load) and .names (sample ids)."""
+
+    def __init__(self, path: str, prefix: str = ""):
+        h = _vp()
+        _check(lib.cms_list_load(path.encode(), prefix.encode(), C.byref(h)))
+        try:
+            n = lib.cms_list_n(h)
+            self.paths = [lib.cms_list_path(h, i).decode() for i in range(n)]
+            self.names = [lib.cms_list_name(h, i).decode() for i in range(n)]
+        finally:
+            lib.cms_list_free(h)
+
+    def __len__(self):
+        return len(self.paths)
 
 
 def write_dist(path: str, names, sim_packed: np.ndarray, is_sim: bool = False):

@@ -6,6 +6,12 @@
 //   cuda-comp -T table -D G -o out.dist [-r T|F] [-d T|F] [-M 0..5] [-t n]
 //             [--kernel walk|sharedmin] [--storage f32|bf16|f16] [--device k]
 //             [--format dist|binary]
+//   cuda-comp -l list [-p prefix] -D G -o out.dist [...]   one sample per file
+//   cuda-comp -i file1 file2 -D G [-r T|F] [-d T|F] [-M 0..5]   one pair, printed
+//
+// -l and -i read classification files (Multi_Comp / Single_Comp,
+// cms_cuda.cu:1344-1403); the last of -T / -l / -i on the command line
+// selects the mode, as in the reference.
 //
 // --format binary writes the packed upper triangle (pair (i, j), i < j, at
 // i*n + j - (i+1)(i+2)/2; the values .dist would print: 1 - sim, or sim with
@@ -33,6 +39,11 @@ namespace {
     printf("Usage: \ncuda-comp [Option] Value\nOptions: \n");
     printf("\t-D (upper) ref database, default is G\n");
     printf("\t[Input options, required]\n");
+    printf("\t  -i Two samples path for single sample comparison\n");
+    printf("\tor\n");
+    printf("\t  -l Input files list for multi-sample comparison\n");
+    printf("\t  -p List files path prefix [Optional for -l]\n");
+    printf("\tor\n");
     printf("\t  -T (upper) Input OTU count table (*.OTU.Count) for multi-sample comparison\n");
     printf("\t[Output options]\n");
     printf("\t  -o Output file\n");
@@ -81,7 +92,8 @@ void write_band(void *user, int64_t first, int64_t count, const float *sims) {
 
 int main(int argc, char **argv) {
     char ref_db = 'G';
-    std::string table_path, out_path;
+    std::string table_path, out_path, list_path, list_prefix, query1, query2;
+    int mode = 0;  // 0: -i (the reference's default), 1: -l, 2: -T
     bool cp_correct = true, is_sim = false;
     int metric = 0, device = 0;
     bool binary = false;
@@ -120,15 +132,21 @@ int main(int argc, char **argv) {
         } else {
             switch (a[1]) {
             case 'D': ref_db = need(1)[0]; break;
-            case 'T': table_path = need(1); break;
+            case 'T': table_path = need(1); mode = 2; break;
+            case 'l': list_path = need(1); mode = 1; break;
+            case 'p': list_prefix = need(1); break;
+            case 'i':
+                query1 = need(1);
+                query2 = need(2);
+                mode = 0;
+                i++;
+                break;
             case 'o': out_path = need(1); break;
             case 'M': metric = atoi(need(1)); break;
             case 'r': { const char *v = need(1); if (v[0] == 'f' || v[0] == 'F') cp_correct = false; } break;
             case 'd': { const char *v = need(1); if (v[0] == 'f' || v[0] == 'F') is_sim = true; } break;
             case 't': (void)need(1); break;
             case 'P': case 'c': (void)need(1); break;
-            case 'i': case 'l': case 'p':
-                die("single-file (-i) and list (-l/-p) modes are not part of this build; use -T");
             case 'h': printhelp();
             default:
                 fprintf(stderr, "Error: Unrec argument %s\n", argv[i]);
@@ -137,7 +155,7 @@ int main(int argc, char **argv) {
         }
         i += 2;
     }
-    if (table_path.empty()) die("an OTU table (-T) is required");
+    if (mode == 0 && (query1.empty() || query2.empty())) die("an input is required: -i file1 file2, -l list or -T table");
     if (metric < 0 || metric > 5) metric = 4;  // Calc_sim's default case (comp.h:322-325)
 
     auto t0 = std::chrono::steady_clock::now();
@@ -147,17 +165,41 @@ int main(int argc, char **argv) {
     const bool has_tree = cms_db_has_tree(db) && cms_db_order_n(db) > 0;
     if (!has_tree && metric < 2) metric = 4;
     opts.metric = metric;  // -M 1 runs on the shared-minimum kernels, -M 2..5 on the leaf kernels
+    const int L = cms_db_leaf_n(db), on = cms_db_order_n(db);
     cms_table *table = nullptr;
-    if (cms_table_load(table_path.c_str(), &table) != CMS_OK) die(cms_last_error(nullptr));
-    const int n = cms_table_n(table), L = cms_db_leaf_n(db), on = cms_db_order_n(db);
+    cms_list *list = nullptr;
+    int n = 0;
     int64_t *rowptr = nullptr;
     int32_t *leaf = nullptr;
     float *val = nullptr;
-    if (cms_table_abundance_csr(db, table, cp_correct ? 1 : 0, &rowptr, &leaf, &val) != CMS_OK)
-        die(cms_last_error(nullptr));
-    printf("%d files loaded\n", n);
+    std::vector<int> mapped;
+    if (mode == 2) {
+        if (cms_table_load(table_path.c_str(), &table) != CMS_OK) die(cms_last_error(nullptr));
+        n = cms_table_n(table);
+        if (cms_table_abundance_csr(db, table, cp_correct ? 1 : 0, &rowptr, &leaf, &val) != CMS_OK)
+            die(cms_last_error(nullptr));
+    } else {
+        std::vector<const char *> files;
+        if (mode == 1) {
+            if (cms_list_load(list_path.c_str(), list_prefix.c_str(), &list) != CMS_OK) die(cms_last_error(nullptr));
+            n = cms_list_n(list);
+            for (int k = 0; k < n; k++) files.push_back(cms_list_path(list, k));
+        } else {
+            n = 2;
+            files = {query1.c_str(), query2.c_str()};
+        }
+        mapped.assign((size_t)std::max(n, 1), 0);
+        if (cms_sample_files_csr(db, n, files.data(), cp_correct ? 1 : 0, &rowptr, &leaf, &val, mapped.data()) !=
+            CMS_OK)
+            die(cms_last_error(nullptr));
+    }
+    if (mode == 0) {
+        printf("%d OTUs in file 1\n%d OTUs in file 2\n", mapped[0], mapped[1]);
+    } else {
+        printf("%d files loaded\n", n);
+    }
     auto t1 = std::chrono::steady_clock::now();
-    if (out_path.empty()) die("Cannot open output file : ");
+    if (mode != 0 && out_path.empty()) die("Cannot open output file : ");
 
     const int32_t *o1 = nullptr, *o2 = nullptr, *od = nullptr;
     const float *d1 = nullptr, *d2 = nullptr;
@@ -175,9 +217,15 @@ int main(int argc, char **argv) {
     cms_free(leaf);
     cms_free(val);
     std::vector<const char *> names(n);
-    for (int k = 0; k < n; k++) names[k] = cms_table_sample_name(table, k);
+    for (int k = 0; k < n; k++) names[k] = mode == 2 ? cms_table_sample_name(table, k) : mode == 1 ? cms_list_name(list, k) : "";
     auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
-    if (binary) {
+    if (mode == 0) {
+        // Single_Comp (cms_cuda.cu:1344-1360): the value on stdout, printed
+        // as std::ostream prints a float (sim) or a double (1.0 - sim)
+        float sim = 0.0f;
+        if (cms_all_pairs(ctx, &sim) != CMS_OK) die(cms_last_error(ctx));
+        printf("%.6g\n", is_sim ? (double)sim : 1.0 - (double)sim);
+    } else if (binary) {
         const std::string names_path = out_path + ".names";
         FILE *nf = fopen(names_path.c_str(), "w");
         if (!nf) die("Cannot open output file : " + names_path);
@@ -213,7 +261,8 @@ int main(int argc, char **argv) {
                st.kernel_ms, sec(t2, t3));
     }
     cms_destroy(ctx);
-    cms_table_free(table);
+    if (list) cms_list_free(list);
+    if (table) cms_table_free(table);
     cms_db_close(db);
     return 0;
 }

@@ -16,6 +16,11 @@
 #include <unordered_map>
 #include <vector>
 
+// the reference's hash container (hash.h), for the iteration order of the
+// sample-file loader
+#define _GLIBCXX_PERMIT_BACKWARD_HASH
+#include <ext/hash_map>
+
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -544,5 +549,242 @@ int cms_write_dist(const char *path, int n, const char *const *names, const floa
 
 void cms_free(void *p) { free(p); }
 const char *cms_version(void) { return "cms-mi355x 0.1"; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ sample files (-i / -l)
+
+// Classification files hold one sample each. The reference keeps a file's
+// OTU counts in a __gnu_cxx::hash_map<string, int> with
+// hash<const char *> over the id (hash.h:12-19) and sums the normalisation
+// total in that container's iteration order (comp.h:194-236). The FP32 total
+// depends on that order, so the same libstdc++ container is used here with
+// the same sequence of container operations (an increment through
+// operator[] can grow the table, so increments go through it too).
+
+struct cms_list {
+    std::vector<std::string> paths, names;
+};
+
+namespace {
+
+struct OtuIdHash {
+    size_t operator()(const std::string &x) const { return __gnu_cxx::hash<const char *>()(x.c_str()); }
+};
+typedef __gnu_cxx::hash_map<std::string, int, OtuIdHash> OtuCounts;
+
+// Leading int of a token as sscanf("%d") reads it (0 when none: the
+// reference's count variable keeps its initial 0).
+int scan_int(const char *ts, const char *te) {
+    std::string t(ts, te);
+    char *end = nullptr;
+    errno = 0;
+    long v = strtol(t.c_str(), &end, 10);
+    if (end == t.c_str()) return 0;
+    return (int)v;
+}
+
+// _OTU_Parser::Load_file_to_hash (otu_parser.h:104-157): the title line
+// selects the format. "#S..." (per-read listing, old format): every line is
+// "<read id> <otu id> ...", one count per line. Otherwise (new format): every
+// line is "<otu id> <count> ...", the last line of an id sets its count.
+// Lines without the fields the format needs are skipped (the reference then
+// reuses the previous line's buffer).
+bool load_class_file(const std::string &path, OtuCounts &counts, std::string &err) {
+    std::string buf;
+    if (!read_file(path, buf)) {
+        err = "Cannot open input file: " + path;
+        return false;
+    }
+    std::vector<std::pair<size_t, size_t>> lines;
+    split_lines(buf, lines);
+    if (lines.empty()) return true;
+    const bool old_format = lines[0].second > 1 && buf[lines[0].first + 1] == 'S';
+    for (size_t k = 1; k < lines.size(); k++) {
+        const char *p = buf.data() + lines[k].first, *e = p + lines[k].second, *ts, *te;
+        if (old_format) {
+            if (!next_token(p, e, ts, te) || !next_token(p, e, ts, te)) continue;
+            const std::string id(ts, te);
+            if (counts.count(id) == 0) counts[id] = 1;
+            else counts[id]++;
+        } else {
+            if (!next_token(p, e, ts, te)) continue;
+            const std::string id(ts, te);
+            int c = 0;
+            if (next_token(p, e, ts, te)) c = scan_int(ts, te);
+            counts[id] = c;
+        }
+    }
+    return true;
+}
+
+// _Comp_Tree::Load_abd(const char *, float *, bool) (comp.h:194-236):
+// count / copy number per OTU, FP32 total in the hash map's order,
+// total = (float)(total / 100.0), a = x / total for ids in the tree.
+// Returns the number of the file's OTUs found in the tree. A file whose
+// counts sum to zero gives an all-zero sample (the reference divides by 0).
+bool sample_file_abundance(const cms_db *db, const std::string &path, bool cp_correct,
+                           std::vector<std::pair<int32_t, float>> &out, int &mapped, std::string &err) {
+    OtuCounts counts;
+    out.clear();
+    mapped = 0;
+    if (!load_class_file(path, counts, err)) return false;
+    std::vector<std::pair<const std::string *, float>> x;
+    x.reserve(counts.size());
+    float total = 0;
+    for (OtuCounts::const_iterator it = counts.begin(); it != counts.end(); ++it) {
+        float cp_no = 1.0f;
+        if (cp_correct) {
+            auto c = db->cp.find(it->first);
+            if (c != db->cp.end()) cp_no = c->second;
+        }
+        const float v = (float)it->second / cp_no;
+        x.emplace_back(&it->first, v);
+        total += v;
+    }
+    total /= 100.0;
+    for (auto &e : x) {
+        auto id = db->id_hash.find(*e.first);
+        if (id == db->id_hash.end()) continue;
+        mapped++;
+        const float a = total != 0.0f ? e.second / total : 0.0f;
+        out.emplace_back(id->second, a);
+    }
+    std::sort(out.begin(), out.end());
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cms_sample_file_abundance(const cms_db *db, const char *path, int cp_correct, float *abd, int *mapped) {
+    if (!db || !path || !abd) {
+        set_global_error("cms_sample_file_abundance: null argument");
+        return CMS_E_INVAL;
+    }
+    std::vector<std::pair<int32_t, float>> v;
+    int m = 0;
+    std::string err;
+    if (!sample_file_abundance(db, path, cp_correct != 0, v, m, err)) {
+        set_global_error(err);
+        return CMS_E_IO;
+    }
+    memset(abd, 0, db->ids.size() * sizeof(float));
+    for (auto &e : v) abd[e.first] = e.second;
+    if (mapped) *mapped = m;
+    return CMS_OK;
+}
+
+int cms_sample_files_csr(const cms_db *db, int n, const char *const *paths, int cp_correct, int64_t **rowptr_out,
+                         int32_t **col_out, float **val_out, int *mapped) {
+    if (!db || n < 0 || (n > 0 && !paths) || !rowptr_out || !col_out || !val_out) {
+        set_global_error("cms_sample_files_csr: null argument");
+        return CMS_E_INVAL;
+    }
+    std::vector<std::vector<std::pair<int32_t, float>>> rows((size_t)n);
+    std::vector<std::string> errs((size_t)n);
+    std::vector<int> m((size_t)n, 0);
+    std::vector<char> ok((size_t)n, 1);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < n; s++) {
+        ok[s] = paths[s] && sample_file_abundance(db, paths[s], cp_correct != 0, rows[s], m[s], errs[s]);
+        if (!paths[s]) errs[s] = "cms_sample_files_csr: null path";
+        // zero abundances are absent entries of the CSR form
+        rows[s].erase(std::remove_if(rows[s].begin(), rows[s].end(),
+                                     [](const std::pair<int32_t, float> &e) { return e.second == 0.0f; }),
+                      rows[s].end());
+    }
+    for (int s = 0; s < n; s++)
+        if (!ok[s]) {
+            set_global_error(errs[s]);
+            return CMS_E_IO;
+        }
+    size_t nnz = 0;
+    for (auto &r : rows) nnz += r.size();
+    int64_t *rowptr = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    int32_t *col = (int32_t *)malloc(sizeof(int32_t) * std::max<size_t>(nnz, 1));
+    float *val = (float *)malloc(sizeof(float) * std::max<size_t>(nnz, 1));
+    if (!rowptr || !col || !val) {
+        free(rowptr);
+        free(col);
+        free(val);
+        set_global_error("cms_sample_files_csr: out of host memory");
+        return CMS_E_NOMEM;
+    }
+    rowptr[0] = 0;
+    for (int s = 0; s < n; s++) {
+        int64_t k = rowptr[s];
+        for (auto &e : rows[s]) {
+            col[k] = e.first;
+            val[k] = e.second;
+            k++;
+        }
+        rowptr[s + 1] = k;
+        if (mapped) mapped[s] = m[s];
+    }
+    *rowptr_out = rowptr;
+    *col_out = col;
+    *val_out = val;
+    return CMS_OK;
+}
+
+// Load_List(list, files, ids, prefix) (utility.h:344-391): per non-empty
+// line, "<id> <file> ..." when it has two or more fields, else the whole
+// line is the file and the id is the directory that holds it (the last
+// path component before the final '/', or everything before a single '/',
+// or the whole line without '/'). The prefix is prepended to every file.
+int cms_list_load(const char *path, const char *prefix, cms_list **out) {
+    if (!path || !out) {
+        set_global_error("cms_list_load: null argument");
+        return CMS_E_INVAL;
+    }
+    *out = nullptr;
+    std::string buf;
+    if (!read_file(path, buf)) {
+        set_global_error(std::string("Cannot open file: ") + path);
+        return CMS_E_IO;
+    }
+    const std::string pre = prefix ? prefix : "";
+    std::vector<std::pair<size_t, size_t>> lines;
+    split_lines(buf, lines);
+    cms_list *l = new cms_list();
+    for (auto &ln : lines) {
+        if (ln.second == 0) continue;
+        const std::string line = buf.substr(ln.first, ln.second);
+        const char *p = line.data(), *e = p + line.size(), *ts, *te;
+        std::vector<std::string> f;
+        while (next_token(p, e, ts, te)) f.emplace_back(ts, te);
+        if (f.size() >= 2) {
+            l->names.push_back(f[0]);
+            l->paths.push_back(pre + f[1]);
+            continue;
+        }
+        l->paths.push_back(pre + line);
+        const size_t first = line.find_first_of('/'), last = line.find_last_of('/');
+        if (last == std::string::npos) {
+            l->names.push_back(line);
+        } else if (first == last) {
+            l->names.push_back(line.substr(0, last));
+        } else {
+            const size_t b = line.rfind('/', last - 1) + 1;
+            l->names.push_back(line.substr(b, last - b));
+        }
+    }
+    *out = l;
+    return CMS_OK;
+}
+
+int cms_list_n(const cms_list *l) { return l ? (int)l->paths.size() : CMS_E_INVAL; }
+
+const char *cms_list_path(const cms_list *l, int i) {
+    return (l && i >= 0 && i < (int)l->paths.size()) ? l->paths[(size_t)i].c_str() : nullptr;
+}
+
+const char *cms_list_name(const cms_list *l, int i) {
+    return (l && i >= 0 && i < (int)l->names.size()) ? l->names[(size_t)i].c_str() : nullptr;
+}
+
+void cms_list_free(cms_list *l) { delete l; }
 
 }  // extern "C"

@@ -313,6 +313,39 @@ int cms_table_abundance(const cms_db *db, const cms_table *t, int cp_correct,
 int cms_table_abundance_csr(const cms_db *db, const cms_table *t, int cp_correct,
                             int64_t **rowptr, int32_t **leaf, float **val);
 
+/* ---------------------------------------- sample files and lists (-i / -l)
+ *
+ * One sample per classification file. Replaces _Comp_Tree::Load_abd(const
+ * char *, float *, bool) (comp.h:194-236) over
+ * _OTU_Parser::Load_file_to_hash (otu_parser.h:104-157): a title line
+ * starting "#S" selects the per-read format ("<read> <otu> ..." per line,
+ * one count each), any other title the count format ("<otu> <count> ...",
+ * the last line of an id wins); count / copy number per OTU, FP32 total in
+ * the reference's hash-map order, a = x / (total / 100) for the OTUs in the
+ * tree. abd: leaf_n floats (zeroed first); *mapped = OTUs found in the tree
+ * (the number `cuda-comp -i` prints). A file that cannot be opened is
+ * CMS_E_IO; counts summing to zero give an all-zero sample. */
+int cms_sample_file_abundance(const cms_db *db, const char *path, int cp_correct,
+                              float *abd, int *mapped);
+
+/* The same for n files as CSR over leaves (zero abundances dropped; loaded
+ * in parallel; arrays allocated by the library, free each with cms_free);
+ * mapped: n ints or NULL. Replaces the Load_abd loop of Multi_Comp
+ * (cms_cuda.cu:1362-1375, comp_sam.cpp:240-258). */
+int cms_sample_files_csr(const cms_db *db, int n, const char *const *paths, int cp_correct,
+                         int64_t **rowptr, int32_t **leaf, float **val, int *mapped);
+
+/* Sample list of `-l list [-p prefix]`: Load_List(list, files, ids, prefix)
+ * (utility.h:344-391). Per non-empty line: "<id> <file> ..." with two or
+ * more fields, else the whole line is the file and the id is the directory
+ * holding it. The prefix is prepended to every file. */
+typedef struct cms_list cms_list;
+int cms_list_load(const char *path, const char *prefix, cms_list **out);
+int cms_list_n(const cms_list *l);
+const char *cms_list_path(const cms_list *l, int i);
+const char *cms_list_name(const cms_list *l, int i);
+void cms_list_free(cms_list *l);
+
 /* .dist writer, Output_Matrix (cms_cuda.cu:1305-1342 = comp_sam.cpp:184-219):
  * header "\t"+names; row i = name then "\t"+value per column, value
  * = 1.0 - (double)sim (0 on the diagonal), or sim (1 on the diagonal) when

@@ -7,17 +7,23 @@
 //   --time P T       time Calc_sim over the first P pairs of the upper-triangle
 //                    pair list (comp_sam.cpp:321-330) on T OpenMP threads,
 //                    dynamic,1 (comp_sam.cpp:341), with a wall clock; prints
-//                    one JSON line.
+//                    one JSON line;
+//   --files LIST     for every path in LIST (one per line), the abundances of
+//                    _Comp_Tree::Load_abd(const char *, float *, bool)
+//                    (comp.h:194-236): "# <k> <mapped>" then "<leaf> <hex>"
+//                    for every nonzero leaf.
 //
 // TEST INFRASTRUCTURE ONLY (the checker / CPU baseline), never part of the
 // product. Needs ParallelMETA set, as the reference does at static-init time
 // (db.h:285, utility.h:27-41).
 //
 // Usage: ref_driver -D <letter> -T <table> [-r T|F] [-M metric] (--hex | --time P T)
+//        ref_driver -D <letter> [-r T|F] --files LIST
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -25,7 +31,7 @@
 
 int main(int argc, char **argv) {
     char db = 'G';
-    std::string table_path;
+    std::string table_path, files_list;
     bool cp = true, hex = false;
     int metric = 0;
     long time_pairs = 0;
@@ -38,7 +44,27 @@ int main(int argc, char **argv) {
         else if (a == "--hex") hex = true;
         else if (a == "-M" && i + 1 < argc) metric = atoi(argv[++i]);
         else if (a == "--time" && i + 2 < argc) { time_pairs = atol(argv[++i]); threads = atoi(argv[++i]); }
+        else if (a == "--files" && i + 1 < argc) files_list = argv[++i];
         else { fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
+    }
+    if (!files_list.empty()) {
+        _Comp_Tree tree(db);
+        const int L = tree.Get_LeafN();
+        std::vector<float> abd(L);
+        std::ifstream in(files_list.c_str());
+        std::string path;
+        int k = 0;
+        while (std::getline(in, path)) {
+            if (path.empty()) continue;
+            const int m = tree.Load_abd(path.c_str(), abd.data(), cp);
+            printf("# %d %d\n", k++, m);
+            for (int l = 0; l < L; l++) {
+                uint32_t bits;
+                memcpy(&bits, &abd[l], 4);
+                if (bits) printf("%d %a\n", l, (double)abd[l]);
+            }
+        }
+        return 0;
     }
     if (table_path.empty()) { fprintf(stderr, "need -T table\n"); return 2; }
     _Comp_Tree tree(db);

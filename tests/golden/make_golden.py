@@ -22,6 +22,15 @@ and stores inputs + outputs as tests/golden/<case>.tar.gz:
     hex_M2..M5.txt           the same for Calc_sim(a, b, mode) modes 2-5
                              (comp.h:296-330: cosine, Euclidean, JSD, Bray-Curtis)
 
+The `files` case holds the single-file inputs of `-i` and `-l` instead of a
+table: classification files in both formats (count lines and per-read
+lines) with the loader's quirks, and three sample lists; it stores
+
+    abd_rT.txt / abd_rF.txt  ref_driver --files: Load_abd(file) per file,
+                             "# k mapped" then "leaf <hex>" per nonzero leaf
+    l_<list>_<tag>.dist      comp -l <list> [-p samples/] ... outputs
+    i_<tag>.txt              comp -i file1 file2 ... standard output
+
 Usage: python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
@@ -113,6 +122,90 @@ def core_case(d):
 
 CASES = {"toy": toy_case, "quirks": quirks_case, "config0": config0_case, "core": core_case}
 
+# -i / -l inputs: (list file, -p prefix or None, outputs as (tag, extra args))
+FILE_LISTS = {
+    "ids": (None, [("M0", ["-M", "0"]), ("rF", ["-M", "0", "-r", "F"]), ("M1", ["-M", "1"]),
+                   ("M4", ["-M", "4"])]),
+    "paths": (None, [("sim", ["-M", "0", "-d", "F"])]),
+    "prefix": ("samples/", [("M0", ["-M", "0"]), ("M5", ["-M", "5"])]),
+}
+# -i pairs: (tag, file1, file2, extra args)
+FILE_PAIRS = [
+    ("M0", "samples/S0/classification.txt", "samples/S1/classification.txt", ["-M", "0"]),
+    ("sim_rF", "samples/S2/classification.txt", "samples/R0/classification.txt", ["-M", "0", "-d", "F", "-r", "F"]),
+    ("M1", "samples/S3/classification.txt", "samples/R1/classification.txt", ["-M", "1"]),
+    ("M4", "samples/S4/classification.txt", "samples/S5/classification.txt", ["-M", "4"]),
+    ("M5", "samples/R2/classification.txt", "samples/S6/classification.txt", ["-M", "5"]),
+]
+
+
+def files_case(d):
+    """600-leaf synthetic tree; 9 count-format files S0..S8 and 3 per-read
+    files R0..R2 under samples/<id>/classification.txt. The count files carry
+    the loader's quirks: OTUs outside the tree (they count in the total), an
+    id given twice (the last count wins), zero counts, an id without a count,
+    counts "12.7" and "3e2" (read as 12 and 3), an unstripped OTU_ prefix,
+    leading blanks, CRLF line ends, and more than 193 distinct ids (the hash
+    map grows while loading, which changes the order of the FP32 total).
+    N0 has a negative count (loader check only)."""
+    rng = np.random.default_rng(4444)
+    tree = cms.synth_tree(4442, 600, 0.05, 0.99)
+    cp = cms.synth_copy_numbers(4443, 600, 15)
+    cms.write_db(d, "files", tree, cp)
+    paths = []
+
+    def put(name, text):
+        p = os.path.join(d, "samples", name)
+        os.makedirs(p, exist_ok=True)
+        with open(os.path.join(p, "classification.txt"), "w", newline="") as f:
+            f.write(text)
+        paths.append(f"samples/{name}/classification.txt")
+
+    for k in range(9):
+        n_ids = [40, 150, 260, 420, 90, 330, 500, 210, 60][k]
+        leaves = rng.choice(600, size=min(n_ids, 560), replace=False)
+        lines = ["#Database_OTU\tCount\tAbundance\tTaxonomy"]
+        for l in leaves:
+            c = int(rng.negative_binomial(0.5, 0.5 / 50.5)) + 1
+            lines.append(f"T{l}\t{c}\t0.0\tk__Bacteria; otu_T{l}")
+        for j in range(int(rng.integers(0, 80))):
+            lines.append(f"U{k}_{j}\t{int(rng.integers(1, 40))}\t0.0\tUnclassified")
+        if k % 3 == 1:
+            lines.append(f"T{leaves[0]}\t{int(rng.integers(1, 99))}\t0\tdup")      # last count wins
+            lines.append(f"T{leaves[1]}\t0\t0\tzero")
+            lines.append(f"T{leaves[2]}")                                            # no count: 0
+            lines.append(f"OTU_T{leaves[3]}\t7\t0\tprefixed")                        # not stripped
+        if k % 3 == 2:
+            lines.append(f"  T{leaves[4]}\t12.7\t0\tblank-led")
+            lines.append(f"T{leaves[5]}\t3e2\t0\texp")
+        eol = "\r\n" if k == 5 else "\n"
+        put(f"S{k}", eol.join(lines) + eol)
+    for k in range(3):
+        lines = ["#Seq_ID\tDatabase_OTU\tSimilarity"]
+        leaves = rng.choice(600, size=[30, 120, 240][k], replace=False)
+        for r in range([400, 900, 1500][k]):
+            l = leaves[min(int(rng.zipf(1.3)) - 1, len(leaves) - 1)]
+            lines.append(f"read{r}\tT{l}\t0.97")
+        put(f"R{k}", "\n".join(lines) + "\n")
+    put("N0", "#Database_OTU\tCount\nT1\t5\nT2\t-3\nT3\t9\n")
+    with open(os.path.join(d, "files_all.txt"), "w") as f:
+        f.write("\n".join(paths) + "\n")
+    with open(os.path.join(d, "list_ids.txt"), "w") as f:
+        for k in (0, 1, 2, 3, 4, 5, 6, 7, 8):
+            f.write(f"id{k}\tsamples/S{k}/classification.txt\n")
+        for k in range(3):
+            f.write(f"rd{k} samples/R{k}/classification.txt extra\n")
+    with open(os.path.join(d, "list_paths.txt"), "w") as f:
+        for k in (3, 1, 4, 1, 5):
+            f.write(f"samples/S{k}/classification.txt\n")
+        f.write("named samples/R0/classification.txt\n")
+        f.write("./samples/S8/classification.txt")                                  # no final newline
+    with open(os.path.join(d, "list_prefix.txt"), "w") as f:
+        for k in (2, 7, 0):
+            f.write(f"S{k}/classification.txt\n")
+        f.write("\n")                                                               # empty line: skipped
+        f.write("R2/classification.txt\n")
+
 
 def run_ref(d, args, out=None):
     env = dict(os.environ, ParallelMETA=d)
@@ -120,7 +213,44 @@ def run_ref(d, args, out=None):
     return r.stdout
 
 
+def make_files():
+    with tempfile.TemporaryDirectory() as tmp:
+        d = os.path.join(tmp, "input")
+        os.makedirs(d)
+        files_case(d)
+        comp = os.path.join(REF, "comp_O0")
+        drv = os.path.join(REF, "ref_driver_O0")
+        outs = {}
+        for tag, extra in [("rT", []), ("rF", ["-r", "F"])]:
+            outs[f"abd_{tag}.txt"] = run_ref(d, [drv, "-D", "G", "--files", "files_all.txt"] + extra).encode()
+        for name, (prefix, runs) in FILE_LISTS.items():
+            for tag, extra in runs:
+                o = os.path.join(tmp, f"l_{name}_{tag}.dist")
+                args = [comp, "-l", f"list_{name}.txt", "-o", o, "-D", "G"] + extra
+                if prefix:
+                    args += ["-p", prefix]
+                run_ref(d, args)
+                outs[f"l_{name}_{tag}.dist"] = open(o, "rb").read()
+        for tag, f1, f2, extra in FILE_PAIRS:
+            outs[f"i_{tag}.txt"] = run_ref(d, [comp, "-i", f1, f2, "-D", "G"] + extra).encode()
+        write_case("files", d, outs)
+
+
+def write_case(case, d, outs):
+    path = os.path.join(HERE, case + ".tar.gz")
+    with tarfile.open(path, "w:gz") as tf:
+        tf.add(d, arcname="input")
+        for name, data in outs.items():
+            ti = tarfile.TarInfo(name)
+            ti.size = len(data)
+            tf.addfile(ti, io.BytesIO(data))
+    print(f"{path}: {os.path.getsize(path)} bytes")
+
+
 def make(case):
+    if case == "files":
+        make_files()
+        return
     with tempfile.TemporaryDirectory() as tmp:
         d = os.path.join(tmp, "input")
         os.makedirs(d)
@@ -139,16 +269,9 @@ def make(case):
         outs["hex_M1.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "-M", "1", "--hex"]).encode()
         for m in (2, 3, 4, 5):  # cosine, Euclidean, JSD, Bray-Curtis (dist.h)
             outs[f"hex_M{m}.txt"] = run_ref(d, [drv, "-D", "G", "-T", "table.txt", "-M", str(m), "--hex"]).encode()
-        path = os.path.join(HERE, case + ".tar.gz")
-        with tarfile.open(path, "w:gz") as tf:
-            tf.add(d, arcname="input")
-            for name, data in outs.items():
-                ti = tarfile.TarInfo(name)
-                ti.size = len(data)
-                tf.addfile(ti, io.BytesIO(data))
-        print(f"{path}: {os.path.getsize(path)} bytes")
+        write_case(case, d, outs)
 
 
 if __name__ == "__main__":
-    for c in (sys.argv[1:] or list(CASES)):
+    for c in (sys.argv[1:] or list(CASES) + ["files"]):
         make(c)
