@@ -29,6 +29,9 @@ sys.path.insert(0, ROOT)
 
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # FP32/INT32 VALU lane-ops per second, non-packed
 HBM_PEAK_GBS = 8000.0
+# shared-minimum tiles: one v_sad_u32 per pair-entry plus one column-sum add
+# per staged row and lane (a lane holds 16 pairs)
+SM_OPS_PER_PAIR_ENTRY = 17.0 / 16.0
 METRIC = "sample-pairs/sec & pair·node updates/sec on 1 MI355X at ≤1e-5 abs err vs FP64"
 
 
@@ -109,8 +112,9 @@ def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, s
 
 def roofline(st, kernel, traffic=None):
     """Roofline entry of a shared-minimum pair kernel from its work counters:
-    VALU (algorithmic lane-ops: 1.5 per visited pair-entry, i.e. per pair of
-    a visited 32 x 32 tile-row) and HBM (algorithmic bytes: 2 x 32 x 4 B of Y
+    VALU (algorithmic lane-ops: 17/16 per visited pair-entry, i.e. per pair of
+    a visited 32 x 32 tile-row: one v_sad_u32 each plus one column-sum add per
+    staged row and lane) and HBM (algorithmic bytes: 2 x 32 x 4 B of Y
     per visited tile-row), each over the kernel time from HIP events; `bound`
     is the one closer to its peak."""
     t = st.kernel_ms * 1e-3
@@ -122,7 +126,8 @@ def roofline(st, kernel, traffic=None):
     main.update({"traffic": traffic, "kernel": kernel, "kernel_ms": st.kernel_ms,
                  "valu": {"achieved_tops": valu, "frac": fv, "ops": st.valu_ops},
                  "hbm": {"achieved_gbs": hbm, "frac": fh, "algorithmic_bytes": st.bytes_moved},
-                 "pair_entries": st.pair_node_updates, "ops_per_pair_entry": 1.5, "tune_ms": list(st.tune_ms)})
+                 "pair_entries": st.pair_node_updates, "ops_per_pair_entry": SM_OPS_PER_PAIR_ENTRY,
+                 "tune_ms": list(st.tune_ms)})
     return main
 
 
@@ -346,7 +351,7 @@ def run_serving(cms, a):
     lat = np.array(lat)
     served = len(lat)
     mean_q = float(sizes[:served].mean()) if served else 0.0
-    valu_ops = entries * 1.5
+    valu_ops = entries * SM_OPS_PER_PAIR_ENTRY
     valu_tops = valu_ops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
     hbm_gbs = entries / 1024 * 256 / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     res = {"metric": "serving: p50/p99 latency and query x reference pairs/s on 1 MI355X",
@@ -387,8 +392,8 @@ def main():
     ap.add_argument("--samples", type=int, default=None, help="override the workload's sample count")
     ap.add_argument("--walk-steps", type=int, default=1, help="timed steps of the FP32 walk drop-in (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-samples", type=int, default=256)
-    ap.add_argument("--cpu-pairs", type=int, default=32640)
+    ap.add_argument("--cpu-samples", type=int, default=800)
+    ap.add_argument("--cpu-pairs", type=int, default=319600, help="about 10 s of reference work on 16 threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (traffic-counter reference)")

@@ -155,12 +155,13 @@ int sharedmin_supported(cms_ctx *c) {
     return CMS_OK;
 }
 
-// Largest fixed-point scale K with max column sum * 2^K + rounding below 2^32.
+// Largest fixed-point scale K with max column sum * 2^K + rounding below
+// kSmFixedMax (2^31).
 int choose_scale_bits(double max_colsum, int64_t nrows) {
     if (!(max_colsum > 0)) return 24;
-    const double room = 4294967295.0 - (double)nrows * 0.5 - 64.0;
+    const double room = kSmFixedMax - (double)nrows * 0.5 - 64.0;
     int K = (int)std::floor(std::log2(room / max_colsum));
-    while (K > -60 && std::ldexp(max_colsum, K) + (double)nrows * 0.5 >= 4294967295.0) K--;
+    while (K > -60 && std::ldexp(max_colsum, K) + (double)nrows * 0.5 >= kSmFixedMax) K--;
     return std::min(K, 60);
 }
 
@@ -1041,9 +1042,9 @@ int cms_get_stats(cms_ctx *c, cms_stats *s) {
         if (use_sm(c)) {
             unsigned long long rows = 0;
             HIPCALL(c, hipMemcpy(&rows, c->d_counter, sizeof rows, hipMemcpyDeviceToHost));
-            // each visited entry is one min+add for all 32 x 32 pairs of the tile
+            // each visited entry is one pair-entry for all 32 x 32 pairs of the tile
             c->stats.pair_node_updates = (double)rows * kSmBlock * kSmBlock;
-            c->stats.valu_ops = c->stats.pair_node_updates * 1.5;
+            c->stats.valu_ops = c->stats.pair_node_updates * kSmOpsPerPairEntry;
             c->stats.bytes_moved = (double)rows * 2.0 * kSmBlock * 4.0;
         }
         c->kern_pending = false;

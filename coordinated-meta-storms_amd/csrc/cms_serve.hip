@@ -569,12 +569,12 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     s->scale_bits = ref->scale_bits;
     s->scale = std::ldexp(1.0, ref->scale_bits);
     // a query entry y(e) <= its column sum (weighted: the sample's consumed
-    // leaf mass); fixed point must stay below 2^32 as for the references
-    s->mass_limit = (4294967295.0 - 0.5 * (double)s->nrows - 64.0) / s->scale;
+    // leaf mass); fixed point must stay below kSmFixedMax as for the references
+    s->mass_limit = (kSmFixedMax - 0.5 * (double)s->nrows - 64.0) / s->scale;
     if (s->metric == CMS_METRIC_UNWEIGHTED) {
         double wmax = 0.0;
         for (const WalkNode &n : ref->tree.nodes) wmax = std::max({wmax, (double)n.w1, (double)n.w2});
-        if (wmax * s->scale + 0.5 >= 4294967295.0) {
+        if (wmax * s->scale + 0.5 >= kSmFixedMax) {
             free_server(s);
             return sfail(CMS_E_RANGE, "unweighted edge coefficient outside the references' fixed-point range");
         }

@@ -35,6 +35,7 @@ __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 
 struct DotOp {  // cosine, Euclidean: sum_I p q
     typedef double Acc;
+    static constexpr bool kColSum = false;
     __device__ __forceinline__ static void add2(Acc &acc, uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1) {
         acc = fma((double)as_f(a0), (double)as_f(b0), acc);
         acc = fma((double)as_f(a1), (double)as_f(b1), acc);
@@ -43,6 +44,7 @@ struct DotOp {  // cosine, Euclidean: sum_I p q
 
 struct MinOp {  // Bray-Curtis: sum_I min(p, q)
     typedef double Acc;
+    static constexpr bool kColSum = false;
     __device__ __forceinline__ static void add2(Acc &acc, uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1) {
         // p >= 0 stored as float: the unsigned order of the bits is the float order
         acc += (double)as_f(min(a0, b0)) + (double)as_f(min(a1, b1));
@@ -51,6 +53,7 @@ struct MinOp {  // Bray-Curtis: sum_I min(p, q)
 
 struct JsdOp {  // JSD: sum_I [(p+q)/2 phi(r) - ln2 (p + q)]
     typedef double Acc;
+    static constexpr bool kColSum = false;
     __device__ __forceinline__ static double term(uint32_t ua, uint32_t ub) {
         const float a = as_f(ua), b = as_f(ub);
         const float t = a + b;

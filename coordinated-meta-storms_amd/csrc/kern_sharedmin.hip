@@ -21,10 +21,11 @@
 //
 // Arithmetic. y is computed in FP64 and stored as fixed point
 // round(y * 2^K) in uint32 with K chosen on the host so that every sample's
-// column sum stays below 2^32 (the column sum equals the sample's total leaf
-// mass, about 100). Pair sums are then exact integer min/add (v_min_u32 +
-// v_add3_u32: 1.5 VALU ops per pair-entry), order-independent and
-// deterministic; the only error is the rounding of y (<= 2^-(K+1) per entry).
+// column sum stays below 2^31 (the column sum equals the sample's total leaf
+// mass, about 100). Pair sums are then exact integer arithmetic (the tiles
+// sum |a - b| with v_sad_u32 and recover the minima from the column sums:
+// 1 + 1/16 VALU ops per pair-entry), order-independent and deterministic;
+// the only error is the rounding of y (<= 2^-(K+1) per entry).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -70,6 +70,12 @@ hipError_t launch_walk_pairs(const WalkData &d, int64_t n_pairs, const int32_t *
 // entries in schedule order, samples padded to a multiple of the block size.
 
 constexpr int kSmBlock = 32;   // samples per block (tile = 32 x 32 pairs per wave)
+// Fixed-point ceiling of the shared-minimum embedding: every stored value and
+// column sum stays below 2^31, so the tiles' |a - b| form is exact in uint32.
+constexpr double kSmFixedMax = 2147483647.0;
+// VALU lane-ops per pair-entry of the shared-minimum tiles: one v_sad_u32,
+// plus one column-sum add per staged row and lane (16 pair-entries).
+constexpr double kSmOpsPerPairEntry = 17.0 / 16.0;
 constexpr int kSmWaves = 4;    // tiles per workgroup
 constexpr int kSmChunk = 16;   // rows staged per LDS chunk
 

@@ -62,27 +62,58 @@ struct CrossRows {
 };
 
 // Per-entry pair operation of the tiles: acc += f(a0, b0) + f(a1, b1) for
-// two staged rows. The shared minimum adds integer minima (one v_min_u32 per
-// pair-entry and one v_add3_u32 per two).
+// two staged rows. kColSum: the tile also sums each lane's own column over
+// the visited rows and hands both to finish().
+//
+// Shared minimum, direct: integer minima added (one v_min_u32 per pair-entry
+// and one v_add3_u32 per two: 1.5 VALU ops per pair-entry).
 struct MinAddOp {
     typedef uint32_t Acc;
+    static constexpr bool kColSum = false;
     __device__ __forceinline__ static void add2(Acc &acc, uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1) {
         acc = acc + min(a0, b0) + min(a1, b1);
     }
 };
+
+// Shared minimum through absolute differences: over the visited rows I,
+//   sum_I min(a, b) = (sum_I a + sum_I b - sum_I |a - b|) / 2,
+// one v_sad_u32 (|a - b| + acc) per pair-entry plus one add per staged row
+// and lane for the column sums (1 + 1/16 VALU ops per pair-entry). Exact in
+// uint32 when every value and column sum is below 2^31: then a - b does not
+// overflow as a signed difference and sum a + sum b < 2^32.
+struct SadOp {
+    typedef uint32_t Acc;
+    static constexpr bool kColSum = true;
+    __device__ __forceinline__ static uint32_t sad(uint32_t a, uint32_t b, uint32_t c) {
+        uint32_t d;
+        asm("v_sad_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+        return d;
+    }
+    __device__ __forceinline__ static void add2(Acc &acc, uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1) {
+        acc = sad(a1, b1, sad(a0, b0, acc));
+    }
+    // acc = sum |a - b| -> sum min(a, b), given both columns' sums
+    __device__ __forceinline__ static uint32_t finish(uint32_t acc, uint32_t sa, uint32_t sb) {
+        return (sa + sb - acc) >> 1;
+    }
+};
+
+// The shared-minimum kernels' pair operation.
+typedef SadOp SmOp;
 
 // One wave = 32 x 32 pairs, lane (ty, tx) = 4 rows x 4 columns.
 //
 // Visit the entries set in both blocks' bitmaps (OR of two blocks when a
 // side is not block-aligned): per 64 bitmap words, compact the set bits into
 // an LDS list, stage their 2 x 32 Y values in LDS CHUNK rows at a time
-// (through `rows`, see the loaders above), and accumulate min/add into the
-// lane's 4 x 4 registers. Returns the number of entries visited.
+// (through `rows`, see the loaders above), and accumulate Op into the lane's
+// 4 x 4 registers (for the shared-minimum ops: the 4 x 4 shared sums).
+// Returns the number of entries visited.
 //
 // Only bitmap words [w_begin, w_end) are visited (w_begin a multiple of 64):
 // a tile's rows can be split over several waves whose integer partial sums
-// add up exactly.
-template <int CHUNK, class Rows, class Op = MinAddOp>
+// add up exactly (each part's sums are shared minima on their own).
+template <int CHUNK, class Rows, class Op = SmOp>
 __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const uint32_t *A1, const uint32_t *B0,
                                                        const uint32_t *B1, int64_t w_begin, int64_t w_end,
                                                        const Rows &rows, uint16_t *L, uint32_t *S, int lane,
@@ -94,6 +125,7 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0;
     uint32_t visited = 0;
+    uint32_t cs = 0;  // kColSum: this lane's column summed over the visited rows
     for (int64_t wb = w_begin; wb < w_end; wb += 64) {
         const int64_t wi = wb + lane;
         uint32_t w = 0u;
@@ -125,13 +157,13 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         Rows win = rows;
         win.window(wb * 32);
-        for (int j = 0; j < tot; j += CHUNK) {
-            // row ids are wave-uniform (every lane reads the same LDS word);
-            // slots past the list end load the chunk's first row (a valid
-            // address) and are never accumulated
+        // Stage CHUNK listed rows into registers. Row ids are wave-uniform
+        // (every lane reads the same LDS word); slots past the list end load
+        // the chunk's first row (a valid address) and are never accumulated.
+        uint32_t v[CHUNK];
+        auto load_chunk = [&](int j) {
             const uint32_t first =
                 (uint32_t)__builtin_amdgcn_readfirstlane((int)*reinterpret_cast<const uint32_t *>(L + j)) & 0xffffu;
-            uint32_t v[CHUNK];
 #pragma unroll
             for (int k = 0; k < CHUNK; k += 2) {
                 const uint32_t pr =
@@ -141,14 +173,28 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
                 v[k] = win.load(r0);
                 v[k + 1] = win.load(r1);
             }
+        };
+        load_chunk(0);
+        for (int j = 0; j < tot; j += CHUNK) {
+            const int kn = min(CHUNK, tot - j);
+            if constexpr (Op::kColSum) {
+                if (kn == CHUNK) {
+#pragma unroll
+                    for (int k = 0; k < CHUNK; k++) cs += v[k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < CHUNK; k++) cs += (k < kn) ? v[k] : 0u;
+                }
+            }
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int k = 0; k < CHUNK; k++) S[k * 64 + lane] = v[k];
-            const int kn = min(CHUNK, tot - j);
             if (kn & 1) S[kn * 64 + lane] = 0u;  // rows are accumulated in pairs
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // the next chunk's loads are in flight while this one accumulates
+            if (j + CHUNK < tot) load_chunk(j + CHUNK);
             for (int k = 0; k < kn; k += 2) {
                 const uint4 a0 = *reinterpret_cast<const uint4 *>(S + k * 64 + ty * 4);
                 const uint4 b0 = *reinterpret_cast<const uint4 *>(S + k * 64 + 32 + tx * 4);
@@ -165,6 +211,19 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }
+    }
+    if constexpr (Op::kColSum) {
+        // lanes 0..31 hold the row block's columns, 32..63 the column block's
+        uint32_t sa[4], sb[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            sa[i] = (uint32_t)__shfl((int)cs, ty * 4 + i, 64);
+            sb[i] = (uint32_t)__shfl((int)cs, 32 + tx * 4 + i, 64);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = Op::finish(acc[i][j], sa[i], sb[j]);
     }
     return visited;
 }
