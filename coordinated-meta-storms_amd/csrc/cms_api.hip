@@ -652,7 +652,7 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
         HIPCALL(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         c->grid_waves = std::max(cus, 1) * 32;  // 8 waves per SIMD
     }
-    if (c->opts.variant >= 1 && c->opts.variant <= kSmVariantsAll) {
+    if (c->opts.variant >= 1 && c->opts.variant <= kSmVariants) {
         c->variant = c->opts.variant - 1;
         return CMS_OK;
     }

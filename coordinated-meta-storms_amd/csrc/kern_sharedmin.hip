@@ -374,69 +374,6 @@ __global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int
     if (d.counter && lane == 0 && visited) atomicAdd(d.counter, (unsigned long long)visited);
 }
 
-// All pairs over quads: a workgroup of 4 waves takes four consecutive tiles
-// of one block row, (ra, cb .. cb + 3), from a global quad counter, and the
-// waves walk their bitmap windows in step (one barrier per window), so the
-// row block's staged rows are read by the four waves close together in time
-// and come from L2 after the first. Rows are split into ceil((nb - ra) / 4)
-// quads; a wave past the row end takes part in the barriers only. Writes the
-// packed upper triangle in the caller's sample order (as OUT 0 above).
-template <int CHUNK>
-__global__ __launch_bounds__(256) void sm_quad_kernel(SmData d, int64_t nb, int64_t nquads,
-                                                      unsigned long long *__restrict__ work,
-                                                      float *__restrict__ out) {
-    __shared__ uint16_t s_list[4][64 * 32];
-    __shared__ uint32_t s_stage[4][CHUNK * 64];
-    __shared__ unsigned long long s_q;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ty = lane >> 3, tx = lane & 7;
-    uint32_t visited = 0;
-    for (;;) {
-        if (threadIdx.x == 0) s_q = atomicAdd(work, 1ull);
-        __syncthreads();
-        const unsigned long long q = s_q;
-        __syncthreads();
-        if (q >= (unsigned long long)nquads) break;  // the same for the whole workgroup
-        int64_t ra = 0, rem = (int64_t)q;
-        for (;;) {
-            const int64_t nq = (nb - ra + 3) >> 2;
-            if (rem < nq) break;
-            rem -= nq;
-            ra++;
-        }
-        const int64_t cb = ra + rem * 4 + wave;
-        const bool active = cb < nb;
-        const int64_t cbx = active ? cb : ra;
-        const int64_t abase = ra * kSmBlock, bbase = cbx * kSmBlock;
-        SquareRows<false> rows;
-        rows.Y = d.Y;
-        rows.stride = d.npad;
-        rows.ok = true;
-        rows.col4 = (uint32_t)((lane < 32) ? abase + lane : bbase + (lane - 32)) * 4u;
-        uint32_t acc[4][4];
-        visited += sm_tile_accumulate<CHUNK, SquareRows<false>, SmOp, true>(
-            d.bm + ra * d.nwords, nullptr, d.bm + cbx * d.nwords, nullptr, 0, d.nwords, rows, s_list[wave],
-            s_stage[wave], lane, acc, active);
-        if (!active) continue;
-#pragma unroll
-        for (int ii = 0; ii < 4; ii++)
-#pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
-                const int64_t pa = abase + ty * 4 + ii, pb = bbase + tx * 4 + jj;
-                if (pa >= pb || pb >= d.n) continue;
-                const float sim = sm_finish(d, acc[ii][jj], pa, pb);
-                int64_t a = d.perm ? d.perm[pa] : pa, b = d.perm ? d.perm[pb] : pb;
-                if (a > b) {
-                    const int64_t x = a;
-                    a = b;
-                    b = x;
-                }
-                out[a * (int64_t)d.n + b - (a + 1) * (a + 2) / 2] = sim;
-            }
-    }
-    if (d.counter && lane == 0 && visited) atomicAdd(d.counter, (unsigned long long)visited);
-}
-
 // Explicit pair list (caller's sample ids; d.pos maps them to stored
 // positions when samples are binned): lane = pair; 32-entry groups skipped
 // when no lane's two samples share mass in them.
@@ -571,20 +508,6 @@ hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_
         host_decode_upper(t_begin + t_count - 1, nb, rb1, cb1);
         const int64_t lo = row_start(std::min(rb0 * kSmBlock, n)), hi = row_start(std::min((rb1 + 1) * kSmBlock, n));
         if (lo < out_base || hi - out_base > out_cap) return hipErrorInvalidValue;
-    }
-    if (variant >= kSmVariants) {
-        // quad kernels: the whole triangle in the caller's order only
-        if (out_mode != 0 || t_begin != 0 || t_count != nb * (nb + 1) / 2) variant = 1;
-        else {
-            int64_t nquads = 0;
-            for (int64_t r = 0; r < nb; r++) nquads += (nb - r + 3) / 4;
-            const int64_t wg = std::max<int64_t>(1, std::min<int64_t>(nquads, grid_waves / 4));
-            if (variant == kSmVariants)
-                sm_quad_kernel<16><<<dim3((unsigned)wg), 256, 0, st>>>(d, nb, nquads, work, out);
-            else
-                sm_quad_kernel<32><<<dim3((unsigned)wg), 256, 0, st>>>(d, nb, nquads, work, out);
-            return hipGetLastError();
-        }
     }
     return out_mode == 0 ? launch_persistent_t<0>(d, variant, nb, t_begin, t_count, work, out, out_base, grid_waves, st)
                          : launch_persistent_t<1>(d, variant, nb, t_begin, t_count, work, out, out_base, grid_waves, st);

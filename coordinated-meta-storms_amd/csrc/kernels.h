@@ -101,8 +101,7 @@ struct SmData {
 // minus out_base. `work` is a zeroed device counter. out holds out_cap
 // floats; a launch that could write outside them is refused on the host
 // (hipErrorInvalidValue) before anything is queued.
-constexpr int kSmVariants = 6;     // persistent-kernel instances the autotuner times
-constexpr int kSmVariantsAll = 8;  // + the quad kernels (CHUNK 16, 32), whole triangle only
+constexpr int kSmVariants = 6;
 hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
                                 unsigned long long *work, float *out, int64_t out_cap, int out_mode,
                                 int64_t out_base, int grid_waves, hipStream_t st);

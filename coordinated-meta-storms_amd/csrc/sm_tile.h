@@ -113,16 +113,11 @@ typedef SadOp SmOp;
 // Only bitmap words [w_begin, w_end) are visited (w_begin a multiple of 64):
 // a tile's rows can be split over several waves whose integer partial sums
 // add up exactly (each part's sums are shared minima on their own).
-//
-// SYNC: every wave of the workgroup calls this with the same word range and
-// they pass one workgroup barrier per 64-word window, so waves that share a
-// block read its rows close together in time; an inactive wave (no tile)
-// only takes part in the barriers.
-template <int CHUNK, class Rows, class Op = SmOp, bool SYNC = false>
+template <int CHUNK, class Rows, class Op = SmOp>
 __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const uint32_t *A1, const uint32_t *B0,
                                                        const uint32_t *B1, int64_t w_begin, int64_t w_end,
                                                        const Rows &rows, uint16_t *L, uint32_t *S, int lane,
-                                                       typename Op::Acc (&acc)[4][4], bool active = true) {
+                                                       typename Op::Acc (&acc)[4][4]) {
     static_assert(CHUNK % 2 == 0, "rows are staged in pairs");
     const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
@@ -132,10 +127,9 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
     uint32_t visited = 0;
     uint32_t cs = 0;  // kColSum: this lane's column summed over the visited rows
     for (int64_t wb = w_begin; wb < w_end; wb += 64) {
-        if constexpr (SYNC) __builtin_amdgcn_s_barrier();
         const int64_t wi = wb + lane;
         uint32_t w = 0u;
-        if (active && wi < w_end) {
+        if (wi < w_end) {
             uint32_t a = A0[wi], b = B0[wi];
             if (A1) a |= A1[wi];
             if (B1) b |= B1[wi];

@@ -30,7 +30,7 @@ def test_binned_and_variants_identical(cms, data, metric):
     print(f"binned variants metric={metric}", flush=True)
     tree, abd = data
     ref = _plain(cms, tree, abd, metric)
-    for variant in range(1, 9):  # 7, 8: quad kernels (waves of a workgroup in step)
+    for variant in range(1, 7):
         with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric, bin=1, variant=variant) as ctx:
             ctx.set_tree(tree)
             ctx.set_samples_csr(abd)
