@@ -195,7 +195,8 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             // the next chunk's loads are in flight while this one accumulates
             if (j + CHUNK < tot) load_chunk(j + CHUNK);
-            for (int k = 0; k < kn; k += 2) {
+            // two staged rows: 4 x 4 pair-entries each
+            auto rows2 = [&](int k) {
                 const uint4 a0 = *reinterpret_cast<const uint4 *>(S + k * 64 + ty * 4);
                 const uint4 b0 = *reinterpret_cast<const uint4 *>(S + k * 64 + 32 + tx * 4);
                 const uint4 a1 = *reinterpret_cast<const uint4 *>(S + (k + 1) * 64 + ty * 4);
@@ -207,6 +208,15 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
 #pragma unroll
                     for (int jj = 0; jj < 4; jj++)
                         Op::add2(acc[ii][jj], av0[ii], bv0[jj], av1[ii], bv1[jj]);
+            };
+            if (Op::kColSum && kn == CHUNK) {
+                // full chunk of the integer tiles: unrolled, LDS offsets are
+                // immediates and the reads can be issued ahead of the
+                // arithmetic (the FP64 leaf ops keep the loop)
+#pragma unroll
+                for (int k = 0; k < CHUNK; k += 2) rows2(k);
+            } else {
+                for (int k = 0; k < kn; k += 2) rows2(k);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
