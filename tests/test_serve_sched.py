@@ -15,7 +15,10 @@ CSRC = os.path.join(ROOT, "coordinated-meta-storms_amd", "csrc")
 @pytest.fixture(scope="module")
 def harness(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("sched") / "sched_harness")
-    subprocess.run(["g++", "-O1", "-std=c++17", "-I", CSRC, os.path.join(ROOT, "tests", "sched_harness.cpp"),
+    # host code only: built with AddressSanitizer / UBSan so that an
+    # out-of-range slot or arena block fails the run
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-fno-omit-frame-pointer", "-I", CSRC, os.path.join(ROOT, "tests", "sched_harness.cpp"),
                     os.path.join(CSRC, "serve_sched.cpp"), "-o", exe], check=True)
     return exe
 
