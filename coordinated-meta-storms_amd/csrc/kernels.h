@@ -12,7 +12,7 @@ namespace cms {
 // Samples per walk block: a wave walks an 8 x 8 tile of pairs (lane = 8 r + c).
 constexpr int kWalkBlock = 8;
 // Leaf-stream positions staged in LDS per chunk.
-constexpr int kWalkLeafChunk = 32;
+constexpr int kWalkLeafChunk = 16;
 // Waves (tiles) per walk workgroup.
 constexpr int kWalkWaves = 4;
 
