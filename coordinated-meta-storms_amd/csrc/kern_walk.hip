@@ -66,7 +66,11 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nwg) {
 
 // MODE 0: all pairs (upper-triangle tiles, packed output).
 // MODE 1: rectangle r0.. x c0.. (row-major output, diagonal = 1).
-template <int STORE, int MODE>
+// VREG: the live register slots (at most kWalkVregSlots) are VGPR arrays
+// indexed by the wave-uniform slot number (s_set_gpr_idx: one v_mov per
+// access, no LDS); otherwise they sit in LDS as two planes (sample a,
+// sample b), one dword per lane at slot * 64 + lane (bank-conflict free).
+template <int STORE, int MODE, bool VREG>
 __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, int64_t ntiles, int64_t nb,
                                                                      int r0, int nr, int c0, int nc,
                                                                      float *__restrict__ out) {
@@ -76,12 +80,34 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
     const int waves_per_wg = (int)(blockDim.x >> 6);
     const int64_t tile = xcd_remap(blockIdx.x, gridDim.x) * waves_per_wg + wave;
     if (tile >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
-    const int per_wave = 16 * kWalkLeafChunk + 2 * 64 * d.nslots;
+    const int per_wave = 16 * kWalkLeafChunk + (VREG ? 0 : 2 * 64 * d.nslots);
     float *stage = lds + (size_t)wave * per_wave;
-    // register slots as two planes (sample a, sample b): stride-4 dword access
-    // per lane, bank-conflict free
     float *slotA = stage + 16 * kWalkLeafChunk;
     float *slotB = slotA + 64 * d.nslots;
+    float vA[VREG ? kWalkVregSlots : 1], vB[VREG ? kWalkVregSlots : 1];
+    if constexpr (VREG) {
+#pragma unroll
+        for (int k = 0; k < kWalkVregSlots; k++) vA[k] = vB[k] = 0.0f;
+    }
+    constexpr uint32_t kSlotMask = VREG ? kWalkVregSlots - 1 : 0xffu;
+    auto slot_read = [&](uint32_t sl, float &a, float &b) {
+        if constexpr (VREG) {
+            a = vA[sl & kSlotMask];
+            b = vB[sl & kSlotMask];
+        } else {
+            a = slotA[sl * 64 + lane];
+            b = slotB[sl * 64 + lane];
+        }
+    };
+    auto slot_write = [&](uint32_t sl, float a, float b) {
+        if constexpr (VREG) {
+            vA[sl & kSlotMask] = a;
+            vB[sl & kSlotMask] = b;
+        } else {
+            slotA[sl * 64 + lane] = a;
+            slotB[sl * 64 + lane] = b;
+        }
+    };
 
     int64_t rowbase, colbase;
     int64_t rowlim, collim;
@@ -151,15 +177,13 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
                 b1 = stage[lp * 16 + 8 + c];
                 lp++;
             } else {
-                a1 = slotA[(code & 0xffu) * 64 + lane];
-                b1 = slotB[(code & 0xffu) * 64 + lane];
+                slot_read(code & 0xffu, a1, b1);
             }
             if (code & kC2Leaf) {
                 a2 = stage[lp * 16 + r];
                 b2 = stage[lp * 16 + 8 + c];
             } else {
-                a2 = slotA[((code >> 8) & 0xffu) * 64 + lane];
-                b2 = slotB[((code >> 8) & 0xffu) * 64 + lane];
+                slot_read((code >> 8) & 0xffu, a2, b2);
             }
             const float m1 = fminf(a1, b1), m2 = fminf(a2, b2);
             if constexpr (KAHAN) {
@@ -177,8 +201,7 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
             if (code & kDstLive) {
                 float ra = (a1 - m1) * w1 + (a2 - m2) * w2;
                 float rb_ = (b1 - m1) * w1 + (b2 - m2) * w2;
-                slotA[((code >> 16) & 0xffu) * 64 + lane] = ra;
-                slotB[((code >> 16) & 0xffu) * 64 + lane] = rb_;
+                slot_write((code >> 16) & 0xffu, ra, rb_);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -332,8 +355,10 @@ __global__ void walk_scatter_kernel(const int64_t *__restrict__ rowptr, const in
 
 // ------------------------------------------------------------------ launchers
 
+static inline bool walk_vreg(int nslots) { return nslots <= kWalkVregSlots; }
+
 static inline size_t walk_lds_per_wave(int nslots) {
-    return (size_t)(16 * kWalkLeafChunk + 2 * 64 * nslots) * sizeof(float);
+    return (size_t)(16 * kWalkLeafChunk + (walk_vreg(nslots) ? 0 : 2 * 64 * nslots)) * sizeof(float);
 }
 
 hipError_t launch_sample_scale(const int64_t *rowptr, const float *val, int n, int storage, float *scale,
@@ -372,17 +397,24 @@ static hipError_t launch_tiles(const WalkData &d, int64_t ntiles, int64_t nb, in
     const int64_t nwg = (ntiles + waves - 1) / waves;
     const size_t lds = walk_lds_per_wave(d.nslots) * waves;
     const dim3 grid((unsigned)nwg), block(64 * waves);
-    switch (d.storage) {
-    case CMS_STORE_F32:
-        walk_tile_kernel<CMS_STORE_F32, MODE><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
-        break;
-    case CMS_STORE_BF16:
-        walk_tile_kernel<CMS_STORE_BF16, MODE><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
-        break;
-    default:
-        walk_tile_kernel<CMS_STORE_F16, MODE><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);
-        break;
+#define CMS_WALK_TILES(VR)                                                                                  \
+    switch (d.storage) {                                                                                    \
+    case CMS_STORE_F32:                                                                                     \
+        walk_tile_kernel<CMS_STORE_F32, MODE, VR><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);  \
+        break;                                                                                              \
+    case CMS_STORE_BF16:                                                                                    \
+        walk_tile_kernel<CMS_STORE_BF16, MODE, VR><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out); \
+        break;                                                                                              \
+    default:                                                                                                \
+        walk_tile_kernel<CMS_STORE_F16, MODE, VR><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);  \
+        break;                                                                                              \
     }
+    if (walk_vreg(d.nslots)) {
+        CMS_WALK_TILES(true)
+    } else {
+        CMS_WALK_TILES(false)
+    }
+#undef CMS_WALK_TILES
     return hipGetLastError();
 }
 

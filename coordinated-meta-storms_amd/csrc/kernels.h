@@ -15,6 +15,9 @@ constexpr int kWalkBlock = 8;
 constexpr int kWalkLeafChunk = 16;
 // Waves (tiles) per walk workgroup.
 constexpr int kWalkWaves = 4;
+// Live register slots the walk tiles keep in VGPRs (a power of two; larger
+// schedules keep them in LDS).
+constexpr int kWalkVregSlots = 16;
 
 // Layout of walk abundances: for sample block b (samples 8b..8b+7) and leaf
 // stream position lp, value of sample 8b+k at ab[(b * nlp + lp) * 8 + k].

@@ -98,6 +98,46 @@ def test_walk_sizes_and_tiles(cms, orc, ctx, n):
         np.testing.assert_array_equal(t.view(np.uint32), ref.view(np.uint32))
 
 
+def _many_live_tree(cms, cherries):
+    """order.txt that keeps `cherries` registers live: every leaf pair is
+    joined into its own register first, then the registers are folded into
+    one (the walk's slots then exceed what it keeps in VGPRs)."""
+    L = 2 * cherries
+    o1, d1, o2, d2, od = [], [], [], [], []
+    for i in range(cherries):
+        o1.append(2 * i); d1.append(0.1 + 0.01 * i); o2.append(2 * i + 1); d2.append(0.2); od.append(-(i + 1))
+    for i in range(1, cherries):
+        o1.append(-1); d1.append(0.05); o2.append(-(i + 1)); d2.append(0.3 + 0.005 * i); od.append(-1)
+    arr = lambda v, t: np.array(v, t)
+    return cms.Tree(L, arr(o1, np.int32), arr(d1, np.float32), arr(o2, np.int32), arr(d2, np.float32),
+                    arr(od, np.int32))
+
+
+@pytest.mark.parametrize("cherries", [12, 40])
+def test_walk_register_slots_vgpr_and_lds(cms, orc, ctx, cherries):
+    """Schedules with up to 16 live registers run with the slots in VGPRs,
+    larger ones with the slots in LDS; both bit-exact with the reference
+    walk."""
+    _flush(f"walk live registers {cherries}")
+    tree = _many_live_tree(cms, cherries)
+    rng = np.random.default_rng(cherries)
+    abd = (rng.random((37, tree.leaf_n)) * (rng.random((37, tree.leaf_n)) < 0.5)).astype(np.float32)
+    abd = (abd / np.maximum(abd.sum(1, keepdims=True), 1e-30) * 100.0).astype(np.float32)
+    ctx.set_opts(cms.KERNEL_WALK, cms.STORE_F32)
+    ctx.set_tree(tree)
+    ctx.set_samples(abd)
+    assert ctx.stats().nslots == cherries
+    got = ctx.all_pairs()
+    exp = orc.all_pairs_f32(tree, abd)
+    np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+    t = ctx.tile(3, 20, 0, 37)
+    rows, cols = np.meshgrid(np.arange(3, 23), np.arange(37), indexing="ij")
+    ref = np.ones((20, 37), np.float32)
+    m = rows != cols
+    ref[m] = exp[cms.packed_index(np.minimum(rows, cols)[m], np.maximum(rows, cols)[m], 37)]
+    np.testing.assert_array_equal(t.view(np.uint32), ref.view(np.uint32))
+
+
 def test_walk_edge_samples(cms, orc, ctx):
     """All-zero samples (sim 0), identical samples (sim ~1), disjoint samples."""
     _flush("walk edge samples")
