@@ -77,6 +77,15 @@ def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, s
                        "sample": f"first {j['pairs']} pairs of the upper triangle of the first {sample_n} samples "
                                  f"(99,322-leaf tree); reference _Comp_Tree::Calc_sim built -O3 -ffp-contract=off, "
                                  f"OpenMP dynamic,1 on {threads} threads, wall clock {j['seconds']:.2f} s"}
+                # the reference as shipped (its Makefile builds -O0), on a tenth of the pairs
+                drv0 = orc.ref_binary("ref_driver_O0")
+                if drv0:
+                    r0 = subprocess.run([drv0, "-D", "G", "-T", table, "--time", str(max(pairs // 10, 1)),
+                                         str(threads)], env=env, capture_output=True, text=True, timeout=600)
+                    if r0.returncode == 0:
+                        j0 = json.loads(r0.stdout.strip().splitlines()[-1])
+                        res["as_shipped_O0"] = {"value": j0["pairs_per_sec"], "pairs": j0["pairs"],
+                                                "seconds": j0["seconds"]}
             else:
                 log(f"reference driver failed: {r.stderr[-500:]}")
     if not res:
