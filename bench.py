@@ -424,6 +424,8 @@ def main():
     ap.add_argument("--band-pairs", type=int, default=1 << 28, help="pairs per streamed band (workload 3)")
     ap.add_argument("--graph", type=int, default=1, help="replay streamed bands from a hipGraph (workload 3)")
     ap.add_argument("--error-pairs", type=int, default=1000000, help="random pairs checked vs FP64 (workload 2)")
+    ap.add_argument("--metrics", type=str, default="",
+                    help="workload 1: also time these comp -M metrics (e.g. 1,2,3,4,5) on the same samples")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every GPU box here has one, and multi-GPU runs are not part of "
@@ -458,6 +460,17 @@ def main():
     log(f"sharedmin: {sec * 1e3:.2f} ms/step, pair kernel {st.kernel_ms:.2f} ms, prep {st.prep_ms:.2f} ms, "
         f"visited pair-entries {st.pair_node_updates:.3e}")
     sm.close()
+
+    others = {}
+    for m in [int(x) for x in a.metrics.split(",") if x.strip()]:
+        with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=m, bin=a.bin, autotune=a.autotune) as oc:
+            oc.set_tree(tree)
+            oc.set_samples_csr(abd)
+            osec = timed_steps(oc, 1, 1)
+            ost = oc.stats()
+        others[f"M{m}"] = {"ms_per_step": osec * 1e3, "pairs_per_s": npairs / osec, "kernel_ms": ost.kernel_ms,
+                           "prep_ms": ost.prep_ms}
+        log(f"-M {m}: {osec * 1e3:.2f} ms/step, kernel {ost.kernel_ms:.2f} ms, prep {ost.prep_ms:.2f} ms")
 
     walk = None
     gpu_walk = None
@@ -498,6 +511,8 @@ def main():
         "prep_ms": st.prep_ms,
         "plain_fp32_walk": walk,
     }
+    if others:
+        res["other_metrics"] = others
     if walk:
         res["speedup_vs_plain_fp32_walk"] = walk["ms_per_step"] / (sec * 1e3)
     if a.copy_calib > 0:

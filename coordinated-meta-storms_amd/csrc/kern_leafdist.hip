@@ -11,11 +11,12 @@
 //   JSD         1 - sqrt(S / 2), S = sum over leaves of p log(2p/(p+q)) +
 //               q log(2q/(p+q)) = ln2 (Pa + Pb - sum_I (p + q))
 //               + sum_I (p+q)/2 phi(r), r = (p-q)/(p+q),
-//               phi(r) = (1+r) log1p(r) + (1-r) log1p(-r)
+//               phi(r) = (1+r) ln(1+r) + (1-r) ln(1-r)
 // (a leaf with mass on one side only contributes p ln 2). The I-sums use the
 // tiles of the shared-minimum kernels over leaf rows (bitmap of leaves with
-// mass per 32-sample block) with FP64 accumulators; phi(r) keeps the JSD free
-// of cancellation for near-identical samples (phi(0) = 0 exactly).
+// mass per 32-sample block) with FP64 accumulators; phi(r), by its series
+// for small |r|, keeps the JSD free of cancellation for near-identical
+// samples (phi(0) = 0 exactly).
 // Degenerate samples follow the reference: an all-zero sample gives 0 for
 // cosine, Euclidean and Bray-Curtis; JSD is 1 for two all-zero samples and
 // NaN (bits 0xffc00000, as 0/0 gives on the host) when exactly one is.
@@ -54,15 +55,28 @@ struct MinOp {  // Bray-Curtis: sum_I min(p, q)
 struct JsdOp {  // JSD: sum_I [(p+q)/2 phi(r) - ln2 (p + q)]
     typedef double Acc;
     static constexpr bool kColSum = false;
+    // phi(r) = (1+r) ln(1+r) + (1-r) ln(1-r) for p, q > 0, r = (p-q)/(p+q):
+    //   |r| <= 1/4: sum_k u^k / (k (2k-1)), u = r^2, six terms (the next is
+    //               below 7e-10 relative): no cancellation for near-equal p, q
+    //   |r| >  1/4: x ln x + y ln y with x = 2p/(p+q), y = 2q/(p+q) (native
+    //               log2; x ln x -> 0 below 1e-30)
+    // The ln2 (p + q) part and the sum stay in FP64 (p + q exact there).
     __device__ __forceinline__ static double term(uint32_t ua, uint32_t ub) {
         const float a = as_f(ua), b = as_f(ub);
         const float t = a + b;
-        const float r = (a - b) / t;
-        // r rounds to +-1 when one side is below 2^-24 of the other: the
-        // vanishing side's term is x log x -> 0, not 0 * -inf
-        const float phi = ((1.0f + r) > 0.0f ? (1.0f + r) * log1pf(r) : 0.0f) +
-                          ((1.0f - r) > 0.0f ? (1.0f - r) * log1pf(-r) : 0.0f);
-        const double g = 0.5 * (double)t * (double)phi - 0.69314718055994530942 * ((double)a + (double)b);
+        const float rt = __builtin_amdgcn_rcpf(t);
+        const float r = (a - b) * rt;
+        const float u = r * r;
+        float ps = fmaf(u, 1.0f / 66.0f, 1.0f / 45.0f);
+        ps = fmaf(u, ps, 1.0f / 28.0f);
+        ps = fmaf(u, ps, 1.0f / 15.0f);
+        ps = fmaf(u, ps, 1.0f / 6.0f);
+        ps = fmaf(u, ps, 1.0f);
+        const float x = 2.0f * a * rt, y = 2.0f * b * rt;
+        const float xl = x > 1e-30f ? x * __log2f(x) : 0.0f;
+        const float yl = y > 1e-30f ? y * __log2f(y) : 0.0f;
+        const float phi = u <= 0.0625f ? u * ps : 0.69314718f * (xl + yl);
+        const double g = ((double)a + (double)b) * (0.5 * (double)phi - 0.69314718055994530942);
         return (ua != 0u && ub != 0u) ? g : 0.0;
     }
     __device__ __forceinline__ static void add2(Acc &acc, uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1) {
