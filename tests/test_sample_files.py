@@ -248,3 +248,24 @@ def test_loader_errors(cms, files, tmp_path):
     p.write_text("#Database_OTU\tCount\nT1\t0\nT2\t0\n")
     a, m = cms.sample_file_abundance(db, str(p))
     assert m == 2 and not a.any()
+
+
+def test_cuda_comp_input_errors(cms, files, tmp_path):
+    """Errors that end cuda-comp before it touches the GPU: exit status 1 and
+    a message (the reference prints and goes on with empty data)."""
+    exe = os.path.join(cms.BIN_DIR, "cuda-comp")
+    env = dict(os.environ, ParallelMETA=files.root)
+
+    def run(args):
+        return subprocess.run([exe] + args, cwd=files.root, env=env, capture_output=True, text=True, timeout=60)
+
+    r = run(["-i", "samples/S0/classification.txt"])          # -i needs two files
+    assert r.returncode == 1 and "missing value" in r.stderr
+    r = run(["-l", "no_such.list", "-o", str(tmp_path / "o.dist")])
+    assert r.returncode == 1 and "Cannot open file" in r.stderr
+    r = run(["-i", "samples/S0/classification.txt", "samples/none/classification.txt"])
+    assert r.returncode == 1 and "Cannot open input file" in r.stderr
+    r = run(["-l", "list_ids.txt"])                            # no -o
+    assert r.returncode == 1 and "Cannot open output file" in r.stderr
+    r = run(["-D", "G"])                                       # no input at all
+    assert r.returncode == 1 and "an input is required" in r.stderr
