@@ -639,13 +639,24 @@ int cms_set_samples(cms_ctx *c, int n, const float *abd) {
 namespace {
 
 // Persistent-kernel variant: forced by opts.variant, else the default, or the
-// fastest of all variants timed on a sample of tiles spread over the range
-// when opts.autotune is set. Tuning writes into `out` (later overwritten) with
-// out_mode as in launch_sm_persistent and out_base 0: with out_mode 1 the
-// caller passes in `ntiles` only tiles whose packed stored-position indices
-// fit `out`.
+// fastest of all variants when opts.autotune is set. Tiles [0, ntiles) are
+// the range the caller is about to compute into `out` (out_mode as in
+// launch_sm_persistent, out_base 0; with out_mode 1 the caller passes only
+// tiles whose packed stored-position indices fit `out`).
+//
+// With at least 12 segments of 8,192 tiles (about one per resident wave, so
+// each launch runs at full occupancy) the tuning launches are real work:
+// every variant computes two consecutive segments from the start of the
+// range, in forward then reverse variant order (binned tiles get sparser
+// along the range, and the two passes cancel that to first order), timed and
+// normalised by the rows they visited. The results stay in `out` and
+// *done = the number of leading tiles already computed; they are counted in
+// c->d_counter, which the caller zeroes before calling. Smaller ranges are
+// timed on three segments at the start, middle and end (results discarded,
+// *done = 0).
 int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out_cap, int out_mode,
-                 hipStream_t st) {
+                 hipStream_t st, int64_t *done) {
+    *done = 0;
     if (!c->d_work) HIPCALL(c, dalloc(c->d_work, 1));
     if (c->grid_waves == 0) {
         int cus = 0;
@@ -661,31 +672,64 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
         c->variant = 1;  // 16-entry LDS chunk, 4 waves per workgroup
         return CMS_OK;
     }
-    // three segments of 8,192 tiles (about one per resident wave, so each
-    // launch runs at full occupancy) at the start, middle and end of the
-    // tile range: with binned samples the first tiles are the densest, the
-    // last the sparsest. One timing per variant: the instances differ by a
-    // few per cent, and the tuning work is not part of the result.
-    const int nseg = ntiles >= 3 * 8192 ? 3 : 1;
-    const int64_t seg_len = std::min<int64_t>(ntiles, 8192);
-    float best = 1e30f;
-    int best_v = 1;
-    for (int v = 0; v < kSmVariants; v++) {
+    constexpr int64_t kSeg = 8192;
+    auto timed = [&](int v, int64_t t0, int64_t len, float &ms) -> int {
         HIPCALL(c, hipEventRecord(c->evp0, st));
-        for (int k = 0; k < nseg; k++) {
-            const int64_t t0 = nseg == 1 ? 0 : (ntiles - seg_len) * k / (nseg - 1);
-            HIPCALL(c, launch_zero_counter(c->d_work, st));
-            HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, t0, seg_len, c->d_work, out, out_cap, out_mode, 0,
-                                            c->grid_waves, st));
-        }
+        HIPCALL(c, launch_zero_counter(c->d_work, st));
+        HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, t0, len, c->d_work, out, out_cap, out_mode, 0,
+                                        c->grid_waves, st));
         HIPCALL(c, hipEventRecord(c->evp1, st));
         HIPCALL(c, hipEventSynchronize(c->evp1));
-        float ms_min = 0;
-        HIPCALL(c, hipEventElapsedTime(&ms_min, c->evp0, c->evp1));
-        c->stats.tune_ms[v] = ms_min;
-        if (ms_min < best) {
-            best = ms_min;
-            best_v = v;
+        HIPCALL(c, hipEventElapsedTime(&ms, c->evp0, c->evp1));
+        return CMS_OK;
+    };
+    float best = 1e30f;
+    int best_v = 1;
+    if (ntiles >= 2 * kSmVariants * kSeg) {
+        std::vector<double> cost(kSmVariants, 0.0), rows(kSmVariants, 0.0);
+        std::vector<float> msum(kSmVariants, 0.0f);
+        unsigned long long before = 0, after = 0;
+        HIPCALL(c, hipMemcpy(&before, c->d_counter, sizeof before, hipMemcpyDeviceToHost));
+        for (int k = 0; k < 2 * kSmVariants; k++) {
+            const int v = k < kSmVariants ? k : 2 * kSmVariants - 1 - k;
+            float ms = 0;
+            int rc = timed(v, (int64_t)k * kSeg, kSeg, ms);
+            if (rc) return rc;
+            HIPCALL(c, hipMemcpy(&after, c->d_counter, sizeof after, hipMemcpyDeviceToHost));
+            msum[v] += ms;
+            rows[v] += (double)(after - before);
+            before = after;
+        }
+        for (int v = 0; v < kSmVariants; v++) {
+            c->stats.tune_ms[v] = msum[v];
+            const double per_row = rows[v] > 0 ? msum[v] / rows[v] : 1e30;
+            if (per_row < best) {
+                best = (float)per_row;
+                best_v = v;
+            }
+        }
+        *done = 2 * kSmVariants * kSeg;
+    } else {
+        // three segments at the start, middle and end of the range: with
+        // binned samples the first tiles are the densest, the last the
+        // sparsest; one timing per variant (the instances differ by a few
+        // per cent)
+        const int nseg = ntiles >= 3 * kSeg ? 3 : 1;
+        const int64_t seg_len = std::min<int64_t>(ntiles, kSeg);
+        for (int v = 0; v < kSmVariants; v++) {
+            float total = 0;
+            for (int k = 0; k < nseg; k++) {
+                const int64_t t0 = nseg == 1 ? 0 : (ntiles - seg_len) * k / (nseg - 1);
+                float ms = 0;
+                int rc = timed(v, t0, seg_len, ms);
+                if (rc) return rc;
+                total += ms;
+            }
+            c->stats.tune_ms[v] = total;
+            if (total < best) {
+                best = total;
+                best_v = v;
+            }
         }
     }
     c->variant = best_v;
@@ -734,13 +778,19 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
         const int64_t nb = (c->n + kSmBlock - 1) / kSmBlock;
         const int64_t ntiles = nb * (nb + 1) / 2;
         const int64_t npairs = (int64_t)c->n * (c->n > 0 ? c->n - 1 : 0) / 2;  // d_sim holds the whole triangle
-        rc = pick_variant(c, nb, ntiles, d_sim, npairs, 0, st);
-        if (rc) return rc;
-        HIPCALL(c, hipEventRecord(c->ev0, st));  // after any tuning launches
+        // the kernel time and visited rows include tuning launches that computed real tiles
+        HIPCALL(c, hipEventRecord(c->ev0, st));
         HIPCALL(c, launch_zero_counter(c->d_counter, st));
+        int64_t done = 0;
+        rc = pick_variant(c, nb, ntiles, d_sim, npairs, 0, st, &done);
+        if (rc) return rc;
+        if (done == 0) {
+            HIPCALL(c, hipEventRecord(c->ev0, st));  // discarded tuning launches are not counted
+            HIPCALL(c, launch_zero_counter(c->d_counter, st));
+        }
         HIPCALL(c, launch_zero_counter(c->d_work, st));
-        HIPCALL(c, launch_sm_persistent(sm_data(c), c->variant, nb, 0, ntiles, c->d_work, d_sim, npairs, 0, 0,
-                                        c->grid_waves, st));
+        HIPCALL(c, launch_sm_persistent(sm_data(c), c->variant, nb, done, ntiles - done, c->d_work, d_sim, npairs, 0,
+                                        0, c->grid_waves, st));
         c->stats.tiles = ntiles;
     } else {
         HIPCALL(c, launch_walk_all_pairs(walk_data(c), d_sim, st));
@@ -956,24 +1006,30 @@ int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *u
     }
     if (!c->copy_stream) STREAMCALL(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     // tune on the first band's tiles: their stored-position pairs are the
-    // first count[0] <= cap entries of the packed triangle
-    rc = pick_variant(c, nb, tile_start(cuts[1]), dbuf[0], cap, 1, c->stream);
+    // first count[0] <= cap entries of the packed triangle; tiles the tuner
+    // computed for real are skipped by the first band's launch
+    STREAMCALL(hipEventRecord(c->ev0, c->stream));
+    STREAMCALL(launch_zero_counter(c->d_counter, c->stream));
+    int64_t tuned = 0;
+    rc = pick_variant(c, nb, tile_start(cuts[1]), dbuf[0], cap, 1, c->stream, &tuned);
     if (rc) {
         cleanup();
         return rc;
     }
+    if (tuned == 0) {
+        STREAMCALL(hipEventRecord(c->ev0, c->stream));
+        STREAMCALL(launch_zero_counter(c->d_counter, c->stream));
+    }
     const SmData d = sm_data(c);
     const int nbands = (int)cuts.size() - 1;
     std::vector<int64_t> first(nbands), count(nbands);
-    STREAMCALL(hipEventRecord(c->ev0, c->stream));
-    STREAMCALL(launch_zero_counter(c->d_counter, c->stream));
     for (int k = 0; k <= nbands; k++) {
         if (k < nbands) {
             const int b = k & 1;
             const int64_t r0 = cuts[k] * kSmBlock, r1 = std::min(cuts[k + 1] * kSmBlock, n);
             first[k] = row_start(r0);
             count[k] = row_start(r1) - first[k];
-            const int64_t t0 = tile_start(cuts[k]), t1 = tile_start(cuts[k + 1]);
+            const int64_t t0 = tile_start(cuts[k]) + (k == 0 ? tuned : 0), t1 = tile_start(cuts[k + 1]);
             if (k >= 2) STREAMCALL(hipStreamWaitEvent(c->stream, done_c[b], 0));  // device buffer free again
             if (c->opts.use_graph) {
                 // capture this band's launches and replay them as a graph; later

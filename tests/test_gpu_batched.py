@@ -118,3 +118,53 @@ def test_sliced_embedding_identical(cms, data, metric):
             ctx.set_samples_csr(abd)
             got = ctx.all_pairs()
         np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def wide(cms, gpu_ok):
+    # 15,000 samples: 110,215 block tiles, enough for the tuner to time its
+    # variants on real tiles (12 segments of 8,192) and keep their results
+    tree = cms.synth_tree(1077, 2000, 0.05)
+    cp = cms.synth_copy_numbers(3077, 2000, 15)
+    counts = cms.synth_counts(2077, 15000, 2000, 5, 60, 1.1, 30, 0.5)
+    return tree, cms.counts_to_abundance(counts, cp)
+
+
+def test_autotune_on_real_tiles(cms, wide):
+    """The tuned segments' results stay in the output and the rest of the
+    range is computed once: the same bits as a forced variant, for the
+    all-pairs and the streamed entry points."""
+    print("autotune on real tiles", flush=True)
+    tree, abd = wide
+    n = abd.n
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, variant=2) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        ref = ctx.all_pairs()
+        rows_ref = ctx.stats().pair_node_updates
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, autotune=1) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        got = ctx.all_pairs()
+        st = ctx.stats()
+        assert all(t > 0 for t in st.tune_ms)
+        assert st.pair_node_updates == rows_ref  # tuned tiles counted once
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+    for use_graph in (0, 1):
+        with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, autotune=1, use_graph=use_graph) as ctx:
+            ctx.set_tree(tree)
+            ctx.set_samples_csr(abd)
+            stored = np.full(n * (n - 1) // 2, np.nan, np.float32)
+
+            def fn(first, count, sims):
+                stored[first:first + count] = sims
+
+            ctx.all_pairs_stream(fn, band_pairs=105_000_000)  # first band: 348 of 469 row blocks, ~103,000 tiles
+            order = ctx.sample_order()
+            assert ctx.stats().pair_node_updates == rows_ref
+        assert not np.isnan(stored).any()
+        pi, qi = np.triu_indices(n, 1)
+        a, b = order[pi], order[qi]
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        np.testing.assert_array_equal(stored.view(np.uint32), ref[cms.packed_index(lo, hi, n)].view(np.uint32))
+        del pi, qi, a, b, lo, hi
