@@ -87,7 +87,7 @@ bool is_leaf_metric(const cms_ctx *c) { return c->opts.metric >= CMS_METRIC_COS;
 // The shared-minimum kernels serve CMS_KERNEL_SHAREDMIN and every
 // unweighted request (the walk computes the weighted metric only).
 bool use_sm(const cms_ctx *c) {
-    return !is_leaf_metric(c) && (c->opts.kernel == CMS_KERNEL_SHAREDMIN || c->opts.metric == CMS_METRIC_UNWEIGHTED);
+    return !is_leaf_metric(c) && c->opts.kernel == CMS_KERNEL_SHAREDMIN;
 }
 
 int check_ready(cms_ctx *c) {
@@ -141,7 +141,14 @@ WalkData walk_data(const cms_ctx *c) {
     d.inv_scale = c->d_inv;
     d.n = c->n;
     d.storage = c->ab_storage;
+    d.unweighted = c->opts.metric == CMS_METRIC_UNWEIGHTED ? 1 : 0;
     return d;
+}
+
+// Abundance storage of the walk: the unweighted walk tests presence and keeps
+// F32 (a 16-bit value can round a small abundance to zero).
+int walk_storage(const cms_ctx *c) {
+    return c->opts.metric == CMS_METRIC_UNWEIGHTED ? CMS_STORE_F32 : c->opts.storage;
 }
 
 int sharedmin_supported(cms_ctx *c) {
@@ -362,9 +369,9 @@ int prepare(cms_ctx *c) {
             return fail(c, CMS_E_UNSUPPORTED,
                         "schedule keeps " + std::to_string(c->tree.nslots) + " registers live; the walk kernel holds " +
                             std::to_string(walk_max_slots()) + " (re-order the tree, e.g. Sethi-Ullman)");
-        if (c->ab_storage == c->opts.storage && !c->ab_binned) return CMS_OK;
+        if (c->ab_storage == walk_storage(c) && !c->ab_binned) return CMS_OK;
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
-        rc = ensure_walk_layout(c, c->opts.storage, false);
+        rc = ensure_walk_layout(c, walk_storage(c), false);
     }
     if (rc) return rc;
     HIPCALL(c, hipEventRecord(c->evp1, c->stream));
@@ -797,7 +804,9 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
         const int64_t nb = (c->n + kWalkBlock - 1) / kWalkBlock;
         c->stats.tiles = nb * (nb + 1) / 2;
         c->stats.pair_node_updates = (double)c->stats.tiles * 64.0 * c->tree.order_n;
-        c->stats.valu_ops = c->stats.pair_node_updates * 14.0;
+        // weighted: 2 min, 2 add, 4 sub, 4 mul, 2 add; unweighted: per child
+        // 2 selects and 2 adds, 2 presence ORs
+        c->stats.valu_ops = c->stats.pair_node_updates * (c->opts.metric == CMS_METRIC_UNWEIGHTED ? 10.0 : 14.0);
         const double elem = c->ab_storage == CMS_STORE_F32 ? 4.0 : 2.0;
         c->stats.bytes_moved = (double)c->stats.tiles * 16.0 * elem * (double)c->tree.leaf_stream.size();
     }

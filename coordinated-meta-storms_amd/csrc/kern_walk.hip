@@ -64,13 +64,27 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nwg) {
     return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
+// total / count, clamped to [0, 1] (comp.h:437-440); two samples with no
+// present leaf divide 0 by 0, which gives the sign-set NaN on the host.
+__device__ __forceinline__ float unweighted_sim(float total, float count) {
+    if (count == 0.0f && total == 0.0f) return __uint_as_float(0xffc00000u);
+    float t = __fdiv_rn(total, count);
+    t = (t > 1.0f) ? 1.0f : t;
+    t = (t < 0.0f) ? 0.0f : t;
+    return t;
+}
+
 // MODE 0: all pairs (upper-triangle tiles, packed output).
 // MODE 1: rectangle r0.. x c0.. (row-major output, diagonal = 1).
 // VREG: the live register slots (at most kWalkVregSlots) are VGPR arrays
 // indexed by the wave-uniform slot number (s_set_gpr_idx: one v_mov per
 // access, no LDS); otherwise they sit in LDS as two planes (sample a,
 // sample b), one dword per lane at slot * 64 + lane (bank-conflict free).
-template <int STORE, int MODE, bool VREG>
+// UNW: the unweighted walk of Calc_sim_unweight (comp.h:378-446): presence
+// flags propagate, and each child edge adds its weight to the shared total
+// when both samples are present and to the count when either is; sim =
+// total / count, clamped (F32 storage only).
+template <int STORE, int MODE, bool VREG, bool UNW = false>
 __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, int64_t ntiles, int64_t nb,
                                                                      int r0, int nr, int c0, int nc,
                                                                      float *__restrict__ out) {
@@ -185,6 +199,17 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
             } else {
                 slot_read((code >> 8) & 0xffu, a2, b2);
             }
+            if constexpr (UNW) {
+                // leaf values are abundances, register values presence flags
+                const bool p1a = a1 > 0.0f, p1b = b1 > 0.0f, p2a = a2 > 0.0f, p2b = b2 > 0.0f;
+                total += (p1a && p1b) ? w1 : 0.0f;
+                comp += (p1a || p1b) ? w1 : 0.0f;  // the count of comp.h:382
+                total += (p2a && p2b) ? w2 : 0.0f;
+                comp += (p2a || p2b) ? w2 : 0.0f;
+                if (code & kDstLive)
+                    slot_write((code >> 16) & 0xffu, (p1a || p2a) ? 1.0f : 0.0f, (p1b || p2b) ? 1.0f : 0.0f);
+                continue;
+            }
             const float m1 = fminf(a1, b1), m2 = fminf(a2, b2);
             if constexpr (KAHAN) {
                 float y = m1 - comp, t = total + y;
@@ -209,7 +234,9 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
     }
 
     float sim;
-    if constexpr (KAHAN) {
+    if constexpr (UNW) {
+        sim = unweighted_sim(total, comp);
+    } else if constexpr (KAHAN) {
         double t = ((double)total - (double)comp) / 100.0;
         t = t > 1.0 ? 1.0 : t;
         t = t < 0.0 ? 0.0 : t;
@@ -233,7 +260,7 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
 }
 
 // Explicit pair list: one lane per pair, leaf values gathered from HBM.
-template <int STORE>
+template <int STORE, bool UNW = false>
 __global__ __launch_bounds__(256) void walk_pairs_kernel(WalkData d, int64_t n_pairs, const int32_t *__restrict__ prow,
                                                          const int32_t *__restrict__ pcol, float *__restrict__ out) {
     extern __shared__ float lds[];
@@ -271,6 +298,18 @@ __global__ __launch_bounds__(256) void walk_pairs_kernel(WalkData d, int64_t n_p
             a2 = slotA[((code >> 8) & 0xffu) * 64 + lane];
             b2 = slotB[((code >> 8) & 0xffu) * 64 + lane];
         }
+        if constexpr (UNW) {
+            const bool p1a = a1 > 0.0f, p1b = b1 > 0.0f, p2a = a2 > 0.0f, p2b = b2 > 0.0f;
+            total += (p1a && p1b) ? w1 : 0.0f;
+            comp += (p1a || p1b) ? w1 : 0.0f;
+            total += (p2a && p2b) ? w2 : 0.0f;
+            comp += (p2a || p2b) ? w2 : 0.0f;
+            if (code & kDstLive) {
+                slotA[((code >> 16) & 0xffu) * 64 + lane] = (p1a || p2a) ? 1.0f : 0.0f;
+                slotB[((code >> 16) & 0xffu) * 64 + lane] = (p1b || p2b) ? 1.0f : 0.0f;
+            }
+            continue;
+        }
         const float m1 = fminf(a1, b1), m2 = fminf(a2, b2);
         if constexpr (KAHAN) {
             float y = m1 - comp, t = total + y;
@@ -292,7 +331,9 @@ __global__ __launch_bounds__(256) void walk_pairs_kernel(WalkData d, int64_t n_p
         }
     }
     float sim;
-    if constexpr (KAHAN) {
+    if constexpr (UNW) {
+        sim = unweighted_sim(total, comp);
+    } else if constexpr (KAHAN) {
         double t = ((double)total - (double)comp) / 100.0;
         t = t > 1.0 ? 1.0 : t;
         t = t < 0.0 ? 0.0 : t;
@@ -400,7 +441,11 @@ static hipError_t launch_tiles(const WalkData &d, int64_t ntiles, int64_t nb, in
 #define CMS_WALK_TILES(VR)                                                                                  \
     switch (d.storage) {                                                                                    \
     case CMS_STORE_F32:                                                                                     \
-        walk_tile_kernel<CMS_STORE_F32, MODE, VR><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);  \
+        if (d.unweighted)                                                                                   \
+            walk_tile_kernel<CMS_STORE_F32, MODE, VR, true>                                                 \
+                <<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out);                             \
+        else                                                                                                \
+            walk_tile_kernel<CMS_STORE_F32, MODE, VR><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out); \
         break;                                                                                              \
     case CMS_STORE_BF16:                                                                                    \
         walk_tile_kernel<CMS_STORE_BF16, MODE, VR><<<grid, block, lds, st>>>(d, ntiles, nb, r0, nr, c0, nc, out); \
@@ -439,7 +484,10 @@ hipError_t launch_walk_pairs(const WalkData &d, int64_t n_pairs, const int32_t *
     const dim3 grid((unsigned)nwg), block(64 * waves);
     switch (d.storage) {
     case CMS_STORE_F32:
-        walk_pairs_kernel<CMS_STORE_F32><<<grid, block, lds, st>>>(d, n_pairs, row, col, out);
+        if (d.unweighted)
+            walk_pairs_kernel<CMS_STORE_F32, true><<<grid, block, lds, st>>>(d, n_pairs, row, col, out);
+        else
+            walk_pairs_kernel<CMS_STORE_F32><<<grid, block, lds, st>>>(d, n_pairs, row, col, out);
         break;
     case CMS_STORE_BF16:
         walk_pairs_kernel<CMS_STORE_BF16><<<grid, block, lds, st>>>(d, n_pairs, row, col, out);

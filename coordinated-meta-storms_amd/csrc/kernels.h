@@ -34,6 +34,7 @@ struct WalkData {
     const float *inv_scale;  // per-sample 2^-e (16-bit storage), may be null for F32
     int n;              // samples
     int storage;        // CMS_STORE_*
+    int unweighted;     // Calc_sim_unweight (F32 storage only)
 };
 
 // Largest live-register count the walk kernels can hold in LDS.

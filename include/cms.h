@@ -54,7 +54,7 @@ enum {
 /* Distance metric (comp -M). */
 enum {
     CMS_METRIC_WEIGHTED = 0,   /* -M 0, Meta-Storms (comp.h:332-375) */
-    CMS_METRIC_UNWEIGHTED = 1, /* -M 1, unweighted Meta-Storms (comp.h:378-446); shared-minimum kernel */
+    CMS_METRIC_UNWEIGHTED = 1, /* -M 1, unweighted Meta-Storms (comp.h:378-446); walk (bit-exact) or shared-minimum */
     CMS_METRIC_COS = 2,        /* -M 2..5: leaf-level metrics, dist.h:13-159 via Calc_sim(a, b, mode) */
     CMS_METRIC_EU = 3,         /*   (comp.h:296-330), similarity = 1 - dist; cms_all_pairs and */
     CMS_METRIC_JSD = 4,        /*   cms_all_pairs_device only, no tree schedule needed (order_n may */
