@@ -427,6 +427,8 @@ def main():
     ap.add_argument("--metrics", type=str, default="",
                     help="workload 1: also time these comp -M metrics (e.g. 1,2,3,4,5) on the same samples")
     a = ap.parse_args()
+    if a.cpu_pairs < 1 or a.cpu_samples < 2:
+        ap.error("--cpu-pairs must be >= 1 and --cpu-samples >= 2 (--no-cpu skips the CPU baseline)")
     if a.gpus != 1:
         print("bench.py measures one GPU: every GPU box here has one, and multi-GPU runs are not part of "
               "this build (pairs shard without communication; see DESIGN.md)", file=sys.stderr)
@@ -471,6 +473,17 @@ def main():
         others[f"M{m}"] = {"ms_per_step": osec * 1e3, "pairs_per_s": npairs / osec, "kernel_ms": ost.kernel_ms,
                            "prep_ms": ost.prep_ms}
         log(f"-M {m}: {osec * 1e3:.2f} ms/step, kernel {ost.kernel_ms:.2f} ms, prep {ost.prep_ms:.2f} ms")
+        if m >= 2:
+            # the same metric in the reference's FP32 operation order (per-pair merge)
+            with cms.Context(0, kernel=cms.KERNEL_WALK, metric=m) as oc:
+                oc.set_tree(tree)
+                oc.set_samples_csr(abd)
+                osec = timed_steps(oc, 1, 1)
+                ost = oc.stats()
+            others[f"M{m}_reference_order"] = {"ms_per_step": osec * 1e3, "pairs_per_s": npairs / osec,
+                                               "kernel_ms": ost.kernel_ms, "prep_ms": ost.prep_ms}
+            log(f"-M {m} reference order: {osec * 1e3:.2f} ms/step, kernel {ost.kernel_ms:.2f} ms, "
+                f"prep {ost.prep_ms:.2f} ms")
 
     walk = None
     gpu_walk = None

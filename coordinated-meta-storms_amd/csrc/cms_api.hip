@@ -79,10 +79,17 @@ void drop_layouts(cms_ctx *c) {
     dfree(c->d_leafstats);
     c->cap_leafP = c->cap_leafbm = c->cap_leafstats = 0;
     c->leaf_ready = false;
+    dfree(c->d_lment);
+    dfree(c->d_lmstat);
+    dfree(c->d_lmperm);
+    c->lm_layout = -1;
 }
 
-// -M 2..5 run on the leaf-metric kernels.
+// -M 2..5 run on the leaf-metric kernels: with CMS_KERNEL_WALK the merge
+// kernels (the reference's FP32 arithmetic), with CMS_KERNEL_SHAREDMIN the
+// leaf tiles (FP64 sums).
 bool is_leaf_metric(const cms_ctx *c) { return c->opts.metric >= CMS_METRIC_COS; }
+bool use_leaf_merge(const cms_ctx *c) { return is_leaf_metric(c) && c->opts.kernel == CMS_KERNEL_WALK; }
 
 // The shared-minimum kernels serve CMS_KERNEL_SHAREDMIN and every
 // unweighted request (the walk computes the weighted metric only).
@@ -340,6 +347,29 @@ int ensure_leaf_layout(cms_ctx *c) {
     return CMS_OK;
 }
 
+// Entry lists, per-sample FP32 totals and the visiting order of the merge
+// kernels; cosine merges the values, the others a / sum a.
+int ensure_leaf_merge(cms_ctx *c) {
+    const int layout = c->opts.metric == CMS_METRIC_COS ? 0 : 1;
+    if (c->lm_layout == layout) return CMS_OK;
+    const int n = c->n;
+    std::vector<int64_t> rp((size_t)n + 1, 0);
+    if (n > 0) HIPCALL(c, hipMemcpy(rp.data(), c->d_rowptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    // most entries first: the 8 x 8 pairs of a wave merge lists of similar length
+    std::vector<int32_t> perm((size_t)n);
+    for (int s = 0; s < n; s++) perm[s] = s;
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](int32_t a, int32_t b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+    HIPCALL(c, dalloc(c->d_lment, (size_t)std::max<int64_t>(c->nnz, 1)));
+    HIPCALL(c, dalloc(c->d_lmstat, (size_t)std::max(n, 1)));
+    HIPCALL(c, dalloc(c->d_lmperm, (size_t)std::max(n, 1)));
+    if (n > 0) HIPCALL(c, hipMemcpy(c->d_lmperm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCALL(c, launch_leaf_merge_prepare(c->d_rowptr, c->d_col, c->d_val, n, layout == 1, c->d_lment, c->d_lmstat,
+                                         c->stream));
+    c->lm_layout = layout;
+    return CMS_OK;
+}
+
 LeafData leaf_data(const cms_ctx *c) {
     LeafData d{};
     d.P = c->d_leafP;
@@ -356,10 +386,11 @@ int prepare(cms_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (is_leaf_metric(c)) {
-        if (c->leaf_ready) return CMS_OK;
+        if (use_leaf_merge(c) ? c->lm_layout == (c->opts.metric == CMS_METRIC_COS ? 0 : 1) : c->leaf_ready)
+            return CMS_OK;
         if (c->has_negative) return fail(c, CMS_E_UNSUPPORTED, "-M 2..5 on the GPU need non-negative abundances");
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
-        rc = ensure_leaf_layout(c);
+        rc = use_leaf_merge(c) ? ensure_leaf_merge(c) : ensure_leaf_layout(c);
     } else if (use_sm(c)) {
         if (c->sm_ready && c->sm_metric == c->opts.metric && c->sm_binned == (c->opts.bin != 0)) return CMS_OK;
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
@@ -773,8 +804,13 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
         }
         const int64_t npairs = (int64_t)c->n * (c->n > 0 ? c->n - 1 : 0) / 2;
         HIPCALL(c, hipEventRecord(c->ev0, st));
-        HIPCALL(c, launch_zero_counter(c->d_work, st));
-        HIPCALL(c, launch_leaf_all_pairs(leaf_data(c), c->opts.metric, c->d_work, d_sim, npairs, c->grid_waves, st));
+        if (use_leaf_merge(c)) {
+            LeafMergeData md{c->d_rowptr, c->d_lment, c->d_lmstat, c->d_lmperm, c->n};
+            HIPCALL(c, launch_leaf_merge_all_pairs(md, c->opts.metric, d_sim, npairs, st));
+        } else {
+            HIPCALL(c, launch_zero_counter(c->d_work, st));
+            HIPCALL(c, launch_leaf_all_pairs(leaf_data(c), c->opts.metric, c->d_work, d_sim, npairs, c->grid_waves, st));
+        }
         HIPCALL(c, hipEventRecord(c->ev1, st));
         c->stats.pair_node_updates = c->stats.valu_ops = c->stats.bytes_moved = 0.0;
         c->kern_pending = true;

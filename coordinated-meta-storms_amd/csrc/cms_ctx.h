@@ -73,6 +73,11 @@ struct cms_ctx {
     uint32_t *d_leafbm = nullptr;
     double *d_leafstats = nullptr;
     size_t cap_leafP = 0, cap_leafbm = 0, cap_leafstats = 0;
+    // leaf metrics in the reference's arithmetic (merge kernels)
+    int lm_layout = -1;  // -1 none, 0 raw values (cosine), 1 normalised
+    uint2 *d_lment = nullptr;
+    float2 *d_lmstat = nullptr;
+    int32_t *d_lmperm = nullptr;
     std::vector<int32_t> h_leaf_uses;  // per leaf: leaf-stream positions consuming it
     unsigned long long *d_counter = nullptr;  // entries visited by shared-minimum tiles
 

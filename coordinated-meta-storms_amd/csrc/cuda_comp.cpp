@@ -53,7 +53,8 @@ namespace {
     printf("\t     4: Jensen-Shannon; 5: Bray-Curtis; default is 0\n");
     printf("\t  -r rRNA copy number correction, T(rue) or F(alse), default is T\n");
     printf("\t  -t Number of host threads for loading, default is auto\n");
-    printf("\t  --kernel walk|sharedmin  GPU kernel for -M 0/1 (walk is bit-identical to comp)\n");
+    printf("\t  --kernel walk|sharedmin  walk (default): the reference's FP32 arithmetic per pair, bit-identical\n");
+    printf("\t                           to comp (-M 4: to the last bit of logf); sharedmin: the fast kernels\n");
     printf("\t  --storage f32|bf16|f16   walk abundance storage, default f32\n");
     printf("\t  --device k               GPU index, default 0\n");
     printf("\t  --format dist|binary     text .dist (default) or packed float32 triangle + .names\n");
@@ -164,7 +165,7 @@ int main(int argc, char **argv) {
     // without a tree, -M 0 and -M 1 fall back to JSD as in Calc_sim (comp.h:299-302)
     const bool has_tree = cms_db_has_tree(db) && cms_db_order_n(db) > 0;
     if (!has_tree && metric < 2) metric = 4;
-    opts.metric = metric;  // -M 0/1: the walk (bit-identical) or --kernel sharedmin; -M 2..5: the leaf kernels
+    opts.metric = metric;  // walk: the reference's arithmetic (walk or leaf merge); sharedmin: the fast kernels
     const int L = cms_db_leaf_n(db), on = cms_db_order_n(db);
     cms_table *table = nullptr;
     cms_list *list = nullptr;

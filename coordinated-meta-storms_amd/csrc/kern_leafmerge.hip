@@ -1,0 +1,226 @@
+// Non-tree metrics of comp -M 2..5 in the reference's FP32 arithmetic:
+// Calc_Dist_Cos, Calc_Dist_E, Calc_Dist_JSD and Calc_Dist_Bray_Curtis
+// (dist.h:13-159), similarity = (float)(1.0 - dist) (comp.h:313-326).
+//
+// The reference loops over all leaves. At a leaf where both samples are zero
+// each of its sums gains +0 (cosine, Euclidean, Bray-Curtis: sums that start
+// at +0 and only gain non-negative terms are unchanged) or the leaf is skipped
+// (JSD), so a lane that merges the two samples' leaf lists in ascending leaf
+// order and applies the reference's step at every leaf of the union computes
+// the same FP32 values in the same order. Per-sample quantities that the
+// reference recomputes inside each call (sum of a, sum of a^2, a / sum a) are
+// computed once per sample with the same FP32 operations in leaf order.
+// Cosine, Euclidean and Bray-Curtis are bit-identical to the reference; JSD
+// calls the device logf, which may differ from the host's in the last bit.
+// All code here is built without FMA contraction; divisions and square roots
+// are the correctly rounded ones.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "kernels.h"
+
+namespace cms {
+
+namespace {
+
+constexpr uint32_t kEnd = 0xffffffffu;  // past the last leaf of a list (leaf ids are < 2^31)
+constexpr int kMergeWavesPerBlock = 4;
+
+// One thread per sample: sum of a and sum of a^2 in FP32, leaf order
+// (dist.h:20-30 and 49-57), and the entry list (leaf, a) or (leaf, a / sum a)
+// (dist.h:62-65).
+template <bool NORM>
+__global__ __launch_bounds__(64) void leaf_merge_prepare_kernel(const int64_t *__restrict__ rowptr,
+                                                                const int32_t *__restrict__ col,
+                                                                const float *__restrict__ val, int n,
+                                                                uint2 *__restrict__ ent, float2 *__restrict__ stat) {
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= n) return;
+    const int64_t b = rowptr[s], e = rowptr[s + 1];
+    float sum = 0.0f, sq = 0.0f;
+    for (int64_t k = b; k < e; k++) {
+        const float v = val[k];
+        sum = sum + v;
+        sq = sq + v * v;
+    }
+    for (int64_t k = b; k < e; k++) {
+        const float v = val[k];
+        ent[k] = make_uint2((uint32_t)col[k], __float_as_uint(NORM ? __fdiv_rn(v, sum) : v));
+    }
+    stat[s] = make_float2(sum, sq);
+}
+
+// One leaf of the union, p and q being the two samples' values there (zero
+// where a sample has no entry).
+template <int MODE>
+struct MergeAcc {
+    float f = 0.0f, g = 0.0f;  // cos: f; Euclidean: f; Bray-Curtis: sum, diff; JSD: dkl_m, dkl_n
+    __device__ __forceinline__ void step(float p, float q) {
+        if constexpr (MODE == CMS_METRIC_COS) {
+            f = f + p * q;  // dist.h:25
+        } else if constexpr (MODE == CMS_METRIC_EU) {
+            const float d = p - q;  // pow(d, 2) is exact in double (dist.h:71-72)
+            f = __double2float_rn(__dadd_rn((double)f, (double)d * (double)d));
+        } else if constexpr (MODE == CMS_METRIC_BC) {
+            f = f + (p + q);  // dist.h:146-150
+            float d = p - q;
+            if (d < 0.0f) d = -d;
+            g = g + d;
+        } else {
+            if (p == 0.0f && q == 0.0f) return;  // dist.h:107-118
+            const float h = __fdiv_rn(p + q, 2.0f);
+            if (p > 0.0f) f = f + p * logf(__fdiv_rn(p, h));
+            if (q > 0.0f) g = g + q * logf(__fdiv_rn(q, h));
+        }
+    }
+};
+
+// The reference's distance for a pair whose totals are degenerate, or a
+// negative value when the lists must be merged.
+template <int MODE>
+__device__ __forceinline__ bool degenerate(float2 xa, float2 xb, float &dist) {
+    if constexpr (MODE == CMS_METRIC_COS) {
+        if (xa.y * xb.y == 0.0f) {  // dist.h:33
+            dist = 1.0f;
+            return true;
+        }
+    } else if constexpr (MODE == CMS_METRIC_EU || MODE == CMS_METRIC_BC) {
+        if (xa.x <= 0.0f || xb.x <= 0.0f) {  // dist.h:59-60, 138-139
+            dist = 1.0f;
+            return true;
+        }
+    } else {
+        // JSD has no guard: an all-zero sample divides 0 by 0 at every leaf.
+        // Both all-zero: every term is skipped (NaN > 0 is false), dist 0.
+        // One all-zero: the other sample's first leaf adds q * log(q / NaN),
+        // the x86 default NaN (sign set) that 0/0 gives.
+        if (xa.x == 0.0f && xb.x == 0.0f) {
+            dist = 0.0f;
+            return true;
+        }
+        if (xa.x == 0.0f || xb.x == 0.0f) {
+            dist = __uint_as_float(0xffc00000u);
+            return true;
+        }
+    }
+    return false;
+}
+
+template <int MODE>
+__device__ __forceinline__ float finish(const MergeAcc<MODE> &acc, float2 xa, float2 xb) {
+    if constexpr (MODE == CMS_METRIC_COS) {
+        return 1.0f - __fdiv_rn(acc.f, sqrtf(xa.y * xb.y));  // dist.h:35-37
+    } else if constexpr (MODE == CMS_METRIC_EU) {
+        return sqrtf(acc.f);  // dist.h:74
+    } else if constexpr (MODE == CMS_METRIC_BC) {
+        return acc.f <= 0.0f ? 1.0f : __fdiv_rn(acc.g, acc.f);  // dist.h:152-153
+    } else {
+        return __double2float_rn(__dsqrt_rn((double)(acc.f + acc.g) / 2.0));  // dist.h:124
+    }
+}
+
+// Upper-triangle tiles of 8 x 8 pairs over the visiting order (samples with
+// the most entries first, so the lanes of a wave merge lists of similar
+// length), one lane per pair, grid-stride over tiles.
+template <int MODE>
+__global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(LeafMergeData d, int64_t nb,
+                                                                              int64_t t_count,
+                                                                              float *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * kMergeWavesPerBlock;
+    for (int64_t t = (int64_t)blockIdx.x * kMergeWavesPerBlock + (threadIdx.x >> 6); t < t_count; t += stride) {
+        const double nn = (double)nb;
+        int64_t ra = (int64_t)(((2.0 * nn + 1.0) - sqrt((2.0 * nn + 1.0) * (2.0 * nn + 1.0) - 8.0 * (double)t)) * 0.5);
+        if (ra < 0) ra = 0;
+        while (ra > 0 && ra * nb - ra * (ra - 1) / 2 > t) ra--;
+        while ((ra + 1) * nb - (ra + 1) * ra / 2 <= t) ra++;
+        const int64_t cb = ra + (t - (ra * nb - ra * (ra - 1) / 2));
+        const int64_t pa = ra * 8 + (lane >> 3), pb = cb * 8 + (lane & 7);
+        if (pa >= pb || pb >= d.n) continue;
+        int sa = d.perm[pa], sb = d.perm[pb];
+        if (sa > sb) {
+            const int x = sa;
+            sa = sb;
+            sb = x;
+        }
+        const float2 xa = d.stat[sa], xb = d.stat[sb];
+        float dist;
+        if (!degenerate<MODE>(xa, xb, dist)) {
+            const int64_t a0 = d.rowptr[sa], b0 = d.rowptr[sb];
+            const uint2 *A = d.ent + a0, *B = d.ent + b0;
+            const int64_t na = d.rowptr[sa + 1] - a0, nbb = d.rowptr[sb + 1] - b0;
+            // current entry of each list and the one after it, loaded ahead
+            uint2 ea = na > 0 ? A[0] : make_uint2(kEnd, 0u), eb = nbb > 0 ? B[0] : make_uint2(kEnd, 0u);
+            uint2 fa = na > 1 ? A[1] : make_uint2(kEnd, 0u), fb = nbb > 1 ? B[1] : make_uint2(kEnd, 0u);
+            int64_t i = 1, j = 1;
+            MergeAcc<MODE> acc;
+            for (;;) {
+                const uint32_t m = min(ea.x, eb.x);
+                if (m == kEnd) break;
+                const bool ta = ea.x == m, tb = eb.x == m;
+                acc.step(ta ? __uint_as_float(ea.y) : 0.0f, tb ? __uint_as_float(eb.y) : 0.0f);
+                if (ta) {
+                    ea = fa;
+                    ++i;
+                    fa = i < na ? A[i] : make_uint2(kEnd, 0u);
+                }
+                if (tb) {
+                    eb = fb;
+                    ++j;
+                    fb = j < nbb ? B[j] : make_uint2(kEnd, 0u);
+                }
+            }
+            dist = finish<MODE>(acc, xa, xb);
+        }
+        // an x86 invalid operation gives the default NaN (0xffc00000) and
+        // 1.0 - NaN keeps it; the device's arithmetic returns another NaN
+        out[(int64_t)sa * d.n + sb - ((int64_t)sa + 1) * ((int64_t)sa + 2) / 2] =
+            dist != dist ? __uint_as_float(0xffc00000u) : __double2float_rn(1.0 - (double)dist);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_leaf_merge_prepare(const int64_t *rowptr, const int32_t *col, const float *val, int n,
+                                     bool normalise, uint2 *ent, float2 *stat, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 63) / 64);
+    if (normalise)
+        leaf_merge_prepare_kernel<true><<<blocks, 64, 0, st>>>(rowptr, col, val, n, ent, stat);
+    else
+        leaf_merge_prepare_kernel<false><<<blocks, 64, 0, st>>>(rowptr, col, val, n, ent, stat);
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_merge_all_pairs(const LeafMergeData &d, int mode, float *out, int64_t out_cap,
+                                       hipStream_t st) {
+    const int64_t n = d.n;
+    if (n < 2) return hipSuccess;
+    if (out_cap < n * (n - 1) / 2) return hipErrorInvalidValue;
+    const int64_t nb = (n + 7) / 8;
+    const int64_t ntiles = nb * (nb + 1) / 2;
+    const unsigned blocks =
+        (unsigned)std::min<int64_t>((ntiles + kMergeWavesPerBlock - 1) / kMergeWavesPerBlock, (int64_t)1 << 20);
+    const unsigned threads = 64 * kMergeWavesPerBlock;
+    switch (mode) {
+    case CMS_METRIC_COS:
+        leaf_merge_kernel<CMS_METRIC_COS><<<blocks, threads, 0, st>>>(d, nb, ntiles, out);
+        break;
+    case CMS_METRIC_EU:
+        leaf_merge_kernel<CMS_METRIC_EU><<<blocks, threads, 0, st>>>(d, nb, ntiles, out);
+        break;
+    case CMS_METRIC_JSD:
+        leaf_merge_kernel<CMS_METRIC_JSD><<<blocks, threads, 0, st>>>(d, nb, ntiles, out);
+        break;
+    case CMS_METRIC_BC:
+        leaf_merge_kernel<CMS_METRIC_BC><<<blocks, threads, 0, st>>>(d, nb, ntiles, out);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cms

@@ -182,6 +182,22 @@ hipError_t launch_leaf_prepare(const int64_t *rowptr, const int32_t *col, const 
 hipError_t launch_leaf_all_pairs(const LeafData &d, int mode, unsigned long long *work, float *out, int64_t out_cap,
                                  int grid_waves, hipStream_t st);
 
+// The same metrics in the reference's FP32 operation order: per pair, a merge
+// of the two samples' entry lists.
+struct LeafMergeData {
+    const int64_t *rowptr;  // the caller's samples, entries in ascending leaf order
+    const uint2 *ent;       // per entry: leaf, bits of a (cosine) or of a / sum a (the others)
+    const float2 *stat;     // per sample: FP32 sum of a, FP32 sum of a^2, both in leaf order
+    const int32_t *perm;    // visiting position -> sample (most entries first)
+    int n;
+};
+// Fill ent (one per CSR entry) and stat (one per sample).
+hipError_t launch_leaf_merge_prepare(const int64_t *rowptr, const int32_t *col, const float *val, int n,
+                                     bool normalise, uint2 *ent, float2 *stat, hipStream_t st);
+// All pairs of metric `mode`, packed upper triangle of out_cap >= n (n - 1) / 2.
+hipError_t launch_leaf_merge_all_pairs(const LeafMergeData &d, int mode, float *out, int64_t out_cap,
+                                       hipStream_t st);
+
 // ------------------------------------------------------------------ serving
 
 // Query slots (columns of QY, same rows as the references' Y) against the

@@ -47,8 +47,11 @@ enum {
 
 /* Kernel families. */
 enum {
-    CMS_KERNEL_WALK = 0,     /* drop-in: per-pair post-order walk, FP32, bit-exact with comp -M 0 */
-    CMS_KERNEL_SHAREDMIN = 1 /* per-sample decayed-mass embedding + fixed-point shared-minimum tiles */
+    CMS_KERNEL_WALK = 0,     /* drop-in: the reference's FP32 arithmetic per pair, bit-exact with comp:
+                                the post-order walk (-M 0/1), a merge of the two leaf lists (-M 2..5;
+                                -M 4 to the last bit of logf) */
+    CMS_KERNEL_SHAREDMIN = 1 /* fast: decayed-mass embedding + fixed-point shared-minimum tiles (-M 0/1),
+                                leaf tiles with FP64 sums (-M 2..5) */
 };
 
 /* Distance metric (comp -M). */
@@ -58,7 +61,7 @@ enum {
     CMS_METRIC_COS = 2,        /* -M 2..5: leaf-level metrics, dist.h:13-159 via Calc_sim(a, b, mode) */
     CMS_METRIC_EU = 3,         /*   (comp.h:296-330), similarity = 1 - dist; cms_all_pairs and */
     CMS_METRIC_JSD = 4,        /*   cms_all_pairs_device only, no tree schedule needed (order_n may */
-    CMS_METRIC_BC = 5          /*   be 0); within 1e-5 of FP64 */
+    CMS_METRIC_BC = 5          /*   be 0) */
 };
 
 /* Abundance storage of the walk kernel (arithmetic is always FP32). */

@@ -5,9 +5,14 @@ _Comp_Tree::Calc_sim(a, b, mode), comp.h:296-330; similarity = 1 - dist).
 CPU: the oracle's FP32 restatement equals the compiled reference bit for bit
 on every golden pair (hex fixtures), its FP64 form agrees with it and with
 closed forms (identical, disjoint and degenerate samples).
-GPU: the leaf-tile kernels are within 1e-5 of the FP64 oracle (with the
-reference's degenerate-sample semantics) and `cuda-comp -M 2..5` prints the
-reference's .dist values to the 6-digit print.
+GPU, two kernels:
+- the merge kernels (CMS_KERNEL_WALK, cuda-comp's default) follow the
+  reference's FP32 operation order: cosine, Euclidean and Bray-Curtis equal
+  the reference (golden hex) and the oracle's FP32 restatement bit for bit,
+  and `cuda-comp -M 2/3/5` writes the reference's .dist bytes; JSD calls the
+  device logf, so it is held to JSD_TOL of the reference's values;
+- the leaf tiles (CMS_KERNEL_SHAREDMIN) are within 1e-5 of the FP64 oracle,
+  with the reference's degenerate-sample semantics.
 """
 import math
 import os
@@ -19,7 +24,12 @@ import pytest
 from conftest import GOLDEN_CASES
 
 MODES = [2, 3, 4, 5]
+EXACT = (2, 3, 5)   # metrics without a transcendental call: bit-identical
 TOL = 1e-5
+# JSD in the reference's operation order, device logf against the host's
+# (each within about an ulp): far below the 6-digit print
+JSD_TOL = 2e-6
+KERNELS = ["merge", "tiles"]
 
 
 def _hex(g, m):
@@ -109,23 +119,44 @@ def _same(got, exp, tol=TOL):
     return float(err.max()) if err.size else 0.0
 
 
+def _kernel(cms, name):
+    return cms.KERNEL_WALK if name == "merge" else cms.KERNEL_SHAREDMIN
+
+
+def _check_reference_order(got, s32, mode):
+    """Merge kernels against the reference's FP32 values: bit for bit for
+    cosine, Euclidean and Bray-Curtis; JSD within JSD_TOL (NaN bits equal)."""
+    got = np.asarray(got, np.float32)
+    if mode in EXACT:
+        np.testing.assert_array_equal(got.view(np.uint32), s32.view(np.uint32))
+    else:
+        nan = np.isnan(s32)
+        np.testing.assert_array_equal(got.view(np.uint32)[nan], s32.view(np.uint32)[nan])
+        assert _same(got, s32.astype(np.float64), JSD_TOL) <= JSD_TOL
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", GOLDEN_CASES)
 @pytest.mark.parametrize("mode", MODES)
-def test_gpu_golden(cms, orc, gpu_ok, golden, case, mode):
-    """Golden inputs through cms_all_pairs: within 1e-5 of the reference's
-    FP32 values and of the FP64 definitions."""
-    print(f"leaf metric -M {mode} golden {case}", flush=True)
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_gpu_golden(cms, orc, gpu_ok, golden, case, mode, kernel):
+    """Golden inputs through cms_all_pairs: the merge kernels reproduce the
+    reference's FP32 values; the leaf tiles are within 1e-5 of them and of the
+    FP64 definitions."""
+    print(f"leaf metric -M {mode} golden {case} {kernel}", flush=True)
     g = golden(case)
     db = cms.Database(g.root, "G")
     abd, _ = cms.Table(g.table).abundance(db, True)
     i, j, ref = _hex(g, mode)
-    with cms.Context(0, metric=mode) as c:
+    with cms.Context(0, kernel=_kernel(cms, kernel), metric=mode) as c:
         c.set_tree(db.tree())
         c.set_samples(abd)
         got = c.all_pairs()
     n = abd.shape[0]
     p = cms.packed_index(i, j, n)
+    if kernel == "merge":
+        _check_reference_order(got[p], ref, mode)
+        return
     _, s64 = orc.pairs_leaf(mode, abd, i, j)
     assert _same(got[p], ref.astype(np.float64)) <= TOL
     assert _same(got[p], s64) <= TOL
@@ -152,9 +183,9 @@ def _synthetic(seed, n, leaf_n):
 def test_gpu_synthetic_vs_fp64(cms, orc, gpu_ok, mode, n, leaf_n):
     """Ragged sample counts (tiles past n), leaf counts across bitmap windows,
     degenerate samples; no tree (order_n 0) is accepted for these metrics."""
-    print(f"leaf metric -M {mode} synthetic n={n} leaves={leaf_n}", flush=True)
+    print(f"leaf metric -M {mode} synthetic n={n} leaves={leaf_n} tiles", flush=True)
     a = _synthetic(5 + n, max(n, 32), leaf_n)[:n]
-    with cms.Context(0, metric=mode) as c:
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=mode) as c:
         c.set_tree(_leaf_tree(cms, leaf_n))
         c.set_samples(a)
         got = c.all_pairs()
@@ -172,7 +203,50 @@ def test_gpu_synthetic_vs_fp64(cms, orc, gpu_ok, mode, n, leaf_n):
 
 
 @pytest.mark.gpu
-def test_gpu_leaf_metrics_csr_and_guards(cms, orc, gpu_ok):
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,leaf_n", [(2, 40), (45, 33), (203, 3000), (513, 20000)])
+def test_gpu_merge_synthetic_vs_fp32_oracle(cms, orc, gpu_ok, mode, n, leaf_n):
+    """The merge kernels on the synthetic sets (degenerate samples, a mass far
+    below the rest, a single last leaf) against the oracle's FP32
+    restatement of dist.h; a second call and a kernel switch in between give
+    the same bits."""
+    print(f"leaf metric -M {mode} synthetic n={n} leaves={leaf_n} merge", flush=True)
+    a = _synthetic(5 + n, max(n, 32), leaf_n)[:n]
+    with cms.Context(0, metric=mode) as c:
+        c.set_tree(_leaf_tree(cms, leaf_n))
+        c.set_samples(a)
+        got = c.all_pairs()
+        c.set_opts(kernel=cms.KERNEL_SHAREDMIN, metric=mode)
+        c.all_pairs()
+        c.set_opts(kernel=cms.KERNEL_WALK, metric=mode)
+        got2 = c.all_pairs()
+    i, j = np.triu_indices(n, 1)
+    s32, _ = orc.pairs_leaf(mode, a, i.astype(np.int32), j.astype(np.int32))
+    _check_reference_order(got, s32, mode)
+    np.testing.assert_array_equal(got.view(np.uint32), got2.view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [2, 5])
+def test_gpu_merge_switches_layout(cms, orc, gpu_ok, mode):
+    """Cosine merges the values and the others a / sum a: switching between
+    them rebuilds the entry lists."""
+    a = _synthetic(17, 64, 900)
+    i, j = np.triu_indices(64, 1)
+    with cms.Context(0, metric=mode) as c:
+        c.set_tree(_leaf_tree(cms, 900))
+        c.set_samples(a)
+        c.all_pairs()
+        other = 5 if mode == 2 else 2
+        c.set_opts(kernel=cms.KERNEL_WALK, metric=other)
+        got = c.all_pairs()
+    s32, _ = orc.pairs_leaf(other, a, i.astype(np.int32), j.astype(np.int32))
+    np.testing.assert_array_equal(got.view(np.uint32), s32.view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_gpu_leaf_metrics_csr_and_guards(cms, orc, gpu_ok, kernel):
     """CSR input equals dense input bit for bit; entry points other than
     cms_all_pairs(_device) reject -M 2..5 with an error; negative abundances
     are rejected."""
@@ -182,7 +256,7 @@ def test_gpu_leaf_metrics_csr_and_guards(cms, orc, gpu_ok):
     rowptr[1:] = np.cumsum([len(x) for x in nz])
     col = np.concatenate(nz).astype(np.int32)
     val = np.concatenate([a[k, x] for k, x in enumerate(nz)]).astype(np.float32)
-    with cms.Context(0, metric=cms.METRIC_JSD) as c:
+    with cms.Context(0, kernel=_kernel(cms, kernel), metric=cms.METRIC_JSD) as c:
         c.set_tree(_leaf_tree(cms, 500))
         c.set_samples(a)
         dense = c.all_pairs()
@@ -206,15 +280,42 @@ def _dist_values(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", MODES)
-def test_cuda_comp_leaf_metric(cms, gpu_ok, golden, tmp_path, mode):
-    """cuda-comp -M m prints 1 - sim of the reference's values (6 digits)."""
-    print(f"cuda-comp -M {mode}", flush=True)
-    g = golden("core")
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("mode", EXACT)
+def test_cuda_comp_leaf_metric_bytes(cms, gpu_ok, golden, tmp_path, case, mode):
+    """cuda-comp -M 2/3/5 (default kernel) writes the .dist file that the
+    reference's values give through the .dist writer, byte for byte."""
+    print(f"cuda-comp -M {mode} {case} bytes", flush=True)
+    g = golden(case)
     exe = os.path.join(cms.BIN_DIR, "cuda-comp")
     out = str(tmp_path / "o.dist")
     env = dict(os.environ, ParallelMETA=g.root)
     r = subprocess.run([exe, "-T", g.table, "-D", "G", "-o", out, "-M", str(mode)], env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    names = cms.Table(g.table).sample_names()
+    n = len(names)
+    i, j, s = _hex(g, mode)
+    packed = np.zeros(n * (n - 1) // 2, np.float32)
+    packed[cms.packed_index(i, j, n)] = s
+    assert len(i) == packed.size
+    exp = str(tmp_path / "exp.dist")
+    cms.write_dist(exp, names, packed, False)
+    assert open(out, "rb").read() == open(exp, "rb").read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("kernel", ["walk", "sharedmin"])
+def test_cuda_comp_leaf_metric(cms, gpu_ok, golden, tmp_path, mode, kernel):
+    """cuda-comp -M m prints 1 - sim of the reference's values (6 digits),
+    with either kernel."""
+    print(f"cuda-comp -M {mode} --kernel {kernel}", flush=True)
+    g = golden("core")
+    exe = os.path.join(cms.BIN_DIR, "cuda-comp")
+    out = str(tmp_path / "o.dist")
+    env = dict(os.environ, ParallelMETA=g.root)
+    r = subprocess.run([exe, "-T", g.table, "-D", "G", "-o", out, "-M", str(mode), "--kernel", kernel], env=env,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     hdr, names, d = _dist_values(out)
