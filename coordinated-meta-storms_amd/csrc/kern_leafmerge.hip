@@ -27,6 +27,7 @@ namespace {
 
 constexpr uint32_t kEnd = 0xffffffffu;  // past the last leaf of a list (leaf ids are < 2^31)
 constexpr int kMergeWavesPerBlock = 4;
+constexpr int kMergeK = 32;  // entries of each list staged per window (one per lane of a half wave)
 
 // One thread per sample: sum of a and sum of a^2 in FP32, leaf order
 // (dist.h:20-30 and 49-57), and the entry list (leaf, a) or (leaf, a / sum a)
@@ -69,10 +70,19 @@ struct MergeAcc {
             if (d < 0.0f) d = -d;
             g = g + d;
         } else {
-            if (p == 0.0f && q == 0.0f) return;  // dist.h:107-118
-            const float h = __fdiv_rn(p + q, 2.0f);
-            if (p > 0.0f) f = f + p * logf(__fdiv_rn(p, h));
-            if (q > 0.0f) g = g + q * logf(__fdiv_rn(q, h));
+            // dist.h:107-118. x / 2 is x * 0.5 (one rounding of the same
+            // value). Every lane takes one term (p's, or q's when p is not
+            // positive) and only lanes where both are positive a second one,
+            // so a wave evaluates one logf per step unless some lane's leaf
+            // is shared; each sum still gets its terms in leaf order.
+            if (p == 0.0f && q == 0.0f) return;
+            const float h = (p + q) * 0.5f;
+            const bool pp = p > 0.0f, qp = q > 0.0f;
+            const float x = pp ? p : q;
+            const float t = x * logf(__fdiv_rn(x, h));
+            if (pp) f = f + t;
+            else if (qp) g = g + t;
+            if (pp && qp) g = g + q * logf(__fdiv_rn(q, h));
         }
     }
 };
@@ -121,63 +131,130 @@ __device__ __forceinline__ float finish(const MergeAcc<MODE> &acc, float2 xa, fl
     }
 }
 
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Upper-triangle tiles of 8 x 8 pairs over the visiting order (samples with
 // the most entries first, so the lanes of a wave merge lists of similar
-// length), one lane per pair, grid-stride over tiles.
+// length), one lane per pair, grid-stride over tiles. The tile's 16 entry
+// lists (8 row samples, 8 column samples) are staged into LDS kMergeK
+// entries at a time with coalesced loads. A window ends below the smallest
+// leaf that some list has not staged yet (lim); every lane merges its two
+// lists up to lim from LDS, and each list then advances by its staged
+// entries below lim, so the windows split the leaf axis and every lane sees
+// its pair's union in ascending leaf order. The list whose last staged entry
+// set lim advances by kMergeK entries, so the windows always progress; when
+// every list is staged to its end, lim is past all leaves and the tile ends.
 template <int MODE>
 __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(LeafMergeData d, int64_t nb,
                                                                               int64_t t_count,
                                                                               float *__restrict__ out) {
-    const int lane = threadIdx.x & 63;
+    __shared__ uint2 s_ent[kMergeWavesPerBlock][16 * kMergeK];
+    __shared__ int64_t s_pos[kMergeWavesPerBlock][16], s_end[kMergeWavesPerBlock][16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint2 *sbuf = s_ent[wave];
+    int64_t *lpos = s_pos[wave], *lend = s_end[wave];
+    const int r = lane >> 3, c = lane & 7;
     const int64_t stride = (int64_t)gridDim.x * kMergeWavesPerBlock;
-    for (int64_t t = (int64_t)blockIdx.x * kMergeWavesPerBlock + (threadIdx.x >> 6); t < t_count; t += stride) {
+    for (int64_t t = (int64_t)blockIdx.x * kMergeWavesPerBlock + wave; t < t_count; t += stride) {
         const double nn = (double)nb;
         int64_t ra = (int64_t)(((2.0 * nn + 1.0) - sqrt((2.0 * nn + 1.0) * (2.0 * nn + 1.0) - 8.0 * (double)t)) * 0.5);
         if (ra < 0) ra = 0;
         while (ra > 0 && ra * nb - ra * (ra - 1) / 2 > t) ra--;
         while ((ra + 1) * nb - (ra + 1) * ra / 2 <= t) ra++;
         const int64_t cb = ra + (t - (ra * nb - ra * (ra - 1) / 2));
-        const int64_t pa = ra * 8 + (lane >> 3), pb = cb * 8 + (lane & 7);
-        if (pa >= pb || pb >= d.n) continue;
-        int sa = d.perm[pa], sb = d.perm[pb];
-        if (sa > sb) {
-            const int x = sa;
-            sa = sb;
-            sb = x;
+        // list l < 8: visiting position 8 ra + l; l >= 8: 8 cb + l - 8
+        if (lane < 16) {
+            const int64_t p = lane < 8 ? ra * 8 + lane : cb * 8 + (lane - 8);
+            int64_t b = 0, e = 0;
+            if (p < d.n) {
+                const int s = d.perm[p];
+                b = d.rowptr[s];
+                e = d.rowptr[s + 1];
+            }
+            lpos[lane] = b;
+            lend[lane] = e;
         }
-        const float2 xa = d.stat[sa], xb = d.stat[sb];
-        float dist;
-        if (!degenerate<MODE>(xa, xb, dist)) {
-            const int64_t a0 = d.rowptr[sa], b0 = d.rowptr[sb];
-            const uint2 *A = d.ent + a0, *B = d.ent + b0;
-            const int64_t na = d.rowptr[sa + 1] - a0, nbb = d.rowptr[sb + 1] - b0;
-            // current entry of each list and the one after it, loaded ahead
-            uint2 ea = na > 0 ? A[0] : make_uint2(kEnd, 0u), eb = nbb > 0 ? B[0] : make_uint2(kEnd, 0u);
-            uint2 fa = na > 1 ? A[1] : make_uint2(kEnd, 0u), fb = nbb > 1 ? B[1] : make_uint2(kEnd, 0u);
-            int64_t i = 1, j = 1;
-            MergeAcc<MODE> acc;
-            for (;;) {
-                const uint32_t m = min(ea.x, eb.x);
-                if (m == kEnd) break;
-                const bool ta = ea.x == m, tb = eb.x == m;
-                acc.step(ta ? __uint_as_float(ea.y) : 0.0f, tb ? __uint_as_float(eb.y) : 0.0f);
-                if (ta) {
-                    ea = fa;
-                    ++i;
-                    fa = i < na ? A[i] : make_uint2(kEnd, 0u);
-                }
-                if (tb) {
-                    eb = fb;
-                    ++j;
-                    fb = j < nbb ? B[j] : make_uint2(kEnd, 0u);
+        wave_sync();
+        // this lane's pair: list r (sample a) and list 8 + c (sample b)
+        const int64_t pa = ra * 8 + r, pb = cb * 8 + c;
+        const bool active = pa < pb && pb < d.n;
+        int sa = 0, sb = 0;
+        float dist = 0.0f;
+        bool merge = false;
+        float2 xa = make_float2(0.0f, 0.0f), xb = make_float2(0.0f, 0.0f);
+        if (active) {
+            sa = d.perm[pa];
+            sb = d.perm[pb];
+            xa = d.stat[sa];
+            xb = d.stat[sb];
+            merge = !degenerate<MODE>(xa, xb, dist);
+        }
+        const uint2 *A = sbuf + r * kMergeK, *B = sbuf + (8 + c) * kMergeK;
+        MergeAcc<MODE> acc;
+        for (;;) {
+            // stage: lists 2k (lanes 0-31) and 2k + 1 (lanes 32-63), entry lane & 31
+            uint32_t cap = kEnd;
+#pragma unroll 4
+            for (int k = 0; k < 8; k++) {
+                const int l = 2 * k + (lane >> 5), e = lane & 31;
+                const int64_t p = lpos[l] + e, pe = lend[l];
+                const uint2 v = p < pe ? d.ent[p] : make_uint2(kEnd, 0u);
+                sbuf[l * kMergeK + e] = v;
+                if (e == kMergeK - 1 && p + 1 < pe) cap = min(cap, v.x + 1u);
+            }
+            for (int off = 32; off > 0; off >>= 1) cap = min(cap, (uint32_t)__shfl_xor((int)cap, off, 64));
+            const uint32_t lim = cap;
+            wave_sync();
+            // every list advances by its staged entries below lim (read by
+            // the next window's staging, after this window's merge)
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int l = 2 * k + (lane >> 5), e = lane & 31;
+                const unsigned long long below = __ballot(sbuf[l * kMergeK + e].x < lim);
+                if (lane == 0) {
+                    lpos[2 * k] += __popcll(below & 0xffffffffull);
+                    lpos[2 * k + 1] += __popcll(below >> 32);
                 }
             }
-            dist = finish<MODE>(acc, xa, xb);
+            wave_sync();
+            if (merge) {
+                int ia = 0, ib = 0;
+                uint2 ea = A[0], eb = B[0];
+                for (;;) {
+                    const uint32_t m = min(ea.x, eb.x);
+                    if (m >= lim) break;
+                    const bool ta = ea.x == m, tb = eb.x == m;
+                    acc.step(ta ? __uint_as_float(ea.y) : 0.0f, tb ? __uint_as_float(eb.y) : 0.0f);
+                    if (ta) {
+                        ++ia;
+                        ea = ia < kMergeK ? A[ia] : make_uint2(kEnd, 0u);
+                    }
+                    if (tb) {
+                        ++ib;
+                        eb = ib < kMergeK ? B[ib] : make_uint2(kEnd, 0u);
+                    }
+                }
+            }
+            wave_sync();
+            if (lim == kEnd) break;
         }
-        // an x86 invalid operation gives the default NaN (0xffc00000) and
-        // 1.0 - NaN keeps it; the device's arithmetic returns another NaN
-        out[(int64_t)sa * d.n + sb - ((int64_t)sa + 1) * ((int64_t)sa + 2) / 2] =
-            dist != dist ? __uint_as_float(0xffc00000u) : __double2float_rn(1.0 - (double)dist);
+        if (active) {
+            if (merge) dist = finish<MODE>(acc, xa, xb);
+            if (sa > sb) {
+                const int x = sa;
+                sa = sb;
+                sb = x;
+            }
+            // an x86 invalid operation gives the default NaN (0xffc00000) and
+            // 1.0 - NaN keeps it; the device's arithmetic returns another NaN
+            out[(int64_t)sa * d.n + sb - ((int64_t)sa + 1) * ((int64_t)sa + 2) / 2] =
+                dist != dist ? __uint_as_float(0xffc00000u) : __double2float_rn(1.0 - (double)dist);
+        }
+        wave_sync();
     }
 }
 

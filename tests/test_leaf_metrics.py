@@ -227,6 +227,31 @@ def test_gpu_merge_synthetic_vs_fp32_oracle(cms, orc, gpu_ok, mode, n, leaf_n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("wl", [1, 3])
+def test_gpu_merge_workload_trees(cms, orc, gpu_ok, wl):
+    """The merge kernels at the workloads' shape: 99,322 leaves, 512 samples
+    of workload 1 (200-2,000 entries) and workload 3 (log-uniform 50-20,000
+    entries, so the visiting order and ragged list lengths matter); 3,000
+    random pairs of the triangle against the oracle's FP32 restatement."""
+    print(f"leaf merge workload {wl} 512 samples", flush=True)
+    tree, _, _, abd = cms.workload(wl, samples=512)
+    dense = abd.dense(tree.leaf_n)
+    n = dense.shape[0]
+    rng = np.random.default_rng(40 + wl)
+    i = rng.integers(0, n - 1, 3000)
+    j = (i + 1 + rng.integers(0, n - 1 - i)).astype(np.int64)
+    p = cms.packed_index(i, j, n)
+    with cms.Context(0, metric=2) as c:
+        c.set_tree(_leaf_tree(cms, tree.leaf_n))
+        c.set_samples_csr(abd)
+        for mode in MODES:
+            c.set_opts(kernel=cms.KERNEL_WALK, metric=mode)
+            got = c.all_pairs()
+            s32, _ = orc.pairs_leaf(mode, dense, i.astype(np.int32), j.astype(np.int32))
+            _check_reference_order(got[p], s32, mode)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", [2, 5])
 def test_gpu_merge_switches_layout(cms, orc, gpu_ok, mode):
     """Cosine merges the values and the others a / sum a: switching between
