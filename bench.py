@@ -320,7 +320,7 @@ def run_serving(cms, a):
     th = threading.Thread(target=submitter)
     t_run = time.perf_counter()
     th.start()
-    lat, queue_s, service_s, pairs_done = [], [], [], 0
+    lat, queue_s, service_s, compute_s, deliver_s, pairs_done = [], [], [], [], [], 0
     checked = []
     k = 0
     while True:
@@ -331,6 +331,9 @@ def run_serving(cms, a):
             lat.append(i.t_done - i.t_submit)
             queue_s.append(i.t_join - i.t_submit)
             service_s.append(i.t_done - i.t_join)
+            if i.t_computed >= 0:
+                compute_s.append(i.t_computed - i.t_join)
+                deliver_s.append(i.t_done - i.t_computed)
             pairs_done += int(i.nq) * n_ref
             if len(checked) < 3:
                 checked.append((k, r.out))
@@ -378,7 +381,13 @@ def run_serving(cms, a):
                          "p99": float(np.percentile(lat, 99)) if served else None,
                          "mean": float(lat.mean()) if served else None, "max": float(lat.max()) if served else None},
            "queue_s_p50": float(np.percentile(queue_s, 50)) if served else None,
+           "queue_s_p99": float(np.percentile(queue_s, 99)) if served else None,
            "service_s_p50": float(np.percentile(service_s, 50)) if served else None,
+           "service_s_p99": float(np.percentile(service_s, 99)) if served else None,
+           "compute_s_p50_p99": [float(np.percentile(compute_s, 50)), float(np.percentile(compute_s, 99))]
+           if compute_s else None,
+           "delivery_s_p50_p99": [float(np.percentile(deliver_s, 50)), float(np.percentile(deliver_s, 99))]
+           if deliver_s else None,
            "wall_s": wall, "prep_s": prep_s, "steps": st.steps - st0.steps, "kernel_ms": kernel_ms,
            "kernel_pairs_per_s": pairs_done / (kernel_ms * 1e-3) if kernel_ms > 0 else None,
            "arena_peak": st.arena_peak, "slots_peak": st.slots_peak,

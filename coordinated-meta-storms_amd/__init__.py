@@ -78,7 +78,8 @@ class ServerOpts(C.Structure):
 
 class RequestInfo(C.Structure):
     _fields_ = [("rc", C.c_int), ("nq", C.c_int), ("join_step", C.c_int64), ("t_submit", C.c_double),
-                ("t_admit", C.c_double), ("t_join", C.c_double), ("t_done", C.c_double)]
+                ("t_admit", C.c_double), ("t_join", C.c_double), ("t_done", C.c_double),
+                ("t_computed", C.c_double)]
 
 
 class ServerStats(C.Structure):

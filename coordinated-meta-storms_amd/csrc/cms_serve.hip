@@ -84,7 +84,7 @@ struct ReqHost {
     int rc = CMS_OK;
     std::string err;
     hipEvent_t ev_embedded = nullptr, ev_packed = nullptr, ev_last = nullptr;
-    double t_submit = 0, t_admit = -1, t_join = -1, t_done = -1;
+    double t_submit = 0, t_admit = -1, t_join = -1, t_computed = -1, t_done = -1;
     int64_t join_step = -1;
     bool finished = false;
 };
@@ -119,6 +119,11 @@ struct cms_server {
     uint32_t *h_masks[kRing] = {};
     hipEvent_t ring_begin[kRing] = {}, ring_end[kRing] = {};
     hipStream_t s_embed = nullptr, s_comp = nullptr, s_copy = nullptr;
+    // delivery to pageable caller memory: two pinned staging buffers, filled
+    // by DMA on s_copy and drained by a parallel host copy, alternately
+    static constexpr size_t kStageBytes = (size_t)64 << 20;
+    uint8_t *stage[2] = {};
+    hipEvent_t ev_stage[2] = {};
 
     // threads
     std::thread worker, deliverer;
@@ -130,6 +135,7 @@ struct cms_server {
     std::vector<int64_t> embedding_ids, packing_ids;
     std::deque<int64_t> deliver_q;
     std::deque<int> inflight;  // ring entries of queued steps, in launch order
+    std::deque<std::vector<int64_t>> inflight_finishers;  // per queued step: sets whose last band it is
     std::string fatal;         // device error that stopped the loop
 
     // statistics
@@ -403,6 +409,12 @@ void worker_main(cms_server *s) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, s->ring_begin[k], s->ring_end[k]) == hipSuccess) s->kernel_ms += ms;
             s->inflight.pop_front();
+            const double t = s->now();
+            for (int64_t id : s->inflight_finishers.front()) {
+                auto it = s->reqs.find(id);
+                if (it != s->reqs.end() && !it->second.finished) it->second.t_computed = t;
+            }
+            s->inflight_finishers.pop_front();
             progress = true;
         }
         if (!s->fatal.empty()) break;
@@ -429,6 +441,7 @@ void worker_main(cms_server *s) {
                 }
                 s->steps++;
                 s->inflight.push_back(L.ring);
+                s->inflight_finishers.push_back(L.plan.finishers);
                 for (auto &j : L.joins) s->packing_ids.push_back(j.id);
                 for (int64_t id : L.plan.finishers) s->deliver_q.push_back(id);
                 if (!L.plan.finishers.empty()) s->cv_deliver.notify_one();
@@ -449,6 +462,39 @@ void worker_main(cms_server *s) {
     s->cv_deliver.notify_all();
 }
 
+// Device arena -> caller's host buffer. Pinned (or registered) memory takes
+// one DMA; pageable memory goes through the two pinned staging buffers: the
+// DMA of piece i + 1 overlaps the parallel host copy of piece i.
+hipError_t deliver(cms_server *s, float *dst, const uint8_t *src, size_t bytes) {
+    hipPointerAttribute_t attr{};
+    const bool pinned = hipPointerGetAttributes(&attr, dst) == hipSuccess && attr.type == hipMemoryTypeHost;
+    (void)hipGetLastError();  // pageable memory is reported as an error; do not leave it pending
+    hipError_t e;
+    if (pinned || bytes == 0) {
+        e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->s_copy);
+        return e == hipSuccess ? hipStreamSynchronize(s->s_copy) : e;
+    }
+    const size_t piece = cms_server::kStageBytes;
+    const size_t n = (bytes + piece - 1) / piece;
+    auto issue = [&](size_t i) {
+        const size_t len = std::min(piece, bytes - i * piece);
+        hipError_t r = hipMemcpyAsync(s->stage[i & 1], src + i * piece, len, hipMemcpyDeviceToHost, s->s_copy);
+        return r == hipSuccess ? hipEventRecord(s->ev_stage[i & 1], s->s_copy) : r;
+    };
+    e = issue(0);
+    const int threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    for (size_t i = 0; i < n && e == hipSuccess; i++) {
+        // staging buffer (i + 1) & 1 was drained by the host copy of piece i - 1
+        if (i + 1 < n) e = issue(i + 1);
+        if (e == hipSuccess) e = hipEventSynchronize(s->ev_stage[i & 1]);
+        if (e == hipSuccess)
+            parallel_copy(reinterpret_cast<uint8_t *>(dst) + i * piece, s->stage[i & 1],
+                          std::min(piece, bytes - i * piece), threads);
+    }
+    if (e != hipSuccess) (void)hipStreamSynchronize(s->s_copy);
+    return e;
+}
+
 void delivery_main(cms_server *s) {
     (void)hipSetDevice(s->device);
     std::unique_lock<std::mutex> lk(s->mu);
@@ -466,8 +512,7 @@ void delivery_main(cms_server *s) {
         float *dst = r.out;
         lk.unlock();
         hipError_t e = hipEventSynchronize(ev);
-        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->s_copy);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->s_copy);
+        if (e == hipSuccess) e = deliver(s, dst, src, bytes);
         lk.lock();
         if (e != hipSuccess) {
             fatal_locked(s, hip_msg("delivery", e));
@@ -510,6 +555,10 @@ void free_server(cms_server *s) {
     if (s->s_embed) (void)hipStreamDestroy(s->s_embed);
     if (s->s_comp) (void)hipStreamDestroy(s->s_comp);
     if (s->s_copy) (void)hipStreamDestroy(s->s_copy);
+    for (int k = 0; k < 2; k++) {
+        if (s->stage[k]) (void)hipHostFree(s->stage[k]);
+        if (s->ev_stage[k]) (void)hipEventDestroy(s->ev_stage[k]);
+    }
     if (s->ctx) s->ctx->servers--;
     delete s;
 }
@@ -623,6 +672,10 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     CREATE_HIP(hipStreamCreateWithFlags(&s->s_embed, hipStreamNonBlocking));
     CREATE_HIP(hipStreamCreateWithFlags(&s->s_comp, hipStreamNonBlocking));
     CREATE_HIP(hipStreamCreateWithFlags(&s->s_copy, hipStreamNonBlocking));
+    for (int k = 0; k < 2; k++) {
+        CREATE_HIP(hipHostMalloc((void **)&s->stage[k], cms_server::kStageBytes, hipHostMallocDefault));
+        CREATE_HIP(hipEventCreateWithFlags(&s->ev_stage[k], hipEventDisableTiming));
+    }
     // the references' layout was built on the context stream
     CREATE_HIP(hipStreamSynchronize(ref->stream));
 
@@ -745,6 +798,7 @@ int cms_server_wait(cms_server *s, int64_t id, double timeout_s, cms_request_inf
         info->t_admit = r.t_admit;
         info->t_join = r.t_join;
         info->t_done = r.t_done;
+        info->t_computed = r.t_computed;
     }
     const int rc = r.rc;
     if (rc != CMS_OK) set_global_error(r.err);

@@ -2,6 +2,7 @@
 #include "serve_sched.h"
 
 #include <algorithm>
+#include <cstring>
 
 namespace cms {
 
@@ -254,5 +255,17 @@ const SchedRequest *Scheduler::get(int64_t id) const {
 }
 
 size_t Scheduler::active() const { return active_.size(); }
+
+void parallel_copy(void *dst, const void *src, size_t bytes, int threads) {
+    constexpr size_t kPiece = (size_t)4 << 20;
+    const int64_t pieces = (int64_t)((bytes + kPiece - 1) / kPiece);
+    char *d = static_cast<char *>(dst);
+    const char *s = static_cast<const char *>(src);
+#pragma omp parallel for schedule(static) num_threads(threads) if (pieces > 1)
+    for (int64_t k = 0; k < pieces; k++) {
+        const size_t off = (size_t)k * kPiece;
+        std::memcpy(d + off, s + off, std::min(kPiece, bytes - off));
+    }
+}
 
 }  // namespace cms

@@ -119,4 +119,10 @@ private:
     int64_t next_step_ = 0;
 };
 
+// Host memcpy split over up to `threads` OpenMP threads (4 MiB pieces): the
+// serving loop's delivery from its pinned staging buffers into callers'
+// pageable output, where one thread's copy and first-touch page faults set
+// the pace.
+void parallel_copy(void *dst, const void *src, size_t bytes, int threads);
+
 }  // namespace cms

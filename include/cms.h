@@ -223,6 +223,8 @@ typedef struct cms_request_info {
     double t_admit;        /* slots and memory granted, embedding queued */
     double t_join;         /* first step launched */
     double t_done;         /* output delivered to the caller's buffer */
+    double t_computed;     /* last step finished on the device (seen by the server within ~0.1 ms; -1 if
+                              the set was delivered before the server saw it) */
 } cms_request_info;
 
 typedef struct cms_server_stats {

@@ -127,6 +127,8 @@ def test_join_mid_cycle_overlaps(cms, data):
     assert ib.join_step >= ia.join_step
     assert ia.t_submit <= ia.t_admit <= ia.t_join <= ia.t_done
     assert ib.t_submit <= ib.t_admit <= ib.t_join <= ib.t_done
+    for i in (ia, ib):  # device completion, when the server saw it before delivery
+        assert i.t_computed == -1.0 or i.t_join <= i.t_computed <= i.t_done
 
 
 def test_submit_errors_and_readonly_context(cms, data):
