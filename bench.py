@@ -53,6 +53,15 @@ def timed_steps(ctx, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def cpu_model() -> str:
+    """The host CPU's model name and logical CPU count (/proc/cpuinfo)."""
+    try:
+        names = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs visible, {len(os.sched_getaffinity(0))} usable)"
+    except (OSError, IndexError):
+        return "unknown"
+
+
 def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, sample_n, pairs):
     """Reference Calc_sim timed on the host cores over a bounded pair sample,
     plus the accuracy spot-check of the GPU results on pairs of that sample."""
@@ -95,6 +104,7 @@ def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, s
         sec = time.perf_counter() - t0
         res = {"value": len(got) / sec, "unit": "sample-pairs/s", "cores": threads, "kind": "port",
                "sample": f"first {len(got)} pairs of {sample_n} samples, oracle FP32 walk, {threads} threads"}
+    res["cpu_model"] = cpu_model()
     # accuracy of the GPU outputs on 2,000 pairs of the sample vs the oracle
     rng = np.random.default_rng(7)
     ii = rng.integers(0, sample_n, 4000)
