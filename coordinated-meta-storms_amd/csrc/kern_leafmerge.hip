@@ -28,6 +28,11 @@ namespace {
 constexpr uint32_t kEnd = 0xffffffffu;  // past the last leaf of a list (leaf ids are < 2^31)
 constexpr int kMergeWavesPerBlock = 4;
 constexpr int kMergeK = 32;  // entries of each list staged per window (one per lane of a half wave)
+// LDS stride of a staged list: one spare entry after the kMergeK staged ones.
+// It holds a sentinel (leaf kEnd), so a merge reads a list's next entry
+// without a bounds check, and it shifts list l to bank 2 l, so lanes reading
+// the same position of different lists do not conflict.
+constexpr int kMergeStride = kMergeK + 1;
 
 // One thread per sample: sum of a and sum of a^2 in FP32, leaf order
 // (dist.h:20-30 and 49-57), and the entry list (leaf, a) or (leaf, a / sum a)
@@ -152,12 +157,13 @@ template <int MODE>
 __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(LeafMergeData d, int64_t nb,
                                                                               int64_t t_count,
                                                                               float *__restrict__ out) {
-    __shared__ uint2 s_ent[kMergeWavesPerBlock][16 * kMergeK];
+    __shared__ uint2 s_ent[kMergeWavesPerBlock][16 * kMergeStride];
     __shared__ int64_t s_pos[kMergeWavesPerBlock][16], s_end[kMergeWavesPerBlock][16];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint2 *sbuf = s_ent[wave];
     int64_t *lpos = s_pos[wave], *lend = s_end[wave];
     const int r = lane >> 3, c = lane & 7;
+    if (lane < 16) sbuf[lane * kMergeStride + kMergeK] = make_uint2(kEnd, 0u);  // never overwritten
     const int64_t stride = (int64_t)gridDim.x * kMergeWavesPerBlock;
     for (int64_t t = (int64_t)blockIdx.x * kMergeWavesPerBlock + wave; t < t_count; t += stride) {
         const double nn = (double)nb;
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
             xb = d.stat[sb];
             merge = !degenerate<MODE>(xa, xb, dist);
         }
-        const uint2 *A = sbuf + r * kMergeK, *B = sbuf + (8 + c) * kMergeK;
+        const uint2 *A = sbuf + r * kMergeStride, *B = sbuf + (8 + c) * kMergeStride;
         MergeAcc<MODE> acc;
         for (;;) {
             // stage: lists 2k (lanes 0-31) and 2k + 1 (lanes 32-63), entry lane & 31
@@ -203,7 +209,7 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
                 const int l = 2 * k + (lane >> 5), e = lane & 31;
                 const int64_t p = lpos[l] + e, pe = lend[l];
                 const uint2 v = p < pe ? d.ent[p] : make_uint2(kEnd, 0u);
-                sbuf[l * kMergeK + e] = v;
+                sbuf[l * kMergeStride + e] = v;
                 if (e == kMergeK - 1 && p + 1 < pe) cap = min(cap, v.x + 1u);
             }
             for (int off = 32; off > 0; off >>= 1) cap = min(cap, (uint32_t)__shfl_xor((int)cap, off, 64));
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const int l = 2 * k + (lane >> 5), e = lane & 31;
-                const unsigned long long below = __ballot(sbuf[l * kMergeK + e].x < lim);
+                const unsigned long long below = __ballot(sbuf[l * kMergeStride + e].x < lim);
                 if (lane == 0) {
                     lpos[2 * k] += __popcll(below & 0xffffffffull);
                     lpos[2 * k + 1] += __popcll(below >> 32);
@@ -222,21 +228,22 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
             }
             wave_sync();
             if (merge) {
+                // a list never advances past a sentinel (leaf kEnd >= lim), so
+                // ia, ib <= kMergeK
                 int ia = 0, ib = 0;
-                uint2 ea = A[0], eb = B[0];
                 for (;;) {
+                    const uint2 ea = A[ia], eb = B[ib];
+                    // the values are tied here so that each entry is one
+                    // 64-bit LDS read (not a leaf read, then a value read
+                    // after the exit test)
+                    uint32_t va = ea.y, vb = eb.y;
+                    asm volatile("" : "+v"(va), "+v"(vb));
                     const uint32_t m = min(ea.x, eb.x);
                     if (m >= lim) break;
                     const bool ta = ea.x == m, tb = eb.x == m;
-                    acc.step(ta ? __uint_as_float(ea.y) : 0.0f, tb ? __uint_as_float(eb.y) : 0.0f);
-                    if (ta) {
-                        ++ia;
-                        ea = ia < kMergeK ? A[ia] : make_uint2(kEnd, 0u);
-                    }
-                    if (tb) {
-                        ++ib;
-                        eb = ib < kMergeK ? B[ib] : make_uint2(kEnd, 0u);
-                    }
+                    acc.step(ta ? __uint_as_float(va) : 0.0f, tb ? __uint_as_float(vb) : 0.0f);
+                    ia += ta;
+                    ib += tb;
                 }
             }
             wave_sync();
