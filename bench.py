@@ -492,8 +492,9 @@ def main():
         others[f"M{m}"] = {"ms_per_step": osec * 1e3, "pairs_per_s": npairs / osec, "kernel_ms": ost.kernel_ms,
                            "prep_ms": ost.prep_ms}
         log(f"-M {m}: {osec * 1e3:.2f} ms/step, kernel {ost.kernel_ms:.2f} ms, prep {ost.prep_ms:.2f} ms")
-        if m >= 2:
-            # the same metric in the reference's FP32 operation order (per-pair merge)
+        if m >= 1:
+            # the same metric in the reference's FP32 operation order (the
+            # unweighted walk for -M 1, the per-pair merge for -M 2..5)
             with cms.Context(0, kernel=cms.KERNEL_WALK, metric=m) as oc:
                 oc.set_tree(tree)
                 oc.set_samples_csr(abd)
