@@ -370,6 +370,23 @@ int ensure_leaf_merge(cms_ctx *c) {
     return CMS_OK;
 }
 
+// JSD in the reference's arithmetic needs the host logf's exceptions on the
+// device (built once per process, uploaded once per context).
+int ensure_logf_table(cms_ctx *c) {
+    if (c->logf_ntab >= 0) return CMS_OK;
+    const LogfTable &t = host_logf_table();
+    HIPCALL(c, dalloc(c->d_logf_bm, t.bitmap.size()));
+    HIPCALL(c, dalloc(c->d_logf_blocks, t.blocks.size()));
+    HIPCALL(c, hipMemcpy(c->d_logf_blocks, t.blocks.data(), sizeof(uint32_t) * t.blocks.size(), hipMemcpyHostToDevice));
+    std::memcpy(&c->logf_ln2, &t.ln2_bits, 4);
+    HIPCALL(c, dalloc(c->d_logf_tab, std::max<size_t>(t.table.size(), 1)));
+    HIPCALL(c, hipMemcpy(c->d_logf_bm, t.bitmap.data(), sizeof(uint32_t) * t.bitmap.size(), hipMemcpyHostToDevice));
+    if (!t.table.empty())
+        HIPCALL(c, hipMemcpy(c->d_logf_tab, t.table.data(), sizeof(uint64_t) * t.table.size(), hipMemcpyHostToDevice));
+    c->logf_ntab = (int)t.table.size();
+    return CMS_OK;
+}
+
 LeafData leaf_data(const cms_ctx *c) {
     LeafData d{};
     d.P = c->d_leafP;
@@ -386,11 +403,13 @@ int prepare(cms_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (is_leaf_metric(c)) {
-        if (use_leaf_merge(c) ? c->lm_layout == (c->opts.metric == CMS_METRIC_COS ? 0 : 1) : c->leaf_ready)
-            return CMS_OK;
+        const bool merge_ready = c->lm_layout == (c->opts.metric == CMS_METRIC_COS ? 0 : 1) &&
+                                 (c->opts.metric != CMS_METRIC_JSD || c->logf_ntab >= 0);
+        if (use_leaf_merge(c) ? merge_ready : c->leaf_ready) return CMS_OK;
         if (c->has_negative) return fail(c, CMS_E_UNSUPPORTED, "-M 2..5 on the GPU need non-negative abundances");
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
         rc = use_leaf_merge(c) ? ensure_leaf_merge(c) : ensure_leaf_layout(c);
+        if (!rc && use_leaf_merge(c) && c->opts.metric == CMS_METRIC_JSD) rc = ensure_logf_table(c);
     } else if (use_sm(c)) {
         if (c->sm_ready && c->sm_metric == c->opts.metric && c->sm_binned == (c->opts.bin != 0)) return CMS_OK;
         HIPCALL(c, hipEventRecord(c->evp0, c->stream));
@@ -505,6 +524,9 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_work);
     dfree(c->d_perm);
     dfree(c->d_pos);
+    dfree(c->d_logf_bm);
+    dfree(c->d_logf_blocks);
+    dfree(c->d_logf_tab);
     if (c->band_exec) (void)hipGraphExecDestroy(c->band_exec);
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -805,7 +827,8 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
         const int64_t npairs = (int64_t)c->n * (c->n > 0 ? c->n - 1 : 0) / 2;
         HIPCALL(c, hipEventRecord(c->ev0, st));
         if (use_leaf_merge(c)) {
-            LeafMergeData md{c->d_rowptr, c->d_lment, c->d_lmstat, c->d_lmperm, c->n};
+            LeafMergeData md{c->d_rowptr, c->d_lment, c->d_lmstat, c->d_lmperm, c->n,
+                             c->d_logf_bm, c->d_logf_blocks, c->d_logf_tab, std::max(c->logf_ntab, 0), c->logf_ln2};
             HIPCALL(c, launch_leaf_merge_all_pairs(md, c->opts.metric, d_sim, npairs, st));
         } else {
             HIPCALL(c, launch_zero_counter(c->d_work, st));

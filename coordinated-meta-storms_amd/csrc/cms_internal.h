@@ -74,6 +74,20 @@ struct CompiledTree {
     std::vector<int32_t> top_nodes, top_src;
 };
 
+// The host C library's logf on (0, 4] (the JSD of the leaf merge kernels):
+// inputs whose logf is not log((double)x) rounded to float, or whose double
+// evaluation lies within a small margin of a rounding midpoint, with the
+// library's result. bitmap bit u is set for each such input bit pattern u;
+// table holds (u << 32 | result bits), ascending. Built once per process.
+constexpr uint32_t kLogfMaxBits = 0x40800000u;  // 4.0f
+struct LogfTable {
+    std::vector<uint32_t> bitmap;  // (kLogfMaxBits >> 5) + 1 words
+    std::vector<uint32_t> blocks;  // bit b: some input in [256 b, 256 b + 255] is listed
+    std::vector<uint64_t> table;
+    uint32_t ln2_bits = 0;         // logf(2.0f)
+};
+const LogfTable &host_logf_table();
+
 // Split a tree schedule into subtree segments (fills the seg_* and top_*
 // fields). Schedules whose subtrees are not contiguous get no segments: every
 // node is then a top node.

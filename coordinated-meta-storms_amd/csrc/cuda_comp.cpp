@@ -54,7 +54,7 @@ namespace {
     printf("\t  -r rRNA copy number correction, T(rue) or F(alse), default is T\n");
     printf("\t  -t Number of host threads for loading, default is auto\n");
     printf("\t  --kernel walk|sharedmin  walk (default): the reference's FP32 arithmetic per pair, bit-identical\n");
-    printf("\t                           to comp (-M 4: to the last bit of logf); sharedmin: the fast kernels\n");
+    printf("\t                           to comp; sharedmin: the fast kernels\n");
     printf("\t  --storage f32|bf16|f16   walk abundance storage, default f32\n");
     printf("\t  --device k               GPU index, default 0\n");
     printf("\t  --format dist|binary     text .dist (default) or packed float32 triangle + .names\n");

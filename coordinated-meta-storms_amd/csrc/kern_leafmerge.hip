@@ -10,8 +10,9 @@
 // the same FP32 values in the same order. Per-sample quantities that the
 // reference recomputes inside each call (sum of a, sum of a^2, a / sum a) are
 // computed once per sample with the same FP32 operations in leaf order.
-// Cosine, Euclidean and Bray-Curtis are bit-identical to the reference; JSD
-// calls the device logf, which may differ from the host's in the last bit.
+// All four are bit-identical to the reference: JSD's logf is the host C
+// library's, reproduced on the device from log in double and the host's list
+// of inputs where the two differ (host_logf.cpp).
 // All code here is built without FMA contraction; divisions and square roots
 // are the correctly rounded ones.
 #include <hip/hip_runtime.h>
@@ -58,10 +59,38 @@ __global__ __launch_bounds__(64) void leaf_merge_prepare_kernel(const int64_t *_
     stat[s] = make_float2(sum, sq);
 }
 
+// The host C library's logf on (0, 4]: log((double)x) rounded to float,
+// except for the inputs the host listed (host_logf_table: where its logf
+// rounds otherwise, or the double lies too close to a rounding midpoint),
+// whose results it gave.
+struct HostLogf {
+    const uint32_t *bm, *blocks;
+    const unsigned long long *tab;
+    int ntab;
+    float ln2;  // the host's logf(2.0f): a leaf that one sample misses gives x / h = 2
+    __device__ __forceinline__ float operator()(float x) const {
+        if (x == 2.0f) return ln2;
+        const uint32_t u = __float_as_uint(x), b = u >> 8;
+        if (u <= kLogfMaxBits && ((blocks[b >> 5] >> (b & 31u)) & 1u) && ((bm[u >> 5] >> (u & 31u)) & 1u)) {
+            int lo = 0, hi = ntab - 1;  // the listed input is in the table
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((uint32_t)(tab[mid] >> 32) < u)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            return __uint_as_float((uint32_t)tab[lo]);
+        }
+        return __double2float_rn(log((double)x));
+    }
+};
+
 // One leaf of the union, p and q being the two samples' values there (zero
 // where a sample has no entry).
 template <int MODE>
 struct MergeAcc {
+    HostLogf lg;
     float f = 0.0f, g = 0.0f;  // cos: f; Euclidean: f; Bray-Curtis: sum, diff; JSD: dkl_m, dkl_n
     __device__ __forceinline__ void step(float p, float q) {
         if constexpr (MODE == CMS_METRIC_COS) {
@@ -84,10 +113,10 @@ struct MergeAcc {
             const float h = (p + q) * 0.5f;
             const bool pp = p > 0.0f, qp = q > 0.0f;
             const float x = pp ? p : q;
-            const float t = x * logf(__fdiv_rn(x, h));
+            const float t = x * lg(__fdiv_rn(x, h));
             if (pp) f = f + t;
             else if (qp) g = g + t;
-            if (pp && qp) g = g + q * logf(__fdiv_rn(q, h));
+            if (pp && qp) g = g + q * lg(__fdiv_rn(q, h));
         }
     }
 };
@@ -201,6 +230,7 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
         }
         const uint2 *A = sbuf + r * kMergeStride, *B = sbuf + (8 + c) * kMergeStride;
         MergeAcc<MODE> acc;
+        acc.lg = HostLogf{d.logf_bm, d.logf_blocks, d.logf_tab, d.logf_ntab, d.logf_ln2};
         for (;;) {
             // stage: lists 2k (lanes 0-31) and 2k + 1 (lanes 32-63), entry lane & 31
             uint32_t cap = kEnd;

@@ -190,6 +190,12 @@ struct LeafMergeData {
     const float2 *stat;     // per sample: FP32 sum of a, FP32 sum of a^2, both in leaf order
     const int32_t *perm;    // visiting position -> sample (most entries first)
     int n;
+    // JSD: the host C library's logf exceptions (host_logf_table), on device
+    const uint32_t *logf_bm;
+    const uint32_t *logf_blocks;
+    const unsigned long long *logf_tab;
+    int logf_ntab;
+    float logf_ln2;  // the host's logf(2.0f)
 };
 // Fill ent (one per CSR entry) and stat (one per sample).
 hipError_t launch_leaf_merge_prepare(const int64_t *rowptr, const int32_t *col, const float *val, int n,

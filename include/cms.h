@@ -49,7 +49,7 @@ enum {
 enum {
     CMS_KERNEL_WALK = 0,     /* drop-in: the reference's FP32 arithmetic per pair, bit-exact with comp:
                                 the post-order walk (-M 0/1), a merge of the two leaf lists (-M 2..5;
-                                -M 4 to the last bit of logf) */
+                                -M 4 with the host C library's logf) */
     CMS_KERNEL_SHAREDMIN = 1 /* fast: decayed-mass embedding + fixed-point shared-minimum tiles (-M 0/1),
                                 leaf tiles with FP64 sums (-M 2..5) */
 };

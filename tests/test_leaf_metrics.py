@@ -7,10 +7,10 @@ on every golden pair (hex fixtures), its FP64 form agrees with it and with
 closed forms (identical, disjoint and degenerate samples).
 GPU, two kernels:
 - the merge kernels (CMS_KERNEL_WALK, cuda-comp's default) follow the
-  reference's FP32 operation order: cosine, Euclidean and Bray-Curtis equal
-  the reference (golden hex) and the oracle's FP32 restatement bit for bit,
-  and `cuda-comp -M 2/3/5` writes the reference's .dist bytes; JSD calls the
-  device logf, so it is held to JSD_TOL of the reference's values;
+  reference's FP32 operation order and reproduce the host C library's logf
+  for JSD: all four equal the reference (golden hex) and the oracle's FP32
+  restatement bit for bit, and `cuda-comp -M 2..5` writes the reference's
+  .dist bytes;
 - the leaf tiles (CMS_KERNEL_SHAREDMIN) are within 1e-5 of the FP64 oracle,
   with the reference's degenerate-sample semantics.
 """
@@ -24,11 +24,7 @@ import pytest
 from conftest import GOLDEN_CASES
 
 MODES = [2, 3, 4, 5]
-EXACT = (2, 3, 5)   # metrics without a transcendental call: bit-identical
 TOL = 1e-5
-# JSD in the reference's operation order, device logf against the host's
-# (each within about an ulp): far below the 6-digit print
-JSD_TOL = 2e-6
 KERNELS = ["merge", "tiles"]
 
 
@@ -124,15 +120,10 @@ def _kernel(cms, name):
 
 
 def _check_reference_order(got, s32, mode):
-    """Merge kernels against the reference's FP32 values: bit for bit for
-    cosine, Euclidean and Bray-Curtis; JSD within JSD_TOL (NaN bits equal)."""
+    """Merge kernels against the reference's FP32 values, bit for bit (NaN
+    bits included)."""
     got = np.asarray(got, np.float32)
-    if mode in EXACT:
-        np.testing.assert_array_equal(got.view(np.uint32), s32.view(np.uint32))
-    else:
-        nan = np.isnan(s32)
-        np.testing.assert_array_equal(got.view(np.uint32)[nan], s32.view(np.uint32)[nan])
-        assert _same(got, s32.astype(np.float64), JSD_TOL) <= JSD_TOL
+    np.testing.assert_array_equal(got.view(np.uint32), s32.view(np.uint32))
 
 
 @pytest.mark.gpu
@@ -306,9 +297,9 @@ def _dist_values(path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", GOLDEN_CASES)
-@pytest.mark.parametrize("mode", EXACT)
+@pytest.mark.parametrize("mode", MODES)
 def test_cuda_comp_leaf_metric_bytes(cms, gpu_ok, golden, tmp_path, case, mode):
-    """cuda-comp -M 2/3/5 (default kernel) writes the .dist file that the
+    """cuda-comp -M 2..5 (default kernel) writes the .dist file that the
     reference's values give through the .dist writer, byte for byte."""
     print(f"cuda-comp -M {mode} {case} bytes", flush=True)
     g = golden(case)
