@@ -379,6 +379,9 @@ int ensure_logf_table(cms_ctx *c) {
     HIPCALL(c, dalloc(c->d_logf_blocks, t.blocks.size()));
     HIPCALL(c, hipMemcpy(c->d_logf_blocks, t.blocks.data(), sizeof(uint32_t) * t.blocks.size(), hipMemcpyHostToDevice));
     std::memcpy(&c->logf_ln2, &t.ln2_bits, 4);
+    HIPCALL(c, dalloc(c->d_logf_pieces, 2 * (size_t)kLogPieces));
+    HIPCALL(c, hipMemcpy(c->d_logf_pieces, t.invc, sizeof(double) * kLogPieces, hipMemcpyHostToDevice));
+    HIPCALL(c, hipMemcpy(c->d_logf_pieces + kLogPieces, t.logc, sizeof(double) * kLogPieces, hipMemcpyHostToDevice));
     HIPCALL(c, dalloc(c->d_logf_tab, std::max<size_t>(t.table.size(), 1)));
     HIPCALL(c, hipMemcpy(c->d_logf_bm, t.bitmap.data(), sizeof(uint32_t) * t.bitmap.size(), hipMemcpyHostToDevice));
     if (!t.table.empty())
@@ -527,6 +530,7 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_logf_bm);
     dfree(c->d_logf_blocks);
     dfree(c->d_logf_tab);
+    dfree(c->d_logf_pieces);
     if (c->band_exec) (void)hipGraphExecDestroy(c->band_exec);
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -828,7 +832,8 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
         HIPCALL(c, hipEventRecord(c->ev0, st));
         if (use_leaf_merge(c)) {
             LeafMergeData md{c->d_rowptr, c->d_lment, c->d_lmstat, c->d_lmperm, c->n,
-                             c->d_logf_bm, c->d_logf_blocks, c->d_logf_tab, std::max(c->logf_ntab, 0), c->logf_ln2};
+                             c->d_logf_bm, c->d_logf_blocks, c->d_logf_tab, std::max(c->logf_ntab, 0), c->logf_ln2,
+                             c->d_logf_pieces};
             HIPCALL(c, launch_leaf_merge_all_pairs(md, c->opts.metric, d_sim, npairs, st));
         } else {
             HIPCALL(c, launch_zero_counter(c->d_work, st));

@@ -80,6 +80,7 @@ struct cms_ctx {
     int32_t *d_lmperm = nullptr;
     uint32_t *d_logf_bm = nullptr, *d_logf_blocks = nullptr;  // host logf exceptions (JSD), uploaded once
     float logf_ln2 = 0.0f;
+    double *d_logf_pieces = nullptr;
     unsigned long long *d_logf_tab = nullptr;
     int logf_ntab = -1;                       // -1 = not uploaded
     std::vector<int32_t> h_leaf_uses;  // per leaf: leaf-stream positions consuming it

@@ -80,12 +80,22 @@ struct CompiledTree {
 // library's result. bitmap bit u is set for each such input bit pattern u;
 // table holds (u << 32 | result bits), ascending. Built once per process.
 constexpr uint32_t kLogfMaxBits = 0x40800000u;  // 4.0f
+// The device's log in double (table_log): x = 2^k z, z in [0.699, 1.398) cut
+// into 64 pieces at the float bits u - kLogOff; log x = k ln2 + log c_i +
+// log1p(z / c_i - 1), the last by a degree-8 polynomial (|z / c_i - 1| <
+// 2^-7; c = 1 for the piece holding 1, so the result stays relatively exact
+// near x = 1). invc = 1 / c_i and logc = -log(invc) in double.
+constexpr uint32_t kLogOff = 0x3f330000u;
+constexpr int kLogPieces = 64;
 struct LogfTable {
     std::vector<uint32_t> bitmap;  // (kLogfMaxBits >> 5) + 1 words
     std::vector<uint32_t> blocks;  // bit b: some input in [256 b, 256 b + 255] is listed
     std::vector<uint64_t> table;
     uint32_t ln2_bits = 0;         // logf(2.0f)
+    double invc[kLogPieces], logc[kLogPieces];
 };
+// The device's table_log, evaluated on the host with the same IEEE operations.
+double table_log(float x, const double *invc, const double *logc);
 const LogfTable &host_logf_table();
 
 // Split a tree schedule into subtree segments (fills the seg_* and top_*

@@ -59,15 +59,42 @@ __global__ __launch_bounds__(64) void leaf_merge_prepare_kernel(const int64_t *_
     stat[s] = make_float2(sum, sq);
 }
 
-// The host C library's logf on (0, 4]: log((double)x) rounded to float,
+// The host C library's logf on (0, 4]: table_log(x) rounded to float,
 // except for the inputs the host listed (host_logf_table: where its logf
 // rounds otherwise, or the double lies too close to a rounding midpoint),
 // whose results it gave.
+// log(x) in double, the same operations as the host's table_log
+// (host_logf.cpp): x = 2^k z with z in [0.699, 1.398) cut into 64 pieces,
+// log c_i from the table, log1p(z / c_i - 1) by a degree-8 polynomial.
+__device__ __forceinline__ double table_log(float x, const double *invc, const double *logc) {
+    uint32_t u = __float_as_uint(x);
+    int32_t k0 = 0;
+    if (u < 0x00800000u) {  // subnormal: scale into the normal range (exact)
+        x *= 0x1p23f;
+        u = __float_as_uint(x);
+        k0 = -23;
+    }
+    const uint32_t tmp = u - kLogOff;
+    const int i = (int)((tmp >> 17) & (kLogPieces - 1));
+    const int32_t k = ((int32_t)tmp >> 23) + k0;
+    const double z = (double)__uint_as_float(u - (tmp & 0xff800000u));
+    const double r = fma(z, invc[i], -1.0);
+    double q = fma(r, -0.125, 1.0 / 7.0);
+    q = fma(r, q, -1.0 / 6.0);
+    q = fma(r, q, 0.2);
+    q = fma(r, q, -0.25);
+    q = fma(r, q, 1.0 / 3.0);
+    q = fma(r, q, -0.5);
+    const double lp = fma(r * r, q, r);
+    return fma((double)k, 0x1.62e42fefa39efp-1, logc[i]) + lp;
+}
+
 struct HostLogf {
     const uint32_t *bm, *blocks;
     const unsigned long long *tab;
     int ntab;
     float ln2;  // the host's logf(2.0f): a leaf that one sample misses gives x / h = 2
+    const double *invc, *logc;  // table_log's pieces (in LDS)
     __device__ __forceinline__ float operator()(float x) const {
         if (x == 2.0f) return ln2;
         const uint32_t u = __float_as_uint(x), b = u >> 8;
@@ -82,7 +109,7 @@ struct HostLogf {
             }
             return __uint_as_float((uint32_t)tab[lo]);
         }
-        return __double2float_rn(log((double)x));
+        return __double2float_rn(table_log(x, invc, logc));
     }
 };
 
@@ -188,6 +215,11 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
                                                                               float *__restrict__ out) {
     __shared__ uint2 s_ent[kMergeWavesPerBlock][16 * kMergeStride];
     __shared__ int64_t s_pos[kMergeWavesPerBlock][16], s_end[kMergeWavesPerBlock][16];
+    __shared__ double s_pieces[MODE == CMS_METRIC_JSD ? 2 * kLogPieces : 1];
+    if constexpr (MODE == CMS_METRIC_JSD) {
+        for (int k = threadIdx.x; k < 2 * kLogPieces; k += blockDim.x) s_pieces[k] = d.logf_pieces[k];
+        __syncthreads();  // every wave of the block reaches this before its tile loop
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint2 *sbuf = s_ent[wave];
     int64_t *lpos = s_pos[wave], *lend = s_end[wave];
@@ -230,7 +262,8 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
         }
         const uint2 *A = sbuf + r * kMergeStride, *B = sbuf + (8 + c) * kMergeStride;
         MergeAcc<MODE> acc;
-        acc.lg = HostLogf{d.logf_bm, d.logf_blocks, d.logf_tab, d.logf_ntab, d.logf_ln2};
+        acc.lg = HostLogf{d.logf_bm, d.logf_blocks, d.logf_tab, d.logf_ntab, d.logf_ln2, s_pieces,
+                          s_pieces + (MODE == CMS_METRIC_JSD ? kLogPieces : 0)};
         for (;;) {
             // stage: lists 2k (lanes 0-31) and 2k + 1 (lanes 32-63), entry lane & 31
             uint32_t cap = kEnd;

@@ -195,7 +195,8 @@ struct LeafMergeData {
     const uint32_t *logf_blocks;
     const unsigned long long *logf_tab;
     int logf_ntab;
-    float logf_ln2;  // the host's logf(2.0f)
+    float logf_ln2;              // the host's logf(2.0f)
+    const double *logf_pieces;   // table_log's invc[64], then logc[64]
 };
 // Fill ent (one per CSR entry) and stat (one per sample).
 hipError_t launch_leaf_merge_prepare(const int64_t *rowptr, const int32_t *col, const float *val, int n,
