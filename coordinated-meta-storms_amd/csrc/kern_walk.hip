@@ -181,6 +181,11 @@ __global__ __launch_bounds__(64 * kWalkWaves) void walk_tile_kernel(WalkData d, 
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int nb_ = chunks[ch].node_begin, ne = chunks[ch].node_end, lb = chunks[ch].leaf_begin;
         if (ch + 1 < d.nchunks) fetch(ch + 1);
+        // unrolled: the loop's scalar instructions are shared by several
+        // nodes (the walk is scalar-issue bound as much as vector-issue
+        // bound); 16-bit storage unrolls less to keep 8 waves per SIMD
+        constexpr int kNodeUnroll = KAHAN ? 2 : 4;
+#pragma unroll kNodeUnroll
         for (int i = nb_; i < ne; i++) {
             const uint32_t code = nodes[i].code;
             int lp = nodes[i].leafpos - lb;
