@@ -34,6 +34,10 @@ constexpr int kMergeK = 32;  // entries of each list staged per window (one per 
 // without a bounds check, and it shifts list l to bank 2 l, so lanes reading
 // the same position of different lists do not conflict.
 constexpr int kMergeStride = kMergeK + 1;
+// JSD: leaves of a window that both samples of a pair have, queued per wave
+// (up to kJsdQueue at a time) so that the wave evaluates their logs with
+// every lane busy.
+constexpr int kJsdQueue = 320;
 
 // One thread per sample: sum of a and sum of a^2 in FP32, leaf order
 // (dist.h:20-30 and 49-57), and the entry list (leaf, a) or (leaf, a / sum a)
@@ -95,20 +99,23 @@ struct HostLogf {
     int ntab;
     float ln2;  // the host's logf(2.0f): a leaf that one sample misses gives x / h = 2
     const double *invc, *logc;  // table_log's pieces (in LDS)
+    // The listed input u is in the table.
+    __device__ __forceinline__ float listed(uint32_t u) const {
+        int lo = 0, hi = ntab - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((uint32_t)(tab[mid] >> 32) < u)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return __uint_as_float((uint32_t)tab[lo]);
+    }
     __device__ __forceinline__ float operator()(float x) const {
         if (x == 2.0f) return ln2;
         const uint32_t u = __float_as_uint(x), b = u >> 8;
-        if (u <= kLogfMaxBits && ((blocks[b >> 5] >> (b & 31u)) & 1u) && ((bm[u >> 5] >> (u & 31u)) & 1u)) {
-            int lo = 0, hi = ntab - 1;  // the listed input is in the table
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if ((uint32_t)(tab[mid] >> 32) < u)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            return __uint_as_float((uint32_t)tab[lo]);
-        }
+        if (u <= kLogfMaxBits && ((blocks[b >> 5] >> (b & 31u)) & 1u) && ((bm[u >> 5] >> (u & 31u)) & 1u))
+            return listed(u);
         return __double2float_rn(table_log(x, invc, logc));
     }
 };
@@ -130,21 +137,25 @@ struct MergeAcc {
             float d = p - q;
             if (d < 0.0f) d = -d;
             g = g + d;
-        } else {
-            // dist.h:107-118. x / 2 is x * 0.5 (one rounding of the same
-            // value). Every lane takes one term (p's, or q's when p is not
-            // positive) and only lanes where both are positive a second one,
-            // so a wave evaluates one logf per step unless some lane's leaf
-            // is shared; each sum still gets its terms in leaf order.
-            if (p == 0.0f && q == 0.0f) return;
-            const float h = (p + q) * 0.5f;
-            const bool pp = p > 0.0f, qp = q > 0.0f;
-            const float x = pp ? p : q;
-            const float t = x * lg(__fdiv_rn(x, h));
-            if (pp) f = f + t;
-            else if (qp) g = g + t;
-            if (pp && qp) g = g + q * lg(__fdiv_rn(q, h));
         }
+    }
+    // JSD (dist.h:107-118): x log(x / h) for x = p or q, h = (p + q) / 2
+    // computed as (p + q) * 0.5 (one rounding of the same value).
+    __device__ __forceinline__ float jsd_term(float x, float h) const { return x * lg(__fdiv_rn(x, h)); }
+    // One leaf of the union: p's term into f and q's into g, for the values
+    // that are positive. A leaf where both are positive has its two terms
+    // evaluated already (tq). A leaf where one value is positive and the
+    // other zero has h = x / 2 exactly (x >= 2^-125), so x / h = 2 and its
+    // term is x * logf(2). A sum without a term at this leaf adds +0, which
+    // leaves it unchanged: it starts at +0 and a sum is -0 only when both
+    // operands are, so it is never -0.
+    __device__ __forceinline__ void jsd_leaf(float p, float q, bool both, float2 tq) {
+        const bool pp = p > 0.0f, single = !both && (pp || q > 0.0f);
+        const float x = pp ? p : q, o = pp ? q : p;
+        float t = x * lg.ln2;
+        if (single && !(o == 0.0f && x >= 0x1p-125f)) t = jsd_term(x, (p + q) * 0.5f);
+        f = f + (both ? tq.x : (single && pp ? t : 0.0f));
+        g = g + (both ? tq.y : (single && !pp ? t : 0.0f));
     }
 };
 
@@ -198,6 +209,97 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The next leaf m of the union of lists A and B (from positions ia, ib), and
+// which lists hold it; false when m is at or past lim. The values are tied
+// here so that each entry is one 64-bit LDS read (not a leaf read, then a
+// value read after the exit test).
+__device__ __forceinline__ bool merge_next(const uint2 *A, const uint2 *B, int ia, int ib, uint32_t lim,
+                                           uint32_t &m, bool &ta, bool &tb, uint32_t &va, uint32_t &vb) {
+    const uint2 ea = A[ia], eb = B[ib];
+    va = ea.y;
+    vb = eb.y;
+    asm volatile("" : "+v"(va), "+v"(vb));
+    m = min(ea.x, eb.x);
+    ta = ea.x == m;
+    tb = eb.x == m;
+    return m < lim;
+}
+
+// Lanes below this one in mask.
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// One JSD window of the wave's tile, in rounds. Pass 1 merges every lane's
+// two lists towards lim and queues the leaves where both values are positive,
+// in wave step order (slot = leaves queued at earlier steps + lanes below at
+// this step); the wave stops at the first step whose leaves would not fit.
+// Pass 2 evaluates the queued leaves' two terms, 64 at a time. Pass 3 merges
+// the same steps again (up to where pass 1 stopped, so with the same slots)
+// and adds every leaf's terms to the sums in leaf order. The next round
+// resumes pass 1 where it stopped.
+__device__ __forceinline__ void jsd_window(const uint2 *A, const uint2 *B, uint32_t lim, bool merge,
+                                           MergeAcc<CMS_METRIC_JSD> &acc, float2 *queue, int lane) {
+    int ia1 = 0, ib1 = 0, ia3 = 0, ib3 = 0;
+    bool done = !merge;  // pass 1 of this lane has reached lim
+    for (;;) {
+        uint32_t nq = 0;
+        bool full = false;
+        if (!done) {
+            for (;;) {
+                uint32_t m, va, vb;
+                bool ta, tb;
+                if (!merge_next(A, B, ia1, ib1, lim, m, ta, tb, va, vb)) {
+                    done = true;
+                    break;
+                }
+                const float p = ta ? __uint_as_float(va) : 0.0f, q = tb ? __uint_as_float(vb) : 0.0f;
+                const bool both = p > 0.0f && q > 0.0f;
+                const unsigned long long mask = __ballot(both);
+                const uint32_t cnt = (uint32_t)__popcll(mask);
+                // nq and cnt are the same in every lane still merging
+                if (nq + cnt > (uint32_t)kJsdQueue) {
+                    full = true;
+                    break;
+                }
+                // a lane without a shared leaf writes its own spare slot
+                queue[both ? nq + lanes_below(mask) : kJsdQueue + lane] = make_float2(p, q);
+                nq += cnt;
+                ia1 += ta;
+                ib1 += tb;
+            }
+        }
+        // the lane that merged longest counted every queued leaf
+        for (int off = 32; off > 0; off >>= 1) nq = max(nq, (uint32_t)__shfl_xor((int)nq, off, 64));
+        wave_sync();
+        for (uint32_t k = lane; k < nq; k += 64) {
+            const float2 pq = queue[k];
+            const float h = (pq.x + pq.y) * 0.5f;
+            queue[k] = make_float2(acc.jsd_term(pq.x, h), acc.jsd_term(pq.y, h));
+        }
+        wave_sync();
+        if (merge) {
+            uint32_t n3 = 0;
+            for (;;) {
+                if (ia3 == ia1 && ib3 == ib1) break;
+                uint32_t m, va, vb;
+                bool ta, tb;
+                merge_next(A, B, ia3, ib3, lim, m, ta, tb, va, vb);
+                const float p = ta ? __uint_as_float(va) : 0.0f, q = tb ? __uint_as_float(vb) : 0.0f;
+                const bool both = p > 0.0f && q > 0.0f;
+                const unsigned long long mask = __ballot(both);
+                const float2 t = queue[both ? n3 + lanes_below(mask) : kJsdQueue + lane];
+                acc.jsd_leaf(p, q, both, t);
+                n3 += (uint32_t)__popcll(mask);
+                ia3 += ta;
+                ib3 += tb;
+            }
+        }
+        if (__ballot(full) == 0ull) break;
+        wave_sync();  // pass 3's queue reads before the next round's writes
+    }
+}
+
 // Upper-triangle tiles of 8 x 8 pairs over the visiting order (samples with
 // the most entries first, so the lanes of a wave merge lists of similar
 // length), one lane per pair, grid-stride over tiles. The tile's 16 entry
@@ -216,6 +318,8 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
     __shared__ uint2 s_ent[kMergeWavesPerBlock][16 * kMergeStride];
     __shared__ int64_t s_pos[kMergeWavesPerBlock][16], s_end[kMergeWavesPerBlock][16];
     __shared__ double s_pieces[MODE == CMS_METRIC_JSD ? 2 * kLogPieces : 1];
+    __shared__ float2 s_queue[MODE == CMS_METRIC_JSD ? kMergeWavesPerBlock : 1]
+                             [MODE == CMS_METRIC_JSD ? kJsdQueue + 64 : 1];  // + one spare slot per lane
     if constexpr (MODE == CMS_METRIC_JSD) {
         for (int k = threadIdx.x; k < 2 * kLogPieces; k += blockDim.x) s_pieces[k] = d.logf_pieces[k];
         __syncthreads();  // every wave of the block reaches this before its tile loop
@@ -290,20 +394,16 @@ __global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(Le
                 }
             }
             wave_sync();
-            if (merge) {
+            if constexpr (MODE == CMS_METRIC_JSD) {
+                jsd_window(A, B, lim, merge, acc, s_queue[wave], lane);
+            } else if (merge) {
                 // a list never advances past a sentinel (leaf kEnd >= lim), so
                 // ia, ib <= kMergeK
                 int ia = 0, ib = 0;
                 for (;;) {
-                    const uint2 ea = A[ia], eb = B[ib];
-                    // the values are tied here so that each entry is one
-                    // 64-bit LDS read (not a leaf read, then a value read
-                    // after the exit test)
-                    uint32_t va = ea.y, vb = eb.y;
-                    asm volatile("" : "+v"(va), "+v"(vb));
-                    const uint32_t m = min(ea.x, eb.x);
-                    if (m >= lim) break;
-                    const bool ta = ea.x == m, tb = eb.x == m;
+                    uint32_t m, va, vb;
+                    bool ta, tb;
+                    if (!merge_next(A, B, ia, ib, lim, m, ta, tb, va, vb)) break;
                     acc.step(ta ? __uint_as_float(va) : 0.0f, tb ? __uint_as_float(vb) : 0.0f);
                     ia += ta;
                     ib += tb;
