@@ -36,8 +36,11 @@ constexpr int kMergeK = 32;  // entries of each list staged per window (one per 
 constexpr int kMergeStride = kMergeK + 1;
 // JSD: leaves of a window that both samples of a pair have, queued per wave
 // (up to kJsdQueue at a time) so that the wave evaluates their logs with
-// every lane busy.
-constexpr int kJsdQueue = 320;
+// every lane busy. The size keeps a JSD workgroup's LDS at 24.5 KB, so 6 fit
+// in a CU's 160 KB (6 waves per SIMD, as its 77 VGPRs allow): the kernel
+// waits on LDS and memory for most of its cycles and gains from every wave
+// more than it loses to the extra rounds.
+constexpr int kJsdQueue = 128;
 
 // One thread per sample: sum of a and sum of a^2 in FP32, leaf order
 // (dist.h:20-30 and 49-57), and the entry list (leaf, a) or (leaf, a / sum a)
