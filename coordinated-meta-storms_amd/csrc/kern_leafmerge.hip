@@ -36,11 +36,17 @@ constexpr int kMergeK = 32;  // entries of each list staged per window (one per 
 constexpr int kMergeStride = kMergeK + 1;
 // JSD: leaves of a window that both samples of a pair have, queued per wave
 // (up to kJsdQueue at a time) so that the wave evaluates their logs with
-// every lane busy. The size keeps a JSD workgroup's LDS at 24.5 KB, so 6 fit
-// in a CU's 160 KB (6 waves per SIMD, as its 77 VGPRs allow): the kernel
-// waits on LDS and memory for most of its cycles and gains from every wave
-// more than it loses to the extra rounds.
-constexpr int kJsdQueue = 128;
+// every lane busy. The size keeps a JSD workgroup's LDS at 21.5 KB, so 7 fit
+// in a CU's 160 KB: the kernel waits on LDS and memory for most of its
+// cycles and gains from every wave more than it loses to the extra rounds.
+constexpr int kJsdQueue = 96;
+// Workgroups per CU that a merge kernel is built for: 8 (8 waves per SIMD, at
+// most 64 VGPRs) except JSD's 7 (at most 72 VGPRs; 64 would spill more and
+// its LDS would need a smaller queue).
+template <int MODE>
+constexpr int merge_min_blocks() {
+    return MODE == CMS_METRIC_JSD ? 7 : 8;
+}
 
 // One thread per sample: sum of a and sum of a^2 in FP32, leaf order
 // (dist.h:20-30 and 49-57), and the entry list (leaf, a) or (leaf, a / sum a)
@@ -265,8 +271,7 @@ __device__ __forceinline__ void jsd_window(const uint2 *A, const uint2 *B, uint3
                     full = true;
                     break;
                 }
-                // a lane without a shared leaf writes its own spare slot
-                queue[both ? nq + lanes_below(mask) : kJsdQueue + lane] = make_float2(p, q);
+                if (both) queue[nq + lanes_below(mask)] = make_float2(p, q);
                 nq += cnt;
                 ia1 += ta;
                 ib1 += tb;
@@ -291,7 +296,7 @@ __device__ __forceinline__ void jsd_window(const uint2 *A, const uint2 *B, uint3
                 const float p = ta ? __uint_as_float(va) : 0.0f, q = tb ? __uint_as_float(vb) : 0.0f;
                 const bool both = p > 0.0f && q > 0.0f;
                 const unsigned long long mask = __ballot(both);
-                const float2 t = queue[both ? n3 + lanes_below(mask) : kJsdQueue + lane];
+                const float2 t = both ? queue[n3 + lanes_below(mask)] : make_float2(0.0f, 0.0f);
                 acc.jsd_leaf(p, q, both, t);
                 n3 += (uint32_t)__popcll(mask);
                 ia3 += ta;
@@ -315,14 +320,14 @@ __device__ __forceinline__ void jsd_window(const uint2 *A, const uint2 *B, uint3
 // set lim advances by kMergeK entries, so the windows always progress; when
 // every list is staged to its end, lim is past all leaves and the tile ends.
 template <int MODE>
-__global__ __launch_bounds__(64 * kMergeWavesPerBlock) void leaf_merge_kernel(LeafMergeData d, int64_t nb,
+__global__ __launch_bounds__(64 * kMergeWavesPerBlock, merge_min_blocks<MODE>()) void leaf_merge_kernel(LeafMergeData d, int64_t nb,
                                                                               int64_t t_count,
                                                                               float *__restrict__ out) {
     __shared__ uint2 s_ent[kMergeWavesPerBlock][16 * kMergeStride];
     __shared__ int64_t s_pos[kMergeWavesPerBlock][16], s_end[kMergeWavesPerBlock][16];
     __shared__ double s_pieces[MODE == CMS_METRIC_JSD ? 2 * kLogPieces : 1];
     __shared__ float2 s_queue[MODE == CMS_METRIC_JSD ? kMergeWavesPerBlock : 1]
-                             [MODE == CMS_METRIC_JSD ? kJsdQueue + 64 : 1];  // + one spare slot per lane
+                             [MODE == CMS_METRIC_JSD ? kJsdQueue : 1];
     if constexpr (MODE == CMS_METRIC_JSD) {
         for (int k = threadIdx.x; k < 2 * kLogPieces; k += blockDim.x) s_pieces[k] = d.logf_pieces[k];
         __syncthreads();  // every wave of the block reaches this before its tile loop
