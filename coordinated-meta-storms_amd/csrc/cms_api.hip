@@ -523,6 +523,8 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_col);
     dfree(c->d_val);
     dfree(c->d_out);
+    dfree(c->d_prow);
+    dfree(c->d_pcol);
     dfree(c->d_counter);
     dfree(c->d_work);
     dfree(c->d_perm);
@@ -654,6 +656,7 @@ int cms_set_samples_csr(cms_ctx *c, int n, const int64_t *rowptr, const int32_t 
         if (rowptr[s + 1] < rowptr[s]) return fail(c, CMS_E_INVAL, "rowptr not non-decreasing");
         for (int64_t k = rowptr[s]; k < rowptr[s + 1]; k++) {
             if (leaf[k] < 0 || leaf[k] >= c->tree.leaf_n) return fail(c, CMS_E_INVAL, "leaf index out of range");
+            if (!std::isfinite(val[k])) return fail(c, CMS_E_INVAL, "abundances must be finite");
             if (!(val[k] >= 0.0f)) neg = true;
         }
         // a leaf listed twice in one row would be written twice (last wins) in
@@ -969,21 +972,19 @@ int cms_sim_pairs(cms_ctx *c, int64_t n_pairs, const int32_t *row, const int32_t
     for (int64_t i = 0; i < n_pairs; i++)
         if (row[i] < 0 || row[i] >= c->n || col[i] < 0 || col[i] >= c->n)
             return fail(c, CMS_E_INVAL, "cms_sim_pairs: sample index out of range");
-    int32_t *d_r = nullptr, *d_c = nullptr;
-    HIPCALL(c, dalloc(d_r, (size_t)n_pairs));
-    HIPCALL(c, dalloc(d_c, (size_t)n_pairs));
+    // pair lists live in context buffers (grown, reused), so no exit path leaks
+    HIPCALL(c, dreserve(c->d_prow, c->cap_prow, (size_t)n_pairs));
+    HIPCALL(c, dreserve(c->d_pcol, c->cap_pcol, (size_t)n_pairs));
     rc = ensure_out(c, (size_t)n_pairs);
     if (rc) return rc;
-    HIPCALL(c, hipMemcpyAsync(d_r, row, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
-    HIPCALL(c, hipMemcpyAsync(d_c, col, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
+    HIPCALL(c, hipMemcpyAsync(c->d_prow, row, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
+    HIPCALL(c, hipMemcpyAsync(c->d_pcol, col, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
     if (use_sm(c))
-        HIPCALL(c, launch_sm_pairs(sm_data(c), n_pairs, d_r, d_c, c->d_out, c->stream));
+        HIPCALL(c, launch_sm_pairs(sm_data(c), n_pairs, c->d_prow, c->d_pcol, c->d_out, c->stream));
     else
-        HIPCALL(c, launch_walk_pairs(walk_data(c), n_pairs, d_r, d_c, c->d_out, c->stream));
+        HIPCALL(c, launch_walk_pairs(walk_data(c), n_pairs, c->d_prow, c->d_pcol, c->d_out, c->stream));
     HIPCALL(c, hipMemcpyAsync(sim, c->d_out, sizeof(float) * n_pairs, hipMemcpyDeviceToHost, c->stream));
     HIPCALL(c, hipStreamSynchronize(c->stream));
-    dfree(d_r);
-    dfree(d_c);
     return CMS_OK;
 }
 

@@ -208,8 +208,7 @@ hipError_t launch_leaf_all_pairs(const LeafData &d, int mode, unsigned long long
     const int64_t ntiles = nb * (nb + 1) / 2;
     const int64_t n = d.n;
     if (n < 2 || ntiles <= 0) return hipSuccess;
-    if (out_cap < n * (n - 1) / 2 || nb * kSmBlock > d.npad || d.npad >= ((int64_t)1 << 21))
-        return hipErrorInvalidValue;
+    if (out_cap < n * (n - 1) / 2 || nb * kSmBlock > d.npad) return hipErrorInvalidValue;
     constexpr int CH = 16, WV = 4;
     const unsigned wg = (unsigned)std::max<int64_t>(1, std::min<int64_t>((ntiles + WV - 1) / WV, grid_waves / WV));
     switch (mode) {

@@ -123,6 +123,12 @@ struct HostLogf {
     __device__ __forceinline__ float operator()(float x) const {
         if (x == 2.0f) return ln2;
         const uint32_t u = __float_as_uint(x), b = u >> 8;
+        // outside table_log's domain, the C library's values: logf(+-0) =
+        // -inf, logf(+inf) = +inf (x / h is +inf when h rounds to 0 next to
+        // a zero), a NaN is returned as it is, and a negative input gives
+        // the x86 default NaN
+        if ((u & 0x7fffffffu) == 0u) return -__builtin_inff();
+        if (u >= 0x7f800000u) return (u <= 0x7fffffffu || u > 0xff800000u) ? x : __uint_as_float(0xffc00000u);
         if (u <= kLogfMaxBits && ((blocks[b >> 5] >> (b & 31u)) & 1u) && ((bm[u >> 5] >> (u & 31u)) & 1u))
             return listed(u);
         return __double2float_rn(table_log(x, invc, logc));

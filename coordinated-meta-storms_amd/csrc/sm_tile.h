@@ -10,10 +10,10 @@ namespace cms {
 
 // Row loaders: value of this lane's sample at row rbase + r of Y, with
 // window(rbase) called once per 64-word bitmap window and r < 2048. Row
-// addresses come from the wave-uniform row index in scalar registers (32-bit
-// r * stride: strides below 2^21 samples, checked by the launchers), and the
-// lane's column is a 32-bit byte offset, so a staged row costs one vector
-// load and no vector address arithmetic.
+// addresses come from the wave-uniform row index in scalar registers (64-bit
+// r * stride, so any stride the allocation allows is addressed correctly),
+// and the lane's column is a 32-bit byte offset, so a staged row costs one
+// vector load and no vector address arithmetic.
 //
 // One matrix (all-pairs tiles): lane column `col` (row block sample for lanes
 // 0..31, column block sample for lanes 32..63) of Y with row stride `stride`.
@@ -28,7 +28,7 @@ struct SquareRows {
     const uint32_t *win = nullptr;
     __device__ __forceinline__ void window(int64_t rbase) { win = Y + rbase * stride; }
     __device__ __forceinline__ uint32_t load(uint32_t r) const {
-        const char *row = reinterpret_cast<const char *>(win + r * (uint32_t)stride);
+        const char *row = reinterpret_cast<const char *>(win + (uint64_t)r * (uint64_t)stride);
         const uint32_t v = *reinterpret_cast<const uint32_t *>(row + col4);
         return (!CHECK || ok) ? v : 0u;
     }
@@ -51,10 +51,10 @@ struct CrossRows {
     __device__ __forceinline__ uint32_t load(uint32_t r) const {
         uint32_t v;
         if (lo) {
-            const char *row = reinterpret_cast<const char *>(qwin + r * (uint32_t)qs);
+            const char *row = reinterpret_cast<const char *>(qwin + (uint64_t)r * (uint64_t)qs);
             v = *reinterpret_cast<const uint32_t *>(row + col4);
         } else {
-            const char *row = reinterpret_cast<const char *>(ywin + r * (uint32_t)ys);
+            const char *row = reinterpret_cast<const char *>(ywin + (uint64_t)r * (uint64_t)ys);
             v = *reinterpret_cast<const uint32_t *>(row + col4);
         }
         return v;
