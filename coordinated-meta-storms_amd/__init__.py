@@ -31,6 +31,8 @@ BIN_DIR = os.path.join(_HERE, "bin")
 
 OK = 0
 E_INVAL, E_NOMEM, E_HIP, E_SCHEDULE, E_STATE, E_IO, E_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
+#: pairs per cms_sim_pairs_noalloc call (CMS_NOALLOC_MAX_PAIRS)
+NOALLOC_MAX_PAIRS = 1 << 20
 E_RANGE, E_TIMEOUT = -8, -9
 KERNEL_WALK, KERNEL_SHAREDMIN = 0, 1
 METRIC_WEIGHTED, METRIC_UNWEIGHTED = 0, 1
@@ -120,6 +122,7 @@ _BAND_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_flo
 _sig("cms_all_pairs_stream", C.c_int, [_vp, C.c_int64, _BAND_FN, _vp])
 _sig("cms_sample_order", C.c_int, [_vp, _i32p])
 _sig("cms_sim_pairs", C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f32p])
+_sig("cms_sim_pairs_noalloc", C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f32p])
 _sig("cms_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f32p])
 _sig("cms_get_stats", C.c_int, [_vp, C.POINTER(Stats)])
 _sig("cms_prepare", C.c_int, [_vp, C.c_int])
@@ -184,7 +187,7 @@ EXPORTED = [
     "cms_opts_default", "cms_create", "cms_destroy", "cms_last_error", "cms_set_opts", "cms_set_tree",
     "cms_set_samples", "cms_set_samples_csr", "cms_all_pairs", "cms_all_pairs_device", "cms_copy_result",
     "cms_synchronize", "cms_device_count", "cms_calibrate_copy", "cms_all_pairs_stream",
-    "cms_sample_order", "cms_sim_pairs",
+    "cms_sample_order", "cms_sim_pairs", "cms_sim_pairs_noalloc",
     "cms_tile", "cms_get_stats", "cms_prepare", "cms_validate_tree", "cms_db_open", "cms_db_close", "cms_db_leaf_n", "cms_db_order_n",
     "cms_db_has_tree", "cms_db_has_cp", "cms_db_order", "cms_db_leaf_name", "cms_table_load",
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
@@ -693,6 +696,16 @@ class Context:
         out = np.zeros(max(len(row), 1), np.float32)
         _check(lib.cms_sim_pairs(self._h, len(row), row if len(row) else np.zeros(1, np.int32),
                                  col if len(col) else np.zeros(1, np.int32), out), self._h)
+        return out[: len(row)]
+
+    def sim_pairs_noalloc(self, row, col) -> np.ndarray:
+        """cms_sim_pairs_noalloc: the pair list with no device allocation
+        (layouts must be prepared; at most NOALLOC_MAX_PAIRS pairs)."""
+        row = np.ascontiguousarray(row, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        out = np.zeros(max(len(row), 1), np.float32)
+        _check(lib.cms_sim_pairs_noalloc(self._h, len(row), row if len(row) else np.zeros(1, np.int32),
+                                         col if len(col) else np.zeros(1, np.int32), out), self._h)
         return out[: len(row)]
 
     def tile(self, r0: int, nr: int, c0: int, nc: int) -> np.ndarray:

@@ -10,14 +10,26 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "cms_ctx.h"
 
 using namespace cms;
 
+// Device scratch of cms_sim_pairs_noalloc: the staged row list, column list
+// and results of up to CMS_NOALLOC_MAX_PAIRS pairs (12 MiB). Every device the
+// library runs on holds its own copy until the process ends, shared by all
+// contexts on that device (calls that overlap on one device race). In host
+// code the array's name is only a stand-in: the device address comes from
+// hipGetSymbolAddress, taken once per device by cms_create with that device
+// current.
+static __device__ __align__(256) uint8_t scratch[(size_t)CMS_NOALLOC_MAX_PAIRS * 12];
 
 namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // per device, null until taken
 
 int fail(cms_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -498,6 +510,22 @@ int cms_create(int device, const cms_opts *o, cms_ctx **out) {
         hipMalloc((void **)&c->d_counter, sizeof(unsigned long long)) != hipSuccess) {
         delete c;
         return fail(nullptr, CMS_E_HIP, "cms_create: stream/event creation failed");
+    }
+    {
+        // the no-allocation entry point's scratch on this device (device current)
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        if (g_scratch_ptr.size() < (size_t)count) g_scratch_ptr.resize(count, nullptr);
+        if (!g_scratch_ptr[device]) {
+            void *p = nullptr;
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                cms_destroy(c);
+                return fail(nullptr, CMS_E_HIP, std::string("cms_create: hipGetSymbolAddress(scratch): ") +
+                                                    hipGetErrorString(e));
+            }
+            g_scratch_ptr[device] = p;
+        }
+        c->scratch = g_scratch_ptr[device];
     }
     *out = c;
     return CMS_OK;
@@ -984,6 +1012,44 @@ int cms_sim_pairs(cms_ctx *c, int64_t n_pairs, const int32_t *row, const int32_t
     else
         HIPCALL(c, launch_walk_pairs(walk_data(c), n_pairs, c->d_prow, c->d_pcol, c->d_out, c->stream));
     HIPCALL(c, hipMemcpyAsync(sim, c->d_out, sizeof(float) * n_pairs, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    return CMS_OK;
+}
+
+// cms_sim_pairs without device allocation: the layouts must already be
+// built (cms_prepare, or any earlier computing call, on the same kernel and
+// options); the pair lists and results are staged in the device's static
+// scratch.
+int cms_sim_pairs_noalloc(cms_ctx *c, int64_t n_pairs, const int32_t *row, const int32_t *col, float *sim) {
+    if (!c) return fail(nullptr, CMS_E_INVAL, "null context");
+    if (is_leaf_metric(c)) return fail(c, CMS_E_UNSUPPORTED, "-M 2..5 run through cms_all_pairs only");
+    if (n_pairs < 0 || (n_pairs > 0 && (!row || !col || !sim)))
+        return fail(c, CMS_E_INVAL, "cms_sim_pairs_noalloc: bad arrays");
+    if (n_pairs > (int64_t)CMS_NOALLOC_MAX_PAIRS)
+        return fail(c, CMS_E_INVAL, "cms_sim_pairs_noalloc: " + std::to_string(n_pairs) +
+                                        " pairs exceed the scratch of CMS_NOALLOC_MAX_PAIRS = " +
+                                        std::to_string((int64_t)CMS_NOALLOC_MAX_PAIRS));
+    if (n_pairs == 0) return CMS_OK;
+    if (!c->has_tree || c->n < 0) return fail(c, CMS_E_STATE, "cms_sim_pairs_noalloc: no tree or samples loaded");
+    const bool ready = use_sm(c) ? (c->sm_ready && c->sm_metric == c->opts.metric && c->sm_binned == (c->opts.bin != 0))
+                                 : (c->ab_storage == walk_storage(c) && !c->ab_binned);
+    if (!ready)
+        return fail(c, CMS_E_STATE,
+                    "cms_sim_pairs_noalloc: device layouts not built for the current options (call cms_prepare first)");
+    for (int64_t i = 0; i < n_pairs; i++)
+        if (row[i] < 0 || row[i] >= c->n || col[i] < 0 || col[i] >= c->n)
+            return fail(c, CMS_E_INVAL, "cms_sim_pairs_noalloc: sample index out of range");
+    HIPCALL(c, hipSetDevice(c->device));
+    int32_t *d_r = (int32_t *)c->scratch;
+    int32_t *d_c = d_r + CMS_NOALLOC_MAX_PAIRS;
+    float *d_o = (float *)(d_c + CMS_NOALLOC_MAX_PAIRS);
+    HIPCALL(c, hipMemcpyAsync(d_r, row, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
+    HIPCALL(c, hipMemcpyAsync(d_c, col, sizeof(int32_t) * n_pairs, hipMemcpyHostToDevice, c->stream));
+    if (use_sm(c))
+        HIPCALL(c, launch_sm_pairs(sm_data(c), n_pairs, d_r, d_c, d_o, c->stream));
+    else
+        HIPCALL(c, launch_walk_pairs(walk_data(c), n_pairs, d_r, d_c, d_o, c->stream));
+    HIPCALL(c, hipMemcpyAsync(sim, d_o, sizeof(float) * n_pairs, hipMemcpyDeviceToHost, c->stream));
     HIPCALL(c, hipStreamSynchronize(c->stream));
     return CMS_OK;
 }

@@ -90,6 +90,7 @@ struct cms_ctx {
     size_t out_cap = 0;
     int32_t *d_prow = nullptr, *d_pcol = nullptr;  // cms_sim_pairs pair lists (grown, reused)
     size_t cap_prow = 0, cap_pcol = 0;
+    void *scratch = nullptr;  // this device's static scratch (cms_sim_pairs_noalloc), from hipGetSymbolAddress
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
     bool kern_pending = false, prep_pending = false;
     cms_stats stats{};

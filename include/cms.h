@@ -157,6 +157,19 @@ int cms_sample_order(cms_ctx *c, int32_t *order);
 int cms_sim_pairs(cms_ctx *c, int64_t n_pairs, const int32_t *row,
                   const int32_t *col, float *sim);
 
+/* The same pair list with no device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc). The device layouts must already be
+ * built for the context's current kernel and options (cms_prepare, or an
+ * earlier computing call), else CMS_E_STATE; the pair lists and results are
+ * staged in a static per-device scratch of CMS_NOALLOC_MAX_PAIRS pairs
+ * (12 MiB, held until the process ends); more pairs are CMS_E_INVAL. All
+ * contexts on one device share that scratch: calls that overlap on one
+ * device race. */
+#define CMS_NOALLOC_MAX_PAIRS (1 << 20)
+int cms_sim_pairs_noalloc(cms_ctx *c, int64_t n_pairs, const int32_t *row,
+                          const int32_t *col, float *sim);
+
 /* Rectangular tile rows [r0, r0+nr) x cols [c0, c0+nc) into a host buffer of
  * nr*nc floats, row-major. Diagonal entries (same sample) are 1. Replaces one
  * iteration of the tile loop (cms_cuda.cu:1523-1539, defineOrder 961-985). */
