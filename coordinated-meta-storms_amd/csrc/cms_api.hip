@@ -275,6 +275,7 @@ int ensure_sharedmin(cms_ctx *c) {
             e.sums_only = sums_only ? 1 : 0;
             e.Y = c->d_Y ? c->d_Y + p0 : nullptr;
             e.bm = c->d_bm ? c->d_bm + (p0 / kSmBlock) * c->nwords : nullptr;
+            e.lbm = (c->d_lbm && !sums_only) ? c->d_lbm + (p0 / kSmBlock) * c->nwords : nullptr;
             e.colsum = c->d_colsum + p0;
             e.colsum_f64 = c->d_mass + p0;
             HIPCALL(c, launch_embed(e, c->stream));
@@ -304,10 +305,19 @@ int ensure_sharedmin(cms_ctx *c) {
     HIPCALL(c, dreserve(c->d_Y, c->cap_Y, (size_t)std::max<int64_t>(nrows, 1) * std::max<int64_t>(c->npad, 1)));
     HIPCALL(c, dreserve(c->d_bm, c->cap_bm, nbm));
     HIPCALL(c, hipMemsetAsync(c->d_bm, 0, sizeof(uint32_t) * nbm, c->stream));
+    // light-entry bitmaps of the split tiles (filled by the embedding)
+    HIPCALL(c, dreserve(c->d_lbm, c->cap_lbm, nbm));
+    HIPCALL(c, hipMemsetAsync(c->d_lbm, 0, sizeof(uint32_t) * nbm, c->stream));
     HIPCALL(c, hipMemsetAsync(c->d_colsum, 0, sizeof(unsigned long long) * std::max<int64_t>(c->npad, 1), c->stream));
     ed.scale = std::ldexp(1.0, K);
     rc = embed_all(false);
     if (rc) return rc;
+    // records of the light entries in rank order (every block holds at most
+    // nwords * 32 of them)
+    c->crs = std::max<int64_t>(c->nwords, 1) * 32;
+    HIPCALL(c, dreserve(c->d_pl, c->cap_pl, nbm));
+    HIPCALL(c, dreserve(c->d_crl, c->cap_crl, (size_t)std::max<int64_t>(nblk, 1) * c->crs));
+    HIPCALL(c, launch_light_records(c->d_Y, c->npad, c->d_lbm, nblk, c->nwords, c->d_pl, c->d_crl, c->crs, c->stream));
     c->sm_rows = nrows;
     c->sm_metric = metric;
     c->sm_ready = true;
@@ -328,6 +338,9 @@ SmData sm_data(const cms_ctx *c) {
     d.colsum = c->d_colsum;
     d.perm = c->sm_binned ? c->d_perm : nullptr;
     d.pos = c->sm_binned ? c->d_pos : nullptr;
+    d.lbm = c->d_lbm;
+    d.crl = c->d_crl;
+    d.crs = c->crs;
     return d;
 }
 
@@ -484,6 +497,12 @@ void cms_opts_default(cms_opts *o) {
 namespace {
 const char *opts_error(const cms_opts *o) {
     if (o->kernel != CMS_KERNEL_WALK && o->kernel != CMS_KERNEL_SHAREDMIN) return "unknown kernel";
+    if ((o->tile_a != 0 && o->tile_a != kSmBlock) || (o->tile_b != 0 && o->tile_b != kSmBlock))
+        return "tile_a / tile_b: the shared-minimum tiles are 32 x 32 samples per wave (0 or 32)";
+    if (o->chunk != 0 && o->chunk != 8 && o->chunk != 16 && o->chunk != 32) return "chunk must be 0, 8, 16 or 32";
+    if (o->variant < 0 || o->variant > kSmVariants) return "variant must be 0..12";
+    if (o->variant > 0 && o->chunk != 0 && kSmVariantTable[o->variant - 1].chunk != o->chunk)
+        return "variant and chunk disagree";
     if (o->metric < CMS_METRIC_WEIGHTED || o->metric > CMS_METRIC_BC) return "unknown metric";
     if (o->storage < CMS_STORE_F32 || o->storage > CMS_STORE_F16) return "unknown storage";
     if (o->embed_slice < 0) return "embed_slice must be >= 0";
@@ -507,7 +526,7 @@ int cms_create(int device, const cms_opts *o, cms_ctx **out) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->evp0) != hipSuccess || hipEventCreate(&c->evp1) != hipSuccess ||
-        hipMalloc((void **)&c->d_counter, sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc((void **)&c->d_counter, sizeof(unsigned long long) * kCounters) != hipSuccess) {
         delete c;
         return fail(nullptr, CMS_E_HIP, "cms_create: stream/event creation failed");
     }
@@ -550,6 +569,9 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_rowptr);
     dfree(c->d_col);
     dfree(c->d_val);
+    dfree(c->d_lbm);
+    dfree(c->d_pl);
+    dfree(c->d_crl);
     dfree(c->d_out);
     dfree(c->d_prow);
     dfree(c->d_pcol);
@@ -763,8 +785,16 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
         return CMS_OK;
     }
     if (c->variant >= 0) return CMS_OK;
+    // opts.chunk (if set) restricts the candidates to that LDS chunk
+    auto allowed = [&](int v) { return c->opts.chunk == 0 || kSmVariantTable[v].chunk == c->opts.chunk; };
     if (!c->opts.autotune || ntiles <= 0) {
-        c->variant = 1;  // 16-entry LDS chunk, 4 waves per workgroup
+        c->variant = kSmDefaultVariant;
+        if (!allowed(c->variant))
+            for (int v = 0; v < kSmVariants; v++)
+                if (allowed(v)) {
+                    c->variant = v;
+                    break;
+                }
         return CMS_OK;
     }
     constexpr int64_t kSeg = 8192;
@@ -779,14 +809,19 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
         return CMS_OK;
     };
     float best = 1e30f;
-    int best_v = 1;
-    if (ntiles >= 2 * kSmVariants * kSeg) {
-        std::vector<double> cost(kSmVariants, 0.0), rows(kSmVariants, 0.0);
+    int best_v = -1;
+    std::vector<int> cand;
+    for (int v = 0; v < kSmVariants; v++)
+        if (allowed(v)) cand.push_back(v);
+    const int nc = (int)cand.size();
+    for (int v = 0; v < kSmVariants; v++) c->stats.tune_ms[v] = 0.0;
+    if (ntiles >= 2 * (int64_t)nc * kSeg) {
+        std::vector<double> rows(kSmVariants, 0.0);
         std::vector<float> msum(kSmVariants, 0.0f);
         unsigned long long before = 0, after = 0;
         HIPCALL(c, hipMemcpy(&before, c->d_counter, sizeof before, hipMemcpyDeviceToHost));
-        for (int k = 0; k < 2 * kSmVariants; k++) {
-            const int v = k < kSmVariants ? k : 2 * kSmVariants - 1 - k;
+        for (int k = 0; k < 2 * nc; k++) {
+            const int v = cand[k < nc ? k : 2 * nc - 1 - k];
             float ms = 0;
             int rc = timed(v, (int64_t)k * kSeg, kSeg, ms);
             if (rc) return rc;
@@ -795,7 +830,7 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
             rows[v] += (double)(after - before);
             before = after;
         }
-        for (int v = 0; v < kSmVariants; v++) {
+        for (int v : cand) {
             c->stats.tune_ms[v] = msum[v];
             const double per_row = rows[v] > 0 ? msum[v] / rows[v] : 1e30;
             if (per_row < best) {
@@ -803,7 +838,7 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
                 best_v = v;
             }
         }
-        *done = 2 * kSmVariants * kSeg;
+        *done = 2 * (int64_t)nc * kSeg;
     } else {
         // three segments at the start, middle and end of the range: with
         // binned samples the first tiles are the densest, the last the
@@ -811,7 +846,7 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
         // per cent)
         const int nseg = ntiles >= 3 * kSeg ? 3 : 1;
         const int64_t seg_len = std::min<int64_t>(ntiles, kSeg);
-        for (int v = 0; v < kSmVariants; v++) {
+        for (int v : cand) {
             float total = 0;
             for (int k = 0; k < nseg; k++) {
                 const int64_t t0 = nseg == 1 ? 0 : (ntiles - seg_len) * k / (nseg - 1);
@@ -827,7 +862,7 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
             }
         }
     }
-    c->variant = best_v;
+    c->variant = best_v >= 0 ? best_v : kSmDefaultVariant;
     c->prep_pending = false;  // tuning reused the prep events
     return CMS_OK;
 }
@@ -882,13 +917,13 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
         const int64_t npairs = (int64_t)c->n * (c->n > 0 ? c->n - 1 : 0) / 2;  // d_sim holds the whole triangle
         // the kernel time and visited rows include tuning launches that computed real tiles
         HIPCALL(c, hipEventRecord(c->ev0, st));
-        HIPCALL(c, launch_zero_counter(c->d_counter, st));
+        HIPCALL(c, launch_zero_counters(c->d_counter, kCounters, st));
         int64_t done = 0;
         rc = pick_variant(c, nb, ntiles, d_sim, npairs, 0, st, &done);
         if (rc) return rc;
         if (done == 0) {
             HIPCALL(c, hipEventRecord(c->ev0, st));  // discarded tuning launches are not counted
-            HIPCALL(c, launch_zero_counter(c->d_counter, st));
+            HIPCALL(c, launch_zero_counters(c->d_counter, kCounters, st));
         }
         HIPCALL(c, launch_zero_counter(c->d_work, st));
         HIPCALL(c, launch_sm_persistent(sm_data(c), c->variant, nb, done, ntiles - done, c->d_work, d_sim, npairs, 0,
@@ -1149,7 +1184,7 @@ int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *u
     // first count[0] <= cap entries of the packed triangle; tiles the tuner
     // computed for real are skipped by the first band's launch
     STREAMCALL(hipEventRecord(c->ev0, c->stream));
-    STREAMCALL(launch_zero_counter(c->d_counter, c->stream));
+    STREAMCALL(launch_zero_counters(c->d_counter, kCounters, c->stream));
     int64_t tuned = 0;
     rc = pick_variant(c, nb, tile_start(cuts[1]), dbuf[0], cap, 1, c->stream, &tuned);
     if (rc) {
@@ -1158,7 +1193,7 @@ int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *u
     }
     if (tuned == 0) {
         STREAMCALL(hipEventRecord(c->ev0, c->stream));
-        STREAMCALL(launch_zero_counter(c->d_counter, c->stream));
+        STREAMCALL(launch_zero_counters(c->d_counter, kCounters, c->stream));
     }
     const SmData d = sm_data(c);
     const int nbands = (int)cuts.size() - 1;
@@ -1233,12 +1268,25 @@ int cms_get_stats(cms_ctx *c, cms_stats *s) {
         HIPCALL(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         c->stats.kernel_ms = ms;
         if (use_sm(c)) {
-            unsigned long long rows = 0;
-            HIPCALL(c, hipMemcpy(&rows, c->d_counter, sizeof rows, hipMemcpyDeviceToHost));
-            // each visited entry is one pair-entry for all 32 x 32 pairs of the tile
-            c->stats.pair_node_updates = (double)rows * kSmBlock * kSmBlock;
-            c->stats.valu_ops = c->stats.pair_node_updates * kSmOpsPerPairEntry;
-            c->stats.bytes_moved = (double)rows * 2.0 * kSmBlock * 4.0;
+            unsigned long long cnt[kCounters] = {};
+            HIPCALL(c, hipMemcpy(cnt, c->d_counter, sizeof cnt, hipMemcpyDeviceToHost));
+            // a staged entry is one pair-entry for all 32 x 32 pairs of the
+            // tile (17/16 VALU ops each, 2 x 32 values of Y); a sparse entry
+            // costs one minimum and one add per pair with mass on both sides
+            const double staged = (double)cnt[kCntStaged];
+            const double split_pairs = (double)cnt[kCntSparsePairs] + (double)cnt[kCntHalfPairs];
+            c->stats.pair_node_updates = staged * kSmBlock * kSmBlock + split_pairs;
+            c->stats.valu_ops = staged * kSmBlock * kSmBlock * kSmOpsPerPairEntry + 2.0 * split_pairs;
+            // staged entries read 2 x 32 values of Y, sparse ones two 16-byte
+            // records, half ones one block's 32 values and one record
+            c->stats.bytes_moved = staged * 2.0 * kSmBlock * 4.0 + (double)cnt[kCntSparseRows] * 32.0 +
+                                   (double)cnt[kCntHalfRows] * (kSmBlock * 4.0 + 16.0);
+            c->stats.entries_visited = (int64_t)cnt[kCntVisited];
+            c->stats.entries_staged = (int64_t)cnt[kCntStaged];
+            c->stats.entries_sparse = (int64_t)cnt[kCntSparseRows];
+            c->stats.sparse_pairs = (int64_t)cnt[kCntSparsePairs];
+            c->stats.entries_half = (int64_t)cnt[kCntHalfRows];
+            c->stats.half_pairs = (int64_t)cnt[kCntHalfPairs];
         }
         c->kern_pending = false;
     }

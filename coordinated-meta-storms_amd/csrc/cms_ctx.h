@@ -59,6 +59,10 @@ struct cms_ctx {
     unsigned long long *d_colsum = nullptr;
     size_t cap_colsum = 0;
     uint32_t *d_Y = nullptr, *d_bm = nullptr;
+    uint32_t *d_lbm = nullptr, *d_pl = nullptr;  // light-entry bitmaps and their per-word prefix
+    uint4 *d_crl = nullptr;                     // records of the light entries, per block in rank order
+    size_t cap_lbm = 0, cap_pl = 0, cap_crl = 0;
+    int64_t crs = 0;
     double *d_mass = nullptr;
     double *d_xroot = nullptr;
     size_t cap_xroot = 0;

@@ -85,25 +85,31 @@ constexpr int kEmbedNodeChunk = CMS_EMBED_NODE_CHUNK;
 constexpr int kEmbedLeafChunk = 2 * kEmbedNodeChunk;
 
 struct BitmapWriter {
-    uint32_t bits0 = 0, bits1 = 0;
+    uint32_t bits0 = 0, bits1 = 0;    // entries with mass in the wave's first / second block
+    uint32_t light0 = 0, light1 = 0;  // of those, entries where the block has at most kSmRecVals samples with mass
     int64_t cur = -1;
-    __device__ __forceinline__ void flush(uint32_t *bm, int64_t blk0, int64_t nwords, int lane) {
+    __device__ __forceinline__ void flush(uint32_t *bm, uint32_t *lbm, int64_t blk0, int64_t nwords, int lane) {
         if (cur >= 0 && lane == 0) {
             if (bits0) atomicOr(bm + blk0 * nwords + cur, bits0);
             if (bits1) atomicOr(bm + (blk0 + 1) * nwords + cur, bits1);
+            if (lbm && light0) atomicOr(lbm + blk0 * nwords + cur, light0);
+            if (lbm && light1) atomicOr(lbm + (blk0 + 1) * nwords + cur, light1);
         }
-        bits0 = bits1 = 0;
+        bits0 = bits1 = light0 = light1 = 0;
     }
-    __device__ __forceinline__ void add(int32_t row, uint32_t q, uint32_t *bm, int64_t blk0, int64_t nwords,
-                                        int lane) {
+    __device__ __forceinline__ void add(int32_t row, uint32_t q, uint32_t *bm, uint32_t *lbm, int64_t blk0,
+                                        int64_t nwords, int lane) {
         const uint64_t m = __ballot(q != 0u);
         const int64_t w = row >> 5;
         if (w != cur) {
-            flush(bm, blk0, nwords, lane);
+            flush(bm, lbm, blk0, nwords, lane);
             cur = w;
         }
-        if ((uint32_t)m) bits0 |= 1u << (row & 31);
-        if ((uint32_t)(m >> 32)) bits1 |= 1u << (row & 31);
+        const uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32), bit = 1u << (row & 31);
+        if (lo) bits0 |= bit;
+        if (hi) bits1 |= bit;
+        if (lo && __popc(lo) <= kSmRecVals) light0 |= bit;
+        if (hi && __popc(hi) <= kSmRecVals) light1 |= bit;
     }
 };
 
@@ -118,7 +124,7 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
                                                        const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
                                                        int64_t nwords, int64_t ystride, double scale,
                                                        uint32_t *__restrict__ Y,
-                                                       uint32_t *__restrict__ bm, double *__restrict__ xroot,
+                                                       uint32_t *__restrict__ bm, uint32_t *__restrict__ lbm, double *__restrict__ xroot,
                                                        unsigned long long *__restrict__ colsum,
                                                        double *__restrict__ colsum_f64) {
     extern __shared__ double eslots[];  // nslots * 64
@@ -183,7 +189,7 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
                 const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
                 Y[(int64_t)rows[j] * ystride + s] = q;
                 qsum += q;
-                bw.add(rows[j], q, bm, blk0, nwords, lane);
+                bw.add(rows[j], q, bm, lbm, blk0, nwords, lane);
             }
             if (code & kDstLive) {
                 const double xp = UNW ? fmax(x[0], x[1]) : (double)snodes[k].w1 * x[0] + (double)snodes[k].w2 * x[1];
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
         if (valid) atomicAdd(colsum_f64 + s, ysum);
         return;
     }
-    bw.flush(bm, blk0, nwords, lane);
+    bw.flush(bm, lbm, blk0, nwords, lane);
     if (valid && qsum) atomicAdd(colsum + s, qsum);
 }
 
@@ -209,7 +215,7 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
                                                        const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
                                                        int64_t nwords, int64_t ystride, double scale,
                                                        uint32_t *__restrict__ Y,
-                                                       uint32_t *__restrict__ bm, const double *__restrict__ xroot,
+                                                       uint32_t *__restrict__ bm, uint32_t *__restrict__ lbm, const double *__restrict__ xroot,
                                                        unsigned long long *__restrict__ colsum,
                                                        double *__restrict__ colsum_f64) {
     extern __shared__ double eslots[];
@@ -253,7 +259,7 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
             const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
             Y[(int64_t)rows[j] * ystride + s] = q;
             qsum += q;
-            bw.add(rows[j], q, bm, blk0, nwords, lane);
+            bw.add(rows[j], q, bm, lbm, blk0, nwords, lane);
         }
         if (code & kDstLive)
             eslots[((code >> 16) & 0xffu) * 64 + lane] =
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
         if (valid) atomicAdd(colsum_f64 + s, ysum);
         return;
     }
-    bw.flush(bm, blk0, nwords, lane);
+    bw.flush(bm, lbm, blk0, nwords, lane);
     if (valid && qsum) atomicAdd(colsum + s, qsum);
 }
 
@@ -330,16 +336,21 @@ __global__ __launch_bounds__(64 * kSmWaves) void sm_rect_kernel(SmData d, int64_
 //          (stored position -> sample id through d.perm, if any);
 //   OUT 1: write the packed triangle of stored positions, minus out_base
 //          (a band buffer of a streamed run).
-template <int CHUNK, int WAVES, int OUT>
-__global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int64_t nb, int64_t t_begin,
-                                                                   int64_t t_count,
-                                                                   unsigned long long *__restrict__ work,
-                                                                   float *__restrict__ out, int64_t out_base) {
-    __shared__ uint16_t s_list[WAVES][64 * 32];
+//   SPLIT: entries light in both blocks (d.lbm) take the sparse path of
+//          sm_tile_split (records d.crl), the rest are staged.
+template <int CHUNK, int WAVES, int OUT, bool SPLIT>
+__device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, int64_t t_begin, int64_t t_count,
+                                                   unsigned long long *__restrict__ work, float *__restrict__ out,
+                                                   int64_t out_base) {
+    // 2,048 entry rows (uint16) per window; split tiles: the window's word
+    // records and the sparse accumulator
+    __shared__ __align__(16) uint16_t s_list[WAVES][2048];
     __shared__ uint32_t s_stage[WAVES][CHUNK * 64];
+    __shared__ __align__(16) uint4 s_words[SPLIT ? WAVES : 1][SPLIT ? 64 : 1];
+    __shared__ __align__(16) uint32_t s_acc[SPLIT ? WAVES : 1][SPLIT ? 32 * kAccStride : 4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ty = lane >> 3, tx = lane & 7;
-    uint32_t visited = 0;
+    uint32_t visited = 0, staged = 0, sp_rows = 0, sp_pairs = 0, hf_rows = 0, hf_pairs = 0;
     for (;;) {
         unsigned long long t = 0;
         if (lane == 0) t = atomicAdd(work, 1ull);
@@ -347,10 +358,32 @@ __global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int
         if (t >= (unsigned long long)t_count) break;
         int64_t ra, cb;
         sm_decode_upper(t_begin + (int64_t)t, nb, ra, cb);
+        // block indices are wave-uniform: keep them (and every address
+        // derived from them) in scalar registers
+        ra = __builtin_amdgcn_readfirstlane((int)ra);
+        cb = __builtin_amdgcn_readfirstlane((int)cb);
         const int64_t abase = ra * kSmBlock, bbase = cb * kSmBlock;
         uint32_t acc[4][4];
-        visited += sm_square_tile<CHUNK, false>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords, nullptr, abase,
-                                             bbase, s_list[wave], s_stage[wave], lane, acc);
+        if constexpr (SPLIT) {
+            SquareRows<false> rows;
+            rows.Y = d.Y;
+            rows.stride = d.npad;
+            rows.ok = true;
+            rows.col4 = (uint32_t)((lane < 32) ? abase + lane : bbase + (lane - 32)) * 4u;
+            SplitCounts sc;
+            sm_tile_split<CHUNK, SquareRows<false>>(d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords,
+                                                    d.lbm + cb * d.nwords, d.crl + ra * d.crs, d.crl + cb * d.crs,
+                                                    d.nwords, rows, d.Y, d.npad, abase, bbase, s_list[wave],
+                                                    s_words[wave], s_stage[wave], s_acc[wave], lane, acc, sc);
+            visited += sc.visited;
+            sp_rows += sc.sparse;
+            sp_pairs += sc.pairs;
+            hf_rows += sc.half;
+            hf_pairs += sc.half_pairs;
+        } else {
+            visited += sm_square_tile<CHUNK, false>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords, nullptr,
+                                                 abase, bbase, s_list[wave], s_stage[wave], lane, acc);
+        }
 #pragma unroll
         for (int ii = 0; ii < 4; ii++)
 #pragma unroll
@@ -371,7 +404,38 @@ __global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int
                 }
             }
     }
-    if (d.counter && lane == 0 && visited) atomicAdd(d.counter, (unsigned long long)visited);
+    if (d.counter) {
+        staged = visited - sp_rows - hf_rows;  // the split counts are wave totals
+        if (lane == 0 && visited) {
+            atomicAdd(d.counter + kCntVisited, (unsigned long long)visited);
+            atomicAdd(d.counter + kCntStaged, (unsigned long long)staged);
+            if (sp_rows) {
+                atomicAdd(d.counter + kCntSparseRows, (unsigned long long)sp_rows);
+                atomicAdd(d.counter + kCntSparsePairs, (unsigned long long)sp_pairs);
+            }
+            if (hf_rows) {
+                atomicAdd(d.counter + kCntHalfRows, (unsigned long long)hf_rows);
+                atomicAdd(d.counter + kCntHalfPairs, (unsigned long long)hf_pairs);
+            }
+        }
+    }
+}
+
+template <int CHUNK, int WAVES, int OUT>
+__global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int64_t nb, int64_t t_begin,
+                                                                   int64_t t_count,
+                                                                   unsigned long long *__restrict__ work,
+                                                                   float *__restrict__ out, int64_t out_base) {
+    sm_persistent_body<CHUNK, WAVES, OUT, false>(d, nb, t_begin, t_count, work, out, out_base);
+}
+
+// The split tiles held to 4 waves per SIMD (at most 128 VGPRs): their
+// sparse entries wait on record loads, and occupancy hides that latency.
+template <int CHUNK, int WAVES, int OUT>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(CHUNK <= 8 && WAVES <= 2 ? 4 : 1))) void sm_persistent_split_kernel(
+    SmData d, int64_t nb, int64_t t_begin, int64_t t_count, unsigned long long *__restrict__ work,
+    float *__restrict__ out, int64_t out_base) {
+    sm_persistent_body<CHUNK, WAVES, OUT, true>(d, nb, t_begin, t_count, work, out, out_base);
 }
 
 // Explicit pair list (caller's sample ids; d.pos maps them to stored
@@ -402,12 +466,92 @@ __global__ __launch_bounds__(256) void sm_pairs_kernel(SmData d, int64_t n_pairs
     if (ok) out[i] = sim;
 }
 
+// ------------------------------------------------------------ light records
+
+// PL[b][w] = light entries of block b in words before w (one wave per block).
+__global__ __launch_bounds__(64) void light_prefix_kernel(const uint32_t *__restrict__ lbm, int64_t nwords,
+                                                          uint32_t *__restrict__ PL) {
+    const int64_t b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t *Lw = lbm + b * nwords;
+    uint32_t *P = PL + b * nwords;
+    uint32_t base = 0;
+    for (int64_t w0 = 0; w0 < nwords; w0 += 64) {
+        const int64_t w = w0 + lane;
+        const int c = (w < nwords) ? __popc(Lw[w]) : 0;
+        int incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        if (w < nwords) P[w] = base + (uint32_t)(incl - c);
+        base += (uint32_t)__builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    }
+}
+
+// One thread per (block, word): the record {mask, first 3 values} of each
+// light entry of the word, at the entry's rank, from the block's 32 values
+// of Y in that row.
+__global__ __launch_bounds__(256) void light_records_kernel(const uint32_t *__restrict__ Y, int64_t npad,
+                                                            const uint32_t *__restrict__ lbm,
+                                                            const uint32_t *__restrict__ PL, int64_t nblk,
+                                                            int64_t nwords, uint4 *__restrict__ crl, int64_t crs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk * nwords) return;
+    const int64_t b = t / nwords, w = t - b * nwords;
+    uint32_t bits = lbm[t];
+    uint32_t rank = PL[t];
+    uint4 *out = crl + b * crs;
+    while (bits) {
+        const int i = __ffs(bits) - 1;
+        bits &= bits - 1u;
+        const int64_t row = w * 32 + i;
+        const uint4 *src = reinterpret_cast<const uint4 *>(Y + row * npad + b * 32);
+        uint32_t mask = 0u, v[3] = {0u, 0u, 0u};
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint4 q = src[k];
+            const uint32_t e[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (e[j]) {
+                    mask |= 1u << (4 * k + j);
+                    if (n < 3) v[n] = e[j];
+                    n++;
+                }
+        }
+        out[rank++] = make_uint4(mask, v[0], v[1], v[2]);
+    }
+}
+
+hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t *lbm, int64_t nblk, int64_t nwords,
+                                uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st) {
+    if (nblk <= 0 || nwords <= 0) return hipSuccess;
+    if (crs < nwords * 32 || npad < nblk * kSmBlock) return hipErrorInvalidValue;
+    light_prefix_kernel<<<dim3((unsigned)nblk), 64, 0, st>>>(lbm, nwords, PL);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t n = nblk * nwords;
+    light_records_kernel<<<dim3((unsigned)((n + 255) / 256)), 256, 0, st>>>(Y, npad, lbm, PL, nblk, nwords, crl, crs);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ launchers
 
-__global__ void zero_counter_kernel(unsigned long long *c) { *c = 0ull; }
+__global__ void zero_counter_kernel(unsigned long long *c, int n) {
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
 
 hipError_t launch_zero_counter(unsigned long long *counter, hipStream_t st) {
-    zero_counter_kernel<<<1, 1, 0, st>>>(counter);
+    zero_counter_kernel<<<1, 64, 0, st>>>(counter, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_counters(unsigned long long *counter, int n, hipStream_t st) {
+    if (n <= 0 || n > 64) return hipErrorInvalidValue;
+    zero_counter_kernel<<<1, 64, 0, st>>>(counter, n);
     return hipGetLastError();
 }
 
@@ -443,14 +587,14 @@ static hipError_t launch_embed_t(const EmbedData &e, hipStream_t st) {
     if (e.nseg > 0) {
         embed_seg_kernel<UNW, SUMS><<<dim3(waves, (unsigned)e.nseg), 64, lds, st>>>(
             e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.nlp, e.n, e.npad, e.nwords, ystride, e.scale, e.Y,
-            e.bm, e.xroot, e.colsum, e.colsum_f64);
+            e.bm, e.lbm, e.xroot, e.colsum, e.colsum_f64);
         hipError_t r = hipGetLastError();
         if (r != hipSuccess) return r;
     }
     if (e.ntop > 0)
         embed_top_kernel<UNW, SUMS><<<waves, 64, lds, st>>>(e.nodes, e.top_nodes, e.top_src, e.ntop, e.ab, e.nlp,
                                                             e.n, e.npad, e.nwords, ystride, e.scale, e.Y, e.bm,
-                                                            e.xroot, e.colsum, e.colsum_f64);
+                                                            e.lbm, e.xroot, e.colsum, e.colsum_f64);
     return hipGetLastError();
 }
 
@@ -466,19 +610,30 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
                                       unsigned long long *work, float *out, int64_t out_base, int grid_waves,
                                       hipStream_t st) {
     if (t_count <= 0) return hipSuccess;
-#define CMS_PERSIST(CH, WV)                                                                                      \
+#define CMS_PERSIST(CH, WV, SP)                                                                                  \
     {                                                                                                            \
         const int64_t wg = std::max<int64_t>(1, std::min<int64_t>((t_count + WV - 1) / WV, grid_waves / WV));   \
-        sm_persistent_kernel<CH, WV, OUT><<<dim3((unsigned)wg), 64 * WV, 0, st>>>(d, nb, t_begin, t_count, work, \
-                                                                                   out, out_base);                \
+        if constexpr (SP)                                                                                        \
+            sm_persistent_split_kernel<CH, WV, OUT><<<dim3((unsigned)wg), 64 * WV, 0, st>>>(                     \
+                d, nb, t_begin, t_count, work, out, out_base);                                                   \
+        else                                                                                                     \
+            sm_persistent_kernel<CH, WV, OUT><<<dim3((unsigned)wg), 64 * WV, 0, st>>>(d, nb, t_begin, t_count,   \
+                                                                                   work, out, out_base);          \
     }
+    if (variant >= 6 && (!d.lbm || !d.crl)) return hipErrorInvalidValue;  // the split tiles read the light records
     switch (variant) {
-    case 0: CMS_PERSIST(8, 4); break;
-    case 1: CMS_PERSIST(16, 4); break;
-    case 2: CMS_PERSIST(32, 4); break;
-    case 3: CMS_PERSIST(16, 2); break;
-    case 4: CMS_PERSIST(32, 2); break;
-    default: CMS_PERSIST(16, 8); break;
+    case 0: CMS_PERSIST(8, 4, false); break;
+    case 1: CMS_PERSIST(16, 4, false); break;
+    case 2: CMS_PERSIST(32, 4, false); break;
+    case 3: CMS_PERSIST(16, 2, false); break;
+    case 4: CMS_PERSIST(32, 2, false); break;
+    case 5: CMS_PERSIST(16, 8, false); break;
+    case 6: CMS_PERSIST(8, 2, true); break;
+    case 7: CMS_PERSIST(8, 1, true); break;
+    case 8: CMS_PERSIST(8, 4, true); break;
+    case 9: CMS_PERSIST(16, 2, true); break;
+    case 10: CMS_PERSIST(16, 1, true); break;
+    default: CMS_PERSIST(16, 4, true); break;
     }
 #undef CMS_PERSIST
     return hipGetLastError();

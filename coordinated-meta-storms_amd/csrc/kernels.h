@@ -96,7 +96,24 @@ struct SmData {
     const unsigned long long *colsum;  // per-sample sum of Y (unweighted)
     const int32_t *perm;          // stored position -> caller's sample id (null = identity)
     const int32_t *pos;           // caller's sample id -> stored position (null = identity)
+    // Light entries: bit r of lbm[b * nwords + r / 32] is set when block b
+    // has 1 to kSmRecVals samples with mass at entry r; crl[b * crs + k] =
+    // {mask, v0, v1, v2} for block b's k-th light entry (bit j of mask:
+    // Y[r][32 b + j] > 0, v = those values in ascending sample order, zeros
+    // after the last).
+    const uint32_t *lbm;          // null = none (the split tiles need them)
+    const uint4 *crl;
+    int64_t crs;                  // record stride per block (>= its light entries)
 };
+constexpr int kSmRecVals = 3;
+// Counters of a shared-minimum all-pairs run (SmData.counter[k]):
+constexpr int kCntVisited = 0;    // entries set in both blocks' bitmaps, summed over tiles
+constexpr int kCntStaged = 1;     // of those, entries accumulated through the staged |a - b| path
+constexpr int kCntSparseRows = 2; // entries taken by the sparse path
+constexpr int kCntSparsePairs = 3;  // pairs with mass on both sides in those entries
+constexpr int kCntHalfRows = 4;     // entries light in one block only (half path)
+constexpr int kCntHalfPairs = 5;    // pairs with mass on both sides in those entries
+constexpr int kCounters = 8;
 
 // Persistent all-pairs launch over upper-triangle block tiles
 // [t_begin, t_begin + t_count) of an nb x nb block grid. variant selects the
@@ -105,12 +122,27 @@ struct SmData {
 // minus out_base. `work` is a zeroed device counter. out holds out_cap
 // floats; a launch that could write outside them is refused on the host
 // (hipErrorInvalidValue) before anything is queued.
-constexpr int kSmVariants = 6;
+// Variants: CHUNK x WAVES of the plain tiles (1..6) and of the split tiles
+// (7..12), whose entries light in both blocks take the sparse path.
+constexpr int kSmVariants = 12;
+struct SmVariant {
+    int chunk, waves, split;
+};
+constexpr SmVariant kSmVariantTable[kSmVariants] = {{8, 4, 0},  {16, 4, 0}, {32, 4, 0}, {16, 2, 0},
+                                                    {32, 2, 0}, {16, 8, 0}, {8, 2, 1},  {8, 1, 1},
+                                                    {8, 4, 1},  {16, 2, 1}, {16, 1, 1}, {16, 4, 1}};
+// Light-entry prefix (PL[b][w] = light entries of block b before word w) and
+// the records of every block's light entries in rank order, from Y and the
+// light bitmaps; crs >= the most light entries of any block.
+hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t *lbm, int64_t nblk, int64_t nwords,
+                                uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st);
+constexpr int kSmDefaultVariant = 1;  // without autotune: 16-entry chunk, 4 waves per workgroup, all staged
 hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
                                 unsigned long long *work, float *out, int64_t out_cap, int out_mode,
                                 int64_t out_base, int grid_waves, hipStream_t st);
 
 hipError_t launch_zero_counter(unsigned long long *counter, hipStream_t st);
+hipError_t launch_zero_counters(unsigned long long *counter, int n, hipStream_t st);
 hipError_t launch_calib_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
 
 // Per-sample sum of leaf values over consumed leaf positions (FP64).
@@ -148,6 +180,7 @@ struct EmbedData {
     double scale;     // 2^K
     uint32_t *Y;
     uint32_t *bm;     // zeroed by the caller; bits are ORed in
+    uint32_t *lbm;    // optional light-entry bitmaps (SmData.lbm), zeroed by the caller; bits are ORed in
     double *xroot;    // nseg x npad
     int unweighted;   // presence propagation, coefficient w (EmbedNode.beta* hold w)
     int sums_only;    // first pass: FP64 column sums of y into colsum_f64, nothing else

@@ -238,6 +238,259 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
     return visited;
 }
 
+// Split tile (aligned 32 x 32 blocks only): the entries set in both blocks'
+// bitmaps split by the blocks' light bitmaps LA, LB (bit set: the block has
+// at most kSmRecVals samples with mass at that entry).
+//
+//   - sparse entries, light in both blocks: at most 3 x 3 pairs with a
+//     shared minimum, where staging would read 2 x 32 values of Y. Their
+//     records {mask, values} sit in rank order of the light bits (RA[k] for
+//     the k-th light entry of block A), so a tile's sparse entries read
+//     nearby 16-byte records and never touch Y. One lane per entry adds the
+//     minima into the wave's accumulator ACC (LDS atomics);
+//   - half entries, light in one block only: one lane per entry reads the
+//     other block's 32 values of Y (one 128-byte line) and the light block's
+//     record and adds the (at most 3 x 32) minima into ACC: 128 + 16 bytes
+//     instead of 256;
+//   - staged entries, the rest: the |a - b| path of sm_tile_accumulate.
+//
+// ACC is 32 x 33 uint32 (row = row-block sample, padded against bank
+// conflicts). Windows of 64 bitmap
+// words (2,048 entries). L (2,048 uint16) holds the staged entries' rows from
+// the front, then the half entries' rows (bit 15: the light block is B), and
+// the sparse entries' rows from the back (disjoint subsets of the window's
+// entries); W (64 uint4) holds each word's light bits and the ranks of the
+// light entries before it, from which record positions follow. Integer sums:
+// the pair's shared sum is (sa + sb - sum |a - b|) / 2 over the staged
+// entries plus ACC.
+struct SplitCounts {
+    // wave totals: entries, sparse entries and their pairs with mass, half
+    // entries and their pairs with mass
+    uint32_t visited = 0, sparse = 0, pairs = 0, half = 0, half_pairs = 0;
+};
+
+constexpr int kAccStride = 33;
+
+template <int CHUNK, class Rows>
+__device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t *B0, const uint32_t *LA,
+                                              const uint32_t *LB, const uint4 *RA, const uint4 *RB, int64_t nwords,
+                                              const Rows &rows, const uint32_t *Y, int64_t ystride, int64_t abase,
+                                              int64_t bbase, uint16_t *L, uint4 *W, uint32_t *S, uint32_t *ACC,
+                                              int lane, uint32_t (&acc)[4][4], SplitCounts &cnt) {
+    static_assert(CHUNK % 2 == 0, "rows are staged in pairs");
+    const int ty = lane >> 3, tx = lane & 7;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0;
+    for (int k = lane; k < 32 * kAccStride; k += 64) ACC[k] = 0u;
+    uint32_t cs = 0;                // this lane's column summed over the staged rows
+    uint32_t baseA = 0, baseB = 0;  // light entries of each block before the window (ranks)
+    uint32_t pairs_lane = 0;        // this lane's sparse pairs with mass in the window (<= 32 x 9)
+    uint32_t hp_lane = 0;           // this lane's half pairs with mass in the window (<= 32 x 96)
+    for (int64_t wb = 0; wb < nwords; wb += 64) {
+        const int64_t wi = wb + lane;
+        uint32_t a = 0u, b = 0u, la = 0u, lb = 0u;
+        if (wi < nwords) {
+            a = A0[wi];
+            b = B0[wi];
+            la = LA[wi];
+            lb = LB[wi];
+        }
+        const uint32_t both = a & b, sw = both & la & lb, hw = both & (la ^ lb), dw = both & ~(la | lb);
+        // 16-bit counts packed two per scan (totals <= 2,048): (staged | half << 16),
+        // (light A | light B << 16), sparse
+        const int c1 = __popc(dw) | (__popc(hw) << 16), c2 = __popc(la) | (__popc(lb) << 16), c3 = __popc(sw);
+        int i1 = c1, i2 = c2, i3 = c3;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v1 = __shfl_up(i1, off, 64), v2 = __shfl_up(i2, off, 64), v3 = __shfl_up(i3, off, 64);
+            if (lane >= off) {
+                i1 += v1;
+                i2 += v2;
+                i3 += v3;
+            }
+        }
+        const int t1 = __builtin_amdgcn_readfirstlane(__shfl(i1, 63, 64));
+        const int t2 = __builtin_amdgcn_readfirstlane(__shfl(i2, 63, 64));
+        const int tots = __builtin_amdgcn_readfirstlane(__shfl(i3, 63, 64));
+        const int totd = t1 & 0xffff, toth = t1 >> 16;
+        const uint32_t winA = baseA, winB = baseB;
+        baseA += (uint32_t)(t2 & 0xffff);
+        baseB += (uint32_t)(t2 >> 16);
+        if (totd + toth + tots == 0) continue;
+        cnt.visited += (uint32_t)(totd + toth + tots);
+        cnt.sparse += (uint32_t)tots;
+        cnt.half += (uint32_t)toth;
+        const int e1 = i1 - c1, e2 = i2 - c2;  // this lane's exclusive offsets
+        int posd = e1 & 0xffff, posh = totd + (e1 >> 16), poss = i3 - c3;
+        __builtin_amdgcn_wave_barrier();
+        if (tots + toth) W[lane] = make_uint4(la, lb, (uint32_t)(e2 & 0xffff), (uint32_t)(e2 >> 16));
+        for (uint32_t m = dw; m; m &= m - 1u) L[posd++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
+        for (uint32_t m = hw; m; m &= m - 1u) {
+            const int bit = __ffs(m) - 1;
+            L[posh++] = (uint16_t)((lane * 32 + bit) | (((lb >> bit) & 1u) << 15));
+        }
+        for (uint32_t m = sw; m; m &= m - 1u) L[2047 - poss++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // staged entries: as sm_tile_accumulate, from L[0, totd); the first
+        // chunk's loads are issued before the sparse and half entries are
+        // summed, so their latency is hidden behind that work
+        Rows win = rows;
+        win.window(wb * 32);
+        uint32_t v[CHUNK];
+        auto load_chunk = [&](int j) {
+            const uint32_t first =
+                (uint32_t)__builtin_amdgcn_readfirstlane((int)*reinterpret_cast<const uint32_t *>(L + j)) & 0xffffu;
+#pragma unroll
+            for (int k = 0; k < CHUNK; k += 2) {
+                const uint32_t pr =
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)*reinterpret_cast<const uint32_t *>(L + j + k));
+                const uint32_t r0 = (j + k < totd) ? (pr & 0xffffu) : first;
+                const uint32_t r1 = (j + k + 1 < totd) ? (pr >> 16) : first;
+                v[k] = win.load(r0);
+                v[k + 1] = win.load(r1);
+            }
+        };
+        if (totd) load_chunk(0);
+        // sparse entries: one per lane
+        for (int j = 0; j < tots; j += 64) {
+            const int k = j + lane;
+            if (k < tots) {
+                const uint32_t r = L[2047 - k];
+                const uint4 wd = W[r >> 5];
+                const uint32_t below = (1u << (r & 31u)) - 1u;
+                const uint4 qa = RA[winA + wd.z + __popc(wd.x & below)];
+                const uint4 qb = RB[winB + wd.w + __popc(wd.y & below)];
+                const uint32_t va[3] = {qa.y, qa.z, qa.w}, vb[3] = {qb.y, qb.z, qb.w};
+                const int na = __popc(qa.x), nb = __popc(qb.x);
+                uint32_t ia[3], ib[3], ma = qa.x, mb = qb.x;
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    ia[t] = ma ? (uint32_t)(__ffs(ma) - 1) : 0u;
+                    ma &= ma - 1u;
+                    ib[t] = mb ? (uint32_t)(__ffs(mb) - 1) : 0u;
+                    mb &= mb - 1u;
+                }
+#pragma unroll
+                for (int t = 0; t < 3; t++)
+#pragma unroll
+                    for (int u = 0; u < 3; u++)
+                        if (t < na && u < nb) atomicAdd(ACC + ia[t] * kAccStride + ib[u], min(va[t], vb[u]));
+                pairs_lane += (uint32_t)(na * nb);
+            }
+        }
+        // half entries: one per lane, which reads the heavy block's 32 values
+        // of the row (one 128-byte line) and the light block's record
+        for (int j = 0; j < toth; j += 64) {
+            const int k = j + lane;
+            if (k < toth) {
+                const uint32_t e = L[totd + k];
+                const uint32_t r = e & 0x7ffu;
+                const bool blight = (e >> 15) != 0u;  // the light block is B, the heavy one A
+                const uint4 wd = W[r >> 5];
+                const uint32_t below = (1u << (r & 31u)) - 1u;
+                const uint4 q = blight ? RB[winB + wd.w + __popc(wd.y & below)] : RA[winA + wd.z + __popc(wd.x & below)];
+                const uint4 *hrow = reinterpret_cast<const uint4 *>(Y + (wb * 32 + r) * ystride + (blight ? abase : bbase));
+                uint4 h[8];
+#pragma unroll
+                for (int c = 0; c < 8; c++) h[c] = hrow[c];
+                const uint32_t vl[3] = {q.y, q.z, q.w};
+                const int nl = __popc(q.x);
+                uint32_t ml = q.x, il[3];
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    il[t] = ml ? (uint32_t)(__ffs(ml) - 1) : 0u;
+                    ml &= ml - 1u;
+                }
+                // ACC index of (light sample il, heavy sample c)
+                const uint32_t step = blight ? (uint32_t)kAccStride : 1u;
+                uint32_t hn = 0u;
+#pragma unroll
+                for (int c = 0; c < 32; c++) {
+                    const uint4 hq = h[c >> 2];
+                    const uint32_t hv = (c & 3) == 0 ? hq.x : (c & 3) == 1 ? hq.y : (c & 3) == 2 ? hq.z : hq.w;
+                    if (hv) {
+                        hn++;
+#pragma unroll
+                        for (int t = 0; t < 3; t++)
+                            if (t < nl)
+                                atomicAdd(ACC + (blight ? il[t] : il[t] * kAccStride) + (uint32_t)c * step,
+                                          min(vl[t], hv));
+                    }
+                }
+                hp_lane += hn * (uint32_t)nl;
+            }
+        }
+        if (tots | toth) {
+            uint32_t p = pairs_lane, h = hp_lane;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                p += (uint32_t)__shfl_xor((int)p, off, 64);
+                h += (uint32_t)__shfl_xor((int)h, off, 64);
+            }
+            cnt.pairs += (uint32_t)__builtin_amdgcn_readfirstlane((int)p);
+            cnt.half_pairs += (uint32_t)__builtin_amdgcn_readfirstlane((int)h);
+            pairs_lane = hp_lane = 0u;
+        }
+        if (totd == 0) continue;
+        for (int j = 0; j < totd; j += CHUNK) {
+            const int kn = min(CHUNK, totd - j);
+            if (kn == CHUNK) {
+#pragma unroll
+                for (int k = 0; k < CHUNK; k++) cs += v[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < CHUNK; k++) cs += (k < kn) ? v[k] : 0u;
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int k = 0; k < CHUNK; k++) S[k * 64 + lane] = v[k];
+            if (kn & 1) S[kn * 64 + lane] = 0u;  // rows are accumulated in pairs
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (j + CHUNK < totd) load_chunk(j + CHUNK);
+            auto rows2 = [&](int k) {
+                const uint4 a0 = *reinterpret_cast<const uint4 *>(S + k * 64 + ty * 4);
+                const uint4 b0 = *reinterpret_cast<const uint4 *>(S + k * 64 + 32 + tx * 4);
+                const uint4 a1 = *reinterpret_cast<const uint4 *>(S + (k + 1) * 64 + ty * 4);
+                const uint4 b1 = *reinterpret_cast<const uint4 *>(S + (k + 1) * 64 + 32 + tx * 4);
+                const uint32_t av0[4] = {a0.x, a0.y, a0.z, a0.w}, bv0[4] = {b0.x, b0.y, b0.z, b0.w};
+                const uint32_t av1[4] = {a1.x, a1.y, a1.z, a1.w}, bv1[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) SmOp::add2(acc[ii][jj], av0[ii], bv0[jj], av1[ii], bv1[jj]);
+            };
+            if (kn == CHUNK) {
+#pragma unroll
+                for (int k = 0; k < CHUNK; k += 2) rows2(k);
+            } else {
+                for (int k = 0; k < kn; k += 2) rows2(k);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    uint32_t sa[4], sb[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        sa[i] = (uint32_t)__shfl((int)cs, ty * 4 + i, 64);
+        sb[i] = (uint32_t)__shfl((int)cs, 32 + tx * 4 + i, 64);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            acc[i][j] = SmOp::finish(acc[i][j], sa[i], sb[j]) + ACC[(ty * 4 + i) * kAccStride + tx * 4 + j];
+}
+
 // Similarity from a pair's shared sum: weighted, acc / (100 * 2^K)
 // (inv_scale = 1 / (100 * 2^K)); unweighted, the shared weight over the
 // weight of either sample, acc / (W_a + W_b - acc) with W the column sums of

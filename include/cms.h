@@ -74,14 +74,15 @@ enum {
 typedef struct cms_opts {
     int kernel;       /* CMS_KERNEL_* (default WALK) */
     int storage;      /* CMS_STORE_* (default F32; WALK only) */
-    int tile_a;       /* SHAREDMIN row-block size (0 = autotuned/default) */
-    int tile_b;       /* SHAREDMIN column-block size (0 = default) */
-    int chunk;        /* entries staged per LDS chunk (0 = default) */
+    int tile_a;       /* SHAREDMIN row-block samples per wave tile: 0 or 32 (others CMS_E_INVAL) */
+    int tile_b;       /* SHAREDMIN column-block samples per wave tile: 0 or 32 */
+    int chunk;        /* SHAREDMIN entries staged per LDS chunk: 0 (any) or 8, 16, 32; restricts
+                         the variants the default choice and the autotuner use */
     int use_graph;    /* capture per-band tile launches in a hipGraph (SHAREDMIN) */
     int metric;       /* CMS_METRIC_* (default WEIGHTED) */
     int bin;          /* SHAREDMIN: store samples binned by sparsity (densest first) */
     int autotune;     /* SHAREDMIN: time the persistent-kernel variants once, keep the fastest */
-    int variant;      /* SHAREDMIN: 1..6 force a persistent-kernel variant (0 = default/tuned) */
+    int variant;      /* SHAREDMIN: 1..12 force a persistent-kernel variant (0 = default/tuned) */
     int embed_slice;  /* SHAREDMIN: samples per embedding slice, multiple of 64 (0 = by a 16 GiB layout) */
     int reserved[3];
 } cms_opts;
@@ -195,8 +196,14 @@ typedef struct cms_stats {
     int scale_bits;            /* SHAREDMIN fixed-point scale (2^scale_bits per percent) */
     int entries;               /* SHAREDMIN active embedding entries */
     int variant;               /* SHAREDMIN persistent-kernel variant in use (1..6) */
-    double tune_ms[6];         /* autotuner timings per variant (0 = not timed) */
+    double tune_ms[12];        /* autotuner timings per variant (0 = not timed) */
     int64_t bands;             /* bands of the last streamed run */
+    int64_t entries_visited;   /* SHAREDMIN: entries set in both blocks' bitmaps, summed over tiles */
+    int64_t entries_staged;    /*   of those, accumulated for all 32 x 32 pairs of the tile */
+    int64_t entries_sparse;    /*   of those, taken by the sparse path (at most 3 samples with mass in each block) */
+    int64_t sparse_pairs;      /*   pairs with mass on both sides in the sparse entries */
+    int64_t entries_half;      /*   of those, taken by the half path (at most 3 samples with mass in one block) */
+    int64_t half_pairs;        /*   pairs with mass on both sides in the half entries */
 } cms_stats;
 int cms_get_stats(cms_ctx *c, cms_stats *s);
 

@@ -30,13 +30,24 @@ def test_binned_and_variants_identical(cms, data, metric):
     print(f"binned variants metric={metric}", flush=True)
     tree, abd = data
     ref = _plain(cms, tree, abd, metric)
-    for variant in range(1, 7):
+    visited = None
+    for variant in range(1, 13):  # 7..12 take the nearly empty entries out of the staged path
         with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric, bin=1, variant=variant) as ctx:
             ctx.set_tree(tree)
             ctx.set_samples_csr(abd)
             got = ctx.all_pairs()
-            assert ctx.stats().variant == variant
+            st = ctx.stats()
+            assert st.variant == variant
         np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+        # every variant visits the same entries; the sparse ones stage fewer
+        visited = st.entries_visited if visited is None else visited
+        assert st.entries_visited == visited
+        assert st.entries_staged + st.entries_sparse + st.entries_half == st.entries_visited
+        if variant >= 7:
+            assert st.entries_sparse > 0 and st.sparse_pairs >= st.entries_sparse
+            assert st.entries_half > 0 and st.half_pairs >= st.entries_half
+        else:
+            assert st.entries_sparse == 0 and st.entries_half == 0
 
 
 def test_autotune_picks_a_variant(cms, data):
@@ -48,7 +59,7 @@ def test_autotune_picks_a_variant(cms, data):
         ctx.set_samples_csr(abd)
         got = ctx.all_pairs()
         st = ctx.stats()
-        assert 1 <= st.variant <= 6
+        assert 1 <= st.variant <= 12
         assert all(t > 0 for t in st.tune_ms)
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
 
@@ -122,11 +133,11 @@ def test_sliced_embedding_identical(cms, data, metric):
 
 @pytest.fixture(scope="module")
 def wide(cms, gpu_ok):
-    # 15,000 samples: 110,215 block tiles, enough for the tuner to time its
-    # variants on real tiles (12 segments of 8,192) and keep their results
+    # 20,500 samples: 205,761 block tiles, enough for the tuner to time its
+    # 12 variants on real tiles (24 segments of 8,192) and keep their results
     tree = cms.synth_tree(1077, 2000, 0.05)
     cp = cms.synth_copy_numbers(3077, 2000, 15)
-    counts = cms.synth_counts(2077, 15000, 2000, 5, 60, 1.1, 30, 0.5)
+    counts = cms.synth_counts(2077, 20500, 2000, 5, 60, 1.1, 30, 0.5)
     return tree, cms.counts_to_abundance(counts, cp)
 
 
@@ -141,14 +152,14 @@ def test_autotune_on_real_tiles(cms, wide):
         ctx.set_tree(tree)
         ctx.set_samples_csr(abd)
         ref = ctx.all_pairs()
-        rows_ref = ctx.stats().pair_node_updates
+        rows_ref = ctx.stats().entries_visited
     with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, autotune=1) as ctx:
         ctx.set_tree(tree)
         ctx.set_samples_csr(abd)
         got = ctx.all_pairs()
         st = ctx.stats()
         assert all(t > 0 for t in st.tune_ms)
-        assert st.pair_node_updates == rows_ref  # tuned tiles counted once
+        assert st.entries_visited == rows_ref  # tuned tiles counted once
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
     for use_graph in (0, 1):
         with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, autotune=1, use_graph=use_graph) as ctx:
@@ -159,9 +170,9 @@ def test_autotune_on_real_tiles(cms, wide):
             def fn(first, count, sims):
                 stored[first:first + count] = sims
 
-            ctx.all_pairs_stream(fn, band_pairs=105_000_000)  # first band: 348 of 469 row blocks, ~103,000 tiles
+            ctx.all_pairs_stream(fn, band_pairs=190_000_000)  # first band: about 440 of 641 row blocks, ~198,000 tiles
             order = ctx.sample_order()
-            assert ctx.stats().pair_node_updates == rows_ref
+            assert ctx.stats().entries_visited == rows_ref
         assert not np.isnan(stored).any()
         pi, qi = np.triu_indices(n, 1)
         a, b = order[pi], order[qi]
