@@ -490,39 +490,54 @@ __global__ __launch_bounds__(64) void light_prefix_kernel(const uint32_t *__rest
     }
 }
 
-// One thread per (block, word): the record {mask, first 3 values} of each
-// light entry of the word, at the entry's rank, from the block's 32 values
-// of Y in that row.
-__global__ __launch_bounds__(256) void light_records_kernel(const uint32_t *__restrict__ Y, int64_t npad,
-                                                            const uint32_t *__restrict__ lbm,
-                                                            const uint32_t *__restrict__ PL, int64_t nblk,
-                                                            int64_t nwords, uint4 *__restrict__ crl, int64_t crs) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nblk * nwords) return;
-    const int64_t b = t / nwords, w = t - b * nwords;
-    uint32_t bits = lbm[t];
-    uint32_t rank = PL[t];
+// One wave per (block, 64-word window): the window's light entries are
+// listed in LDS, then each half wave reads one entry's 32 values of Y (one
+// 128-byte line) and writes its record {mask, values of the set bits in
+// ascending sample order, zeros after them} at the entry's rank.
+__global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__restrict__ Y, int64_t npad,
+                                                           const uint32_t *__restrict__ lbm,
+                                                           const uint32_t *__restrict__ PL, int64_t nwords,
+                                                           uint4 *__restrict__ crl, int64_t crs) {
+    __shared__ uint16_t rows[2048];
+    __shared__ uint32_t wrank[64];
+    const int64_t b = blockIdx.x, wb = (int64_t)blockIdx.y * 64;
+    const int lane = threadIdx.x;
+    const int64_t w = wb + lane;
+    const uint32_t bits = (w < nwords) ? lbm[b * nwords + w] : 0u;
+    const int c = __popc(bits);
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    const int tot = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    if (tot == 0) return;
+    int pos = incl - c;
+    // rank of the window's first light entry: the block prefix at its first word
+    wrank[lane] = (w < nwords) ? PL[b * nwords + w] - (uint32_t)(incl - c) : 0u;
+    for (uint32_t m = bits; m; m &= m - 1u) rows[pos++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
+    __syncthreads();
+    const uint32_t rank0 = wrank[0];
+    const int half = lane >> 5, col = lane & 31;
     uint4 *out = crl + b * crs;
-    while (bits) {
-        const int i = __ffs(bits) - 1;
-        bits &= bits - 1u;
-        const int64_t row = w * 32 + i;
-        const uint4 *src = reinterpret_cast<const uint4 *>(Y + row * npad + b * 32);
-        uint32_t mask = 0u, v[3] = {0u, 0u, 0u};
-        int n = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const uint4 q = src[k];
-            const uint32_t e[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (e[j]) {
-                    mask |= 1u << (4 * k + j);
-                    if (n < 3) v[n] = e[j];
-                    n++;
-                }
+    for (int k0 = 0; k0 < tot; k0 += 2) {
+        const int k = k0 + half;
+        const bool in = k < tot;
+        const int64_t row = wb * 32 + (in ? rows[k] : rows[k0]);
+        const uint32_t v = Y[row * npad + b * kSmBlock + col];
+        const uint64_t bal = __ballot(in && v != 0u);
+        const uint32_t mask = half ? (uint32_t)(bal >> 32) : (uint32_t)bal;
+        const uint32_t below = mask & ((1u << col) - 1u);
+        const int r = __popc(below), n = __popc(mask);
+        if (in) {
+            uint32_t *rec = reinterpret_cast<uint32_t *>(out + rank0 + (uint32_t)k);
+            if (v != 0u && r < kSmRecVals) rec[1 + r] = v;
+            if (col == 0) {
+                rec[0] = mask;
+                for (int f = n; f < kSmRecVals; f++) rec[1 + f] = 0u;
+            }
         }
-        out[rank++] = make_uint4(mask, v[0], v[1], v[2]);
     }
 }
 
@@ -533,8 +548,8 @@ hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t 
     light_prefix_kernel<<<dim3((unsigned)nblk), 64, 0, st>>>(lbm, nwords, PL);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int64_t n = nblk * nwords;
-    light_records_kernel<<<dim3((unsigned)((n + 255) / 256)), 256, 0, st>>>(Y, npad, lbm, PL, nblk, nwords, crl, crs);
+    light_records_kernel<<<dim3((unsigned)nblk, (unsigned)((nwords + 63) / 64)), 64, 0, st>>>(Y, npad, lbm, PL, nwords,
+                                                                                             crl, crs);
     return hipGetLastError();
 }
 
