@@ -63,13 +63,15 @@ def cpu_model() -> str:
 
 
 def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, sample_n, pairs):
-    """Reference Calc_sim timed on the host cores over a bounded pair sample,
-    plus the accuracy spot-check of the GPU results on pairs of that sample."""
+    """Reference Calc_sim timed on the host cores over a bounded pair sample
+    (every pair of a seeded random subset of sample_n samples), plus the
+    accuracy spot-check of the GPU results on pairs of that subset."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    sub = counts.subset(range(sample_n))
+    pick = np.sort(np.random.default_rng(5).choice(n_all, sample_n, replace=False))
+    sub = counts.subset(pick)
     res = {}
     drv = orc.ref_binary("ref_driver_O3")
     if drv:
@@ -83,9 +85,10 @@ def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, s
             if r.returncode == 0:
                 j = json.loads(r.stdout.strip().splitlines()[-1])
                 res = {"value": j["pairs_per_sec"], "unit": "sample-pairs/s", "cores": threads, "kind": "reference",
-                       "sample": f"first {j['pairs']} pairs of the upper triangle of the first {sample_n} samples "
-                                 f"(99,322-leaf tree); reference _Comp_Tree::Calc_sim built -O3 -ffp-contract=off, "
-                                 f"OpenMP dynamic,1 on {threads} threads, wall clock {j['seconds']:.2f} s"}
+                       "sample": f"{j['pairs']} pairs: the upper triangle of {sample_n} samples drawn at random "
+                                 f"(seed 5) from the {n_all} (99,322-leaf tree); reference _Comp_Tree::Calc_sim built "
+                                 f"-O3 -ffp-contract=off, OpenMP dynamic,1 on {threads} threads, wall clock "
+                                 f"{j['seconds']:.2f} s"}
                 # the reference as shipped (its Makefile builds -O0), on a tenth of the pairs
                 drv0 = orc.ref_binary("ref_driver_O0")
                 if drv0:
@@ -103,18 +106,18 @@ def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, s
         got = orc.all_pairs_f32(tree, dense, threads, pairs)
         sec = time.perf_counter() - t0
         res = {"value": len(got) / sec, "unit": "sample-pairs/s", "cores": threads, "kind": "port",
-               "sample": f"first {len(got)} pairs of {sample_n} samples, oracle FP32 walk, {threads} threads"}
+               "sample": f"{len(got)} pairs of {sample_n} random samples, oracle FP32 walk, {threads} threads"}
     res["cpu_model"] = cpu_model()
-    # accuracy of the GPU outputs on 2,000 pairs of the sample vs the oracle
+    # accuracy of the GPU outputs on 2,000 pairs of the subset vs the oracle
     rng = np.random.default_rng(7)
     ii = rng.integers(0, sample_n, 4000)
     jj = rng.integers(0, sample_n, 4000)
     keep = ii != jj
     ii, jj = np.minimum(ii, jj)[keep][:2000].astype(np.int32), np.maximum(ii, jj)[keep][:2000].astype(np.int32)
-    dense = abd.subset(range(sample_n)).dense(tree.leaf_n)
+    dense = abd.subset(pick).dense(tree.leaf_n)
     f64 = orc.pairs_f64(tree, dense, ii, jj, threads=threads)
     f32 = orc.pairs_f32(tree, dense, ii, jj, threads=threads)
-    p = cms.packed_index(ii, jj, n_all)
+    p = cms.packed_index(pick[ii], pick[jj], n_all)
     acc = {}
     if gpu_sm is not None:
         e = np.abs(gpu_sm[p].astype(np.float64) - f64)
@@ -129,13 +132,123 @@ def cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n_all, threads, s
     return res, acc
 
 
+def exact_pair_entries(tree, abd, sample_n, seed=9):
+    """Mean number of entries (child edges) where both samples of a pair have
+    mass, over every pair of a seeded random subset of sample_n samples: the
+    work an exact sparse intersection would do per pair. Sample sets of the
+    entries are propagated up the schedule as bitsets."""
+    import numpy as np
+    n = abd.n
+    pick = np.sort(np.random.default_rng(seed).choice(n, min(sample_n, n), replace=False))
+    sub = abd.subset(pick)
+    m = sub.n
+    words = (m + 63) // 64
+    leaf = np.zeros((tree.leaf_n, words), np.uint64)
+    rows = np.repeat(np.arange(m), np.diff(sub.rowptr))
+    np.bitwise_or.at(leaf, (sub.col, rows // 64), np.uint64(1) << (rows % 64).astype(np.uint64))
+    pop = np.array([bin(i).count("1") for i in range(1 << 16)], np.int64)
+
+    def popcount(bs):
+        v = bs.view(np.uint16)
+        return int(pop[v].sum())
+
+    slots = {}
+    pairs_total = 0
+    for i in range(tree.order_n):
+        ch = []
+        for c in (int(tree.order_1[i]), int(tree.order_2[i])):
+            ch.append(leaf[c] if c >= 0 else slots.pop(c))
+        for bs in ch:
+            k = popcount(bs)
+            pairs_total += k * (k - 1) // 2
+        slots[int(tree.order_d[i])] = ch[0] | ch[1]
+    return pairs_total / (m * (m - 1) / 2), m
+
+
+def _wall(cmd, env):
+    t0 = time.perf_counter()
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    return time.perf_counter() - t0, r
+
+
+def _cuda_comp_times(stdout):
+    """The '*** load ... s, gpu ... s ... write ... s' line cuda-comp prints."""
+    import re
+    m = re.search(r"\*\*\* load ([0-9.]+) s, gpu(?: \+ write)? ([0-9.]+) s \(pair kernel ([0-9.]+) ms\)"
+                  r"(?:, write ([0-9.]+) s)?", stdout)
+    if not m:
+        return {}
+    out = {"load_s": float(m.group(1)), "gpu_s": float(m.group(2)), "pair_kernel_ms": float(m.group(3))}
+    if m.group(4):
+        out["write_s"] = float(m.group(4))
+    return out
+
+
+def end_to_end(cms, tree, cp, counts, threads, cpu_samples, with_cpu):
+    """cuda-comp run as a program, files in and out, wall clock: the workload's
+    10,000 samples from a sparse table file (parse, copy-number correction,
+    upload, embedding, pair kernel, download, .dist text or packed binary
+    written); and, on one random subset of cpu_samples samples in the
+    reference's text table, cuda-comp beside the reference `comp` as its
+    Makefile builds it (-O0; the -O3 build of comp_sam.cpp crashes at start)
+    (parse, pair loop on the host cores, .dist written)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    exe = os.path.join(cms.BIN_DIR, "cuda-comp")
+    res = {}
+    with tempfile.TemporaryDirectory() as d:
+        cms.write_db(d, "bench", tree, cp)
+        env = dict(os.environ, ParallelMETA=d)
+        sparse = os.path.join(d, "table.cmst")
+        cms.write_table_csr(sparse, counts, tree.leaf_n)
+        gpu = {"samples": counts.n, "input": "sparse table file", "input_bytes": os.path.getsize(sparse)}
+        for fmt, ext in (("dist", "dist"), ("binary", "bin")):
+            out = os.path.join(d, "out." + ext)
+            sec, r = _wall([exe, "-T", sparse, "-D", "G", "-o", out, "--kernel", "sharedmin", "--format", fmt], env)
+            if r.returncode != 0:
+                log(f"cuda-comp {fmt} failed: {r.stderr[-300:]}")
+                continue
+            gpu[fmt] = {"wall_s": sec, "output_bytes": os.path.getsize(out), **_cuda_comp_times(r.stdout)}
+            os.remove(out)
+            log(f"end to end, cuda-comp --format {fmt}: {sec:.2f} s ({gpu[fmt]})")
+        res["cuda_comp_sharedmin"] = gpu
+        if with_cpu:
+            comp = orc.ref_binary("comp_O0")
+            pick = np.sort(np.random.default_rng(6).choice(counts.n, cpu_samples, replace=False))
+            table = os.path.join(d, "small.txt")
+            cms.write_table(table, counts.subset(pick), tree.leaf_n)
+            side = {"samples": cpu_samples, "input": "text table", "input_bytes": os.path.getsize(table),
+                    "sample": f"{cpu_samples} samples drawn at random (seed 6)"}
+            for name, cmd in (("cuda_comp_walk", [exe, "-T", table, "-D", "G", "-o", os.path.join(d, "g.dist")]),
+                              ("reference_comp_as_shipped_O0",
+                               [comp, "-T", table, "-D", "G", "-o", os.path.join(d, "c.dist"), "-M", "0", "-t",
+                                str(threads)] if comp else None)):
+                if cmd is None:
+                    continue
+                sec, r = _wall(cmd, env)
+                if r.returncode == 0:
+                    side[name] = {"wall_s": sec, **(_cuda_comp_times(r.stdout) if "cuda" in name else {})}
+                    log(f"end to end, {name} on {cpu_samples} samples: {sec:.2f} s")
+                else:
+                    log(f"end to end, {name} failed ({r.returncode}): {r.stderr[-300:]}")
+            if "reference_comp_as_shipped_O0" in side and "cuda_comp_walk" in side:
+                side["reference_threads"] = threads
+                side["outputs_identical"] = open(os.path.join(d, "g.dist"), "rb").read() == \
+                    open(os.path.join(d, "c.dist"), "rb").read()
+            res["same_text_table"] = side
+    return res
+
+
 def roofline(st, kernel, traffic=None):
     """Roofline entry of a shared-minimum pair kernel from its work counters:
-    VALU (algorithmic lane-ops: 17/16 per visited pair-entry, i.e. per pair of
-    a visited 32 x 32 tile-row: one v_sad_u32 each plus one column-sum add per
-    staged row and lane) and HBM (algorithmic bytes: 2 x 32 x 4 B of Y
-    per visited tile-row), each over the kernel time from HIP events; `bound`
-    is the one closer to its peak."""
+    VALU (algorithmic lane-ops executed: 17/16 per pair of a staged 32 x 32
+    tile-row, one v_sad_u32 each plus one column-sum add per staged row and
+    lane; one minimum and one add per pair with mass of a sparse or half
+    entry) and HBM (algorithmic bytes: 2 x 32 x 4 B of Y per staged row, two
+    16-byte records per sparse entry, 32 x 4 B of Y and one record per half
+    entry), each over the kernel time from HIP events; `bound` is the one
+    closer to its peak."""
     t = st.kernel_ms * 1e-3
     valu = st.valu_ops / t / 1e12 if t > 0 else 0.0
     hbm = st.bytes_moved / t / 1e9 if t > 0 else 0.0
@@ -145,7 +258,7 @@ def roofline(st, kernel, traffic=None):
     main.update({"traffic": traffic, "kernel": kernel, "kernel_ms": st.kernel_ms,
                  "valu": {"achieved_tops": valu, "frac": fv, "ops": st.valu_ops},
                  "hbm": {"achieved_gbs": hbm, "frac": fh, "algorithmic_bytes": st.bytes_moved},
-                 "pair_entries": st.pair_node_updates, "ops_per_pair_entry": SM_OPS_PER_PAIR_ENTRY,
+                 "pair_entries": st.pair_node_updates, "ops_per_staged_pair_entry": SM_OPS_PER_PAIR_ENTRY,
                  "tune_ms": list(st.tune_ms)})
     return main
 
@@ -198,6 +311,22 @@ def run_batched(cms, a):
     ctx.all_pairs_stream(fn, a.band_pairs)
     stream_s = time.perf_counter() - t2
     st = ctx.stats()
+    # the same stream with the other band-launch mode (hipGraph replay or
+    # direct launches), the tuned variant kept: what the graph buys
+    other = None
+    if a.graph_compare:
+        ctx.set_opts(cms.KERNEL_SHAREDMIN, bin=a.bin, autotune=a.autotune, variant=st.variant,
+                     use_graph=0 if a.graph else 1)
+        got2 = {"pairs": 0}
+
+        def fn2(first, count, sims):
+            got2["pairs"] += count
+
+        t3 = time.perf_counter()
+        ctx.all_pairs_stream(fn2, a.band_pairs)
+        other = {"hipgraph": not bool(a.graph), "stream_s": time.perf_counter() - t3,
+                 "kernel_ms": ctx.stats().kernel_ms, "pairs_streamed": got2["pairs"]}
+        log(f"stream with hipgraph={int(not a.graph)}: {other['stream_s']:.2f} s (kernel {other['kernel_ms']:.0f} ms)")
     res = {"metric": METRIC, "value": npairs / (stream_s + prep_s), "unit": "sample-pairs/s", "n_gpus": 1,
            "steps": 1, "warmup": 0, "ms_per_step": (stream_s + prep_s) * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "u32 fixed-point (FP64 embedding)",
@@ -207,7 +336,8 @@ def run_batched(cms, a):
                       "variant": st.variant, "bands": st.bands, "band_pairs": a.band_pairs,
                       "hipgraph": bool(a.graph)},
            "prep_s": prep_s, "stream_s": stream_s, "pairs_streamed": got["pairs"],
-           "roofline": roofline(st, "sm_persistent_kernel")}
+           "kernel_ms": st.kernel_ms, "other_launch_mode": other,
+           "roofline": roofline(st, "sm_persistent_split_kernel")}
     ctx.close()
     print(json.dumps(res), flush=True)
 
@@ -423,6 +553,11 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=800)
     ap.add_argument("--cpu-pairs", type=int, default=319600, help="about 10 s of reference work on 16 threads")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--e2e", type=int, default=1, help="time cuda-comp end to end (files in and out)")
+    ap.add_argument("--exact-samples", type=int, default=400,
+                    help="random samples whose pairs estimate the exact shared-entry count")
+    ap.add_argument("--e2e-cpu-samples", type=int, default=400,
+                    help="samples of the text table on which cuda-comp and the reference comp run end to end")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (traffic-counter reference)")
     ap.add_argument("--bin", type=int, default=1, help="bin samples by sparsity (shared-minimum path)")
@@ -442,6 +577,8 @@ def main():
                     help="stop offering load beyond this many unfinished sets (workload 4)")
     ap.add_argument("--band-pairs", type=int, default=1 << 28, help="pairs per streamed band (workload 3)")
     ap.add_argument("--graph", type=int, default=1, help="replay streamed bands from a hipGraph (workload 3)")
+    ap.add_argument("--graph-compare", type=int, default=1,
+                    help="workload 3: stream again with the other launch mode (with / without hipGraph)")
     ap.add_argument("--error-pairs", type=int, default=1000000, help="random pairs checked vs FP64 (workload 2)")
     ap.add_argument("--metrics", type=str, default="",
                     help="workload 1: also time these comp -M metrics (e.g. 1,2,3,4,5) on the same samples")
@@ -479,8 +616,23 @@ def main():
     st = sm.stats()
     gpu_sm = sm.copy_result(npairs)
     log(f"sharedmin: {sec * 1e3:.2f} ms/step, pair kernel {st.kernel_ms:.2f} ms, prep {st.prep_ms:.2f} ms, "
-        f"visited pair-entries {st.pair_node_updates:.3e}")
+        f"executed pair-entries {st.pair_node_updates:.3e}")
     sm.close()
+    t_ex = time.perf_counter()
+    exact_per_pair, exact_n = exact_pair_entries(tree, abd, a.exact_samples)
+    exact = exact_per_pair * npairs
+    work = {"exact_pair_entries_est": exact,
+            "exact_per_pair": exact_per_pair,
+            "exact_sample": f"every pair of {exact_n} samples drawn at random (seed 9)",
+            "bitmap_visited_pair_entries": float(st.entries_visited) * 1024.0,
+            "executed_pair_entries": st.pair_node_updates,
+            "visited_vs_exact": float(st.entries_visited) * 1024.0 / exact,
+            "executed_vs_exact": st.pair_node_updates / exact,
+            "entries_visited": st.entries_visited, "entries_staged": st.entries_staged,
+            "entries_sparse": st.entries_sparse, "entries_half": st.entries_half,
+            "sparse_pairs": st.sparse_pairs, "half_pairs": st.half_pairs}
+    log(f"exact pair-entries {exact:.3e} ({exact_per_pair:.0f} per pair, {time.perf_counter() - t_ex:.1f} s); "
+        f"bitmap-visited / exact {work['visited_vs_exact']:.2f}, executed / exact {work['executed_vs_exact']:.2f}")
 
     others = {}
     for m in [int(x) for x in a.metrics.split(",") if x.strip()]:
@@ -540,7 +692,8 @@ def main():
                    "samples": n, "leaves": tree.leaf_n, "pairs": npairs, "kernel": "sharedmin",
                    "binned": bool(a.bin), "variant": st.variant},
         "pair_node_updates_per_s": npairs * tree.order_n / sec,
-        "roofline": roofline(st, "sm_persistent_kernel", traffic_from_profiles()),
+        "roofline": roofline(st, "sm_persistent_split_kernel", traffic_from_profiles()),
+        "work": work,
         "prep_ms": st.prep_ms,
         "plain_fp32_walk": walk,
     }
@@ -553,6 +706,9 @@ def main():
             ms = cc.calibrate_copy(a.copy_calib, 5)
         res["copy_calibration"] = {"bytes": a.copy_calib, "ms": ms, "GBps": 2 * a.copy_calib / ms / 1e6}
         log(f"copy calibration: {a.copy_calib} B in {ms:.3f} ms ({2 * a.copy_calib / ms / 1e6:.0f} GB/s read+write)")
+    if a.e2e:
+        res["end_to_end"] = end_to_end(cms, tree, cp, counts, a.cpu_threads, min(a.e2e_cpu_samples, n),
+                                       not a.no_cpu)
     if not a.no_cpu:
         cb, acc = cpu_baseline(cms, tree, cp, counts, abd, gpu_sm, gpu_walk, n, a.cpu_threads,
                                min(a.cpu_samples, n), a.cpu_pairs)

@@ -147,6 +147,10 @@ _sig("cms_table_f", C.c_int, [_vp])
 _sig("cms_table_sample_name", C.c_char_p, [_vp, C.c_int])
 _sig("cms_table_feature_name", C.c_char_p, [_vp, C.c_int])
 _sig("cms_table_skipped", C.c_int, [_vp])
+_sig("cms_table_save_csr", C.c_int, [_vp, C.c_char_p])
+_sig("cms_table_is_sparse", C.c_int, [_vp])
+_sig("cms_table_from_csr", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), _i64p,
+                                    _i32p, _f32p, C.POINTER(_vp)])
 _sig("cms_table_abundance", C.c_int, [_vp, _vp, C.c_int, _f32p, _vp])
 _sig("cms_table_abundance_csr", C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)])
 _sig("cms_write_dist", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), _f32p, C.c_int])
@@ -165,6 +169,7 @@ _sig("cms_synth_copy_numbers", C.c_int, [C.c_uint64, C.c_int, C.c_int, _f32p])
 _sig("cms_counts_to_abundance_csr", C.c_int, [C.c_int, _i64p, _i32p, _f32p, _vp, _f32p])
 _sig("cms_write_db", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, _i32p, _f32p, _i32p, _f32p, _i32p, _vp])
 _sig("cms_write_table", C.c_int, [C.c_char_p, C.c_int, C.c_int, _i64p, _i32p, _f32p])
+_sig("cms_write_table_csr", C.c_int, [C.c_char_p, C.c_int, C.c_int, _i64p, _i32p, _f32p])
 _sig("cms_free", None, [_vp])
 _sig("cms_version", C.c_char_p, [])
 _sig("cms_newick_parse", C.c_int, [C.c_char_p, C.c_int64, C.POINTER(_vp)])
@@ -193,7 +198,8 @@ EXPORTED = [
     "cms_tile", "cms_get_stats", "cms_prepare", "cms_validate_tree", "cms_db_open", "cms_db_close", "cms_db_leaf_n", "cms_db_order_n",
     "cms_db_has_tree", "cms_db_has_cp", "cms_db_order", "cms_db_leaf_name", "cms_table_load",
     "cms_table_from_dense", "cms_table_free", "cms_table_n", "cms_table_f", "cms_table_sample_name",
-    "cms_table_feature_name", "cms_table_skipped", "cms_table_abundance", "cms_table_abundance_csr",
+    "cms_table_feature_name", "cms_table_skipped", "cms_table_save_csr", "cms_table_is_sparse",
+    "cms_table_from_csr", "cms_table_abundance", "cms_table_abundance_csr", "cms_write_table_csr",
     "cms_write_dist", "cms_sample_file_abundance", "cms_sample_files_csr", "cms_list_load", "cms_list_n",
     "cms_list_path", "cms_list_name", "cms_list_free",
     "cms_synth_tree", "cms_synth_samples", "cms_synth_copy_numbers", "cms_counts_to_abundance_csr",
@@ -369,10 +375,22 @@ class Newick:
 class Table:
     """OTU count table, rows = samples (table_format.h:110-165)."""
 
-    def __init__(self, path: str | None = None, *, counts=None, samples=None, features=None):
+    def __init__(self, path: str | None = None, *, counts=None, samples=None, features=None, csr=None):
+        """From a file (text table or sparse table file), from dense counts
+        (n x f), or from csr=(rowptr, col, counts) over f = len(features)."""
         h = _vp()
         if path is not None:
             _check(lib.cms_table_load(path.encode(), C.byref(h)))
+        elif csr is not None:
+            rp, col, cnt = csr
+            rp = np.ascontiguousarray(rp, np.int64)
+            col = np.ascontiguousarray(col, np.int32)
+            cnt = np.ascontiguousarray(cnt, np.float32)
+            n, f = len(rp) - 1, len(features)
+            sn = (C.c_char_p * max(n, 1))(*[s.encode() for s in samples])
+            fn = (C.c_char_p * max(f, 1))(*[s.encode() for s in features])
+            _check(lib.cms_table_from_csr(n, f, sn, fn, rp, col if col.size else np.zeros(1, np.int32),
+                                          cnt if cnt.size else np.zeros(1, np.float32), C.byref(h)))
         else:
             counts = np.ascontiguousarray(counts, np.float32)
             n, f = counts.shape
@@ -400,6 +418,14 @@ class Table:
     @property
     def skipped(self) -> int:
         return lib.cms_table_skipped(self._h)
+
+    @property
+    def is_sparse(self) -> bool:
+        return bool(lib.cms_table_is_sparse(self._h))
+
+    def save_csr(self, path: str):
+        """Write the table as a sparse table file (cms_table_save_csr)."""
+        _check(lib.cms_table_save_csr(self._h, path.encode()))
 
     def sample_names(self) -> list:
         return [lib.cms_table_sample_name(self._h, i).decode() for i in range(self.n)]
@@ -563,6 +589,12 @@ def write_db(root: str, name: str, tree: Tree, cp: np.ndarray | None):
         cp = np.ascontiguousarray(cp, np.float32)
         cpp = cp.ctypes.data_as(_vp)
     _check(lib.cms_write_db(root.encode(), name.encode(), tree.leaf_n, tree.order_n, *tree.arrays(), cpp))
+
+
+def write_table_csr(path: str, counts: Samples, leaf_n: int):
+    """The table of write_table as a sparse table file (cms_write_table_csr)."""
+    _check(lib.cms_write_table_csr(path.encode(), counts.n, leaf_n, counts.rowptr, counts.col,
+                                   counts.val if counts.val.size else np.zeros(1, np.float32)))
 
 
 def write_table(path: str, counts: Samples, leaf_n: int):

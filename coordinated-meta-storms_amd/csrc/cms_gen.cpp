@@ -2,9 +2,12 @@
 // reference's file formats (db.config, tree/id.txt, tree/order.txt,
 // copy_number.txt, dense -T table).
 //
-//   cms-gen --root DIR --table PATH [--leaves L] [--samples N] [--seed S]
-//           [--bl-mean m] [--bl-max x] [--nnz LO HI] [--log-uniform]
+//   cms-gen --root DIR [--table PATH] [--csr PATH] [--leaves L] [--samples N]
+//           [--seed S] [--bl-mean m] [--bl-max x] [--nnz LO HI] [--log-uniform]
 //           [--zipf s] [--nb MEAN DISP] [--cp-max k] [--no-cp] [--name db]
+//
+// --csr writes the same table as a sparse table file (cms_table_save_csr),
+// which cuda-comp -T reads like the text table.
 //
 // Seeds follow one convention: tree = seed, samples = seed + 1000,
 // copy numbers = seed + 2000.
@@ -17,7 +20,7 @@
 #include "../../include/cms.h"
 
 int main(int argc, char **argv) {
-    std::string root, table, name = "synthetic";
+    std::string root, table, csr, name = "synthetic";
     int L = 5000, N = 64, nnz_lo = 100, nnz_hi = 800, log_uniform = 0, cp_max = 15, use_cp = 1;
     double bl_mean = 0.03, bl_max = 0.99, zipf = 1.1, nb_mean = 50, nb_disp = 0.5;
     unsigned long long seed = 1000;
@@ -29,6 +32,7 @@ int main(int argc, char **argv) {
         };
         if (a == "--root") root = v(1), i++;
         else if (a == "--table") table = v(1), i++;
+        else if (a == "--csr") csr = v(1), i++;
         else if (a == "--name") name = v(1), i++;
         else if (a == "--leaves") L = atoi(v(1)), i++;
         else if (a == "--samples") N = atoi(v(1)), i++;
@@ -58,7 +62,7 @@ int main(int argc, char **argv) {
         fprintf(stderr, "%s\n", cms_last_error(nullptr));
         return 1;
     }
-    if (!table.empty()) {
+    if (!table.empty() || !csr.empty()) {
         cms_synth_samples_opts so;
         memset(&so, 0, sizeof so);
         so.seed = seed + 1000;
@@ -74,7 +78,14 @@ int main(int argc, char **argv) {
         int32_t *col;
         float *cnt;
         if (cms_synth_samples(&so, &rowptr, &col, &cnt)) { fprintf(stderr, "%s\n", cms_last_error(nullptr)); return 1; }
-        if (cms_write_table(table.c_str(), N, L, rowptr, col, cnt)) { fprintf(stderr, "%s\n", cms_last_error(nullptr)); return 1; }
+        if (!table.empty() && cms_write_table(table.c_str(), N, L, rowptr, col, cnt)) {
+            fprintf(stderr, "%s\n", cms_last_error(nullptr));
+            return 1;
+        }
+        if (!csr.empty() && cms_write_table_csr(csr.c_str(), N, L, rowptr, col, cnt)) {
+            fprintf(stderr, "%s\n", cms_last_error(nullptr));
+            return 1;
+        }
         cms_free(rowptr);
         cms_free(col);
         cms_free(cnt);

@@ -110,7 +110,11 @@ int compile_tree(int leaf_n, int order_n, const int32_t *o1, const float *d1,
                  int leaf_chunk, CompiledTree &out, std::string &err);
 
 // Float formatting exactly like std::ostream's default for a double
-// (precision 6, %g conversion). Appends to `s`.
+// (precision 6, %g conversion). Appends to `s`. Values in [1e-15, 1e5) take
+// an integer path (exact six-digit rounding of the binary value), the rest
+// snprintf.
 void append_g6(std::string &s, double v);
+// 1 when the integer path and snprintf("%.6g") disagree on v (tests).
+int g6_fast_differs(double v);
 
 } // namespace cms

@@ -593,7 +593,8 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     if (opts.bands <= 0) opts.bands = 16;
     if (opts.slots % kSmBlock) return sfail(CMS_E_INVAL, "cms_server_create: slots must be a multiple of 32");
     if (opts.memory_budget < 0) return sfail(CMS_E_INVAL, "cms_server_create: negative memory budget");
-    if (opts.variant < 0 || opts.variant > kSmVariants) return sfail(CMS_E_INVAL, "cms_server_create: bad variant");
+    if (opts.variant < 0 || opts.variant > kSmCrossVariants)
+        return sfail(CMS_E_INVAL, "cms_server_create: bad variant (the cross kernel has variants 1..6)");
     int rc = ctx_prepare(ref);
     if (rc) return rc;
     if (ref->n <= 0) return ctx_fail(ref, CMS_E_STATE, "cms_server_create: no reference samples");
@@ -628,7 +629,10 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
             return sfail(CMS_E_RANGE, "unweighted edge coefficient outside the references' fixed-point range");
         }
     }
-    s->variant = opts.variant > 0 ? opts.variant - 1 : (ref->variant >= 0 ? ref->variant : 1);
+    // the cross kernel has the plain tiles' variants only: a context tuned to
+    // a split variant serves with the default plain one
+    s->variant = opts.variant > 0 ? opts.variant - 1
+                                  : (ref->variant >= 0 && ref->variant < kSmCrossVariants ? ref->variant : 1);
     CREATE_HIP(hipSetDevice(s->device));
     int cus = 0;
     CREATE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));

@@ -14,6 +14,7 @@
 #include <map>
 #include <string>
 #include <unordered_map>
+#include <thread>
 #include <vector>
 
 // the reference's hash container (hash.h), for the iteration order of the
@@ -42,8 +43,24 @@ struct cms_db {
 struct cms_table {
     std::vector<std::string> features;
     std::vector<std::string> samples;
-    std::vector<float> abd;  // n x f row-major
+    std::vector<float> abd;  // n x f row-major (text tables)
     int skipped = 0;
+    // sparse tables: row s = (column, count) pairs, columns ascending
+    bool sparse = false;
+    std::vector<int64_t> rowptr;
+    std::vector<int32_t> cols;
+    std::vector<float> cnts;
+    // visit row s's columns with a count in ascending column order: fn(j, v)
+    template <class Fn>
+    void for_row(size_t s, Fn fn) const {
+        if (sparse) {
+            for (int64_t k = rowptr[s]; k < rowptr[s + 1]; k++) fn((size_t)cols[(size_t)k], cnts[(size_t)k]);
+        } else {
+            const size_t F = features.size();
+            const float *row = abd.data() + s * F;
+            for (size_t j = 0; j < F; j++) fn(j, row[j]);
+        }
+    }
 };
 
 namespace {
@@ -295,7 +312,63 @@ const char *cms_db_leaf_name(const cms_db *db, int i) {
     return db->ids[i].c_str();
 }
 
-// _Table_Format::Load_Table (table_format.h:110-165).
+namespace {
+
+// Sparse table file (cms_table_save_csr): little-endian
+//   "CMSTCSR1", u64 samples, u64 features, u64 entries,
+//   features x (u32 length, name bytes), samples x (u32 length, name bytes),
+//   (samples + 1) x i64 row offsets, entries x i32 columns (ascending per
+//   row), entries x f32 counts.
+const char kCsrMagic[8] = {'C', 'M', 'S', 'T', 'C', 'S', 'R', '1'};
+
+int parse_csr_table(const std::string &buf, const char *path, cms_table *t) {
+    size_t at = 8;
+    auto fail = [&](const char *why) {
+        set_global_error(std::string("sparse table ") + path + ": " + why);
+        return CMS_E_IO;
+    };
+    auto get = [&](void *dst, size_t bytes) {
+        if (bytes > buf.size() - at) return false;
+        memcpy(dst, buf.data() + at, bytes);
+        at += bytes;
+        return true;
+    };
+    uint64_t n = 0, f = 0, nnz = 0;
+    if (!get(&n, 8) || !get(&f, 8) || !get(&nnz, 8)) return fail("truncated header");
+    if (n > (uint64_t)INT32_MAX || f > (uint64_t)INT32_MAX || nnz > n * f) return fail("sizes out of range");
+    auto names = [&](uint64_t count, std::vector<std::string> &v) {
+        v.resize(count);
+        for (uint64_t i = 0; i < count; i++) {
+            uint32_t len = 0;
+            if (!get(&len, 4) || len > buf.size() - at) return false;
+            v[i].assign(buf.data() + at, len);
+            at += len;
+        }
+        return true;
+    };
+    if (!names(f, t->features) || !names(n, t->samples)) return fail("truncated names");
+    t->rowptr.resize(n + 1);
+    t->cols.resize(nnz);
+    t->cnts.resize(nnz);
+    if (!get(t->rowptr.data(), 8 * (n + 1)) || !get(t->cols.data(), 4 * nnz) || !get(t->cnts.data(), 4 * nnz))
+        return fail("truncated arrays");
+    if (at != buf.size()) return fail("trailing bytes");
+    if (t->rowptr[0] != 0 || t->rowptr[n] != (int64_t)nnz) return fail("row offsets do not cover the entries");
+    for (uint64_t i = 0; i < n; i++) {
+        if (t->rowptr[i + 1] < t->rowptr[i]) return fail("row offsets decrease");
+        for (int64_t k = t->rowptr[i]; k < t->rowptr[i + 1]; k++) {
+            if (t->cols[k] < 0 || (uint64_t)t->cols[k] >= f) return fail("column out of range");
+            if (k > t->rowptr[i] && t->cols[k] <= t->cols[k - 1]) return fail("columns not ascending in a row");
+        }
+    }
+    t->sparse = true;
+    return CMS_OK;
+}
+
+}  // namespace
+
+// _Table_Format::Load_Table (table_format.h:110-165); a file that starts
+// with the sparse-table magic is read as the same table in CSR form.
 int cms_table_load(const char *path, cms_table **out) {
     if (!path || !out) { set_global_error("cms_table_load: null argument"); return CMS_E_INVAL; }
     *out = nullptr;
@@ -303,6 +376,16 @@ int cms_table_load(const char *path, cms_table **out) {
     if (!read_file(path, buf)) {
         set_global_error(std::string("Cannot open file : ") + path);
         return CMS_E_IO;
+    }
+    if (buf.size() >= 8 && memcmp(buf.data(), kCsrMagic, 8) == 0) {
+        cms_table *t = new cms_table();
+        const int rc = parse_csr_table(buf, path, t);
+        if (rc) {
+            delete t;
+            return rc;
+        }
+        *out = t;
+        return CMS_OK;
     }
     std::vector<std::pair<size_t, size_t>> lines;
     split_lines(buf, lines);
@@ -362,6 +445,75 @@ int cms_table_from_dense(int n, int f, const char *const *sample_names,
     return CMS_OK;
 }
 
+int cms_table_from_csr(int n, int f, const char *const *sample_names, const char *const *feature_names,
+                       const int64_t *rowptr, const int32_t *col, const float *counts, cms_table **out) {
+    if (!out || n < 0 || f < 0 || (n && (!sample_names || !rowptr)) || (f && !feature_names)) {
+        set_global_error("cms_table_from_csr: bad argument");
+        return CMS_E_INVAL;
+    }
+    *out = nullptr;
+    const int64_t nnz = n ? rowptr[n] : 0;
+    if ((n && rowptr[0] != 0) || nnz < 0 || (nnz && (!col || !counts))) {
+        set_global_error("cms_table_from_csr: bad row offsets");
+        return CMS_E_INVAL;
+    }
+    for (int i = 0; i < n; i++) {
+        if (rowptr[i + 1] < rowptr[i]) { set_global_error("cms_table_from_csr: row offsets decrease"); return CMS_E_INVAL; }
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++)
+            if (col[k] < 0 || col[k] >= f || (k > rowptr[i] && col[k] <= col[k - 1])) {
+                set_global_error("cms_table_from_csr: columns out of range or not ascending");
+                return CMS_E_INVAL;
+            }
+    }
+    cms_table *t = new cms_table();
+    for (int j = 0; j < f; j++) t->features.emplace_back(feature_names[j]);
+    for (int i = 0; i < n; i++) t->samples.emplace_back(sample_names[i]);
+    t->sparse = true;
+    t->rowptr.assign(rowptr, rowptr + (n ? n + 1 : 1));
+    if (!n) t->rowptr[0] = 0;
+    t->cols.assign(col, col + nnz);
+    t->cnts.assign(counts, counts + nnz);
+    *out = t;
+    return CMS_OK;
+}
+
+int cms_table_save_csr(const cms_table *t, const char *path) {
+    if (!t || !path) { set_global_error("cms_table_save_csr: null argument"); return CMS_E_INVAL; }
+    const size_t n = t->samples.size(), f = t->features.size();
+    std::vector<int64_t> rp(n + 1, 0);
+    std::vector<int32_t> cj;
+    std::vector<float> cv;
+    for (size_t s = 0; s < n; s++) {
+        t->for_row(s, [&](size_t j, float v) {
+            if (v != 0.0f) {
+                cj.push_back((int32_t)j);
+                cv.push_back(v);
+            }
+        });
+        rp[s + 1] = (int64_t)cj.size();
+    }
+    FILE *fp = fopen(path, "wb");
+    if (!fp) { set_global_error(std::string("cms_table_save_csr: cannot write ") + path); return CMS_E_IO; }
+    bool ok = fwrite(kCsrMagic, 1, 8, fp) == 8;
+    const uint64_t hdr[3] = {(uint64_t)n, (uint64_t)f, (uint64_t)cj.size()};
+    ok = ok && fwrite(hdr, 8, 3, fp) == 3;
+    auto put_names = [&](const std::vector<std::string> &v) {
+        for (const std::string &x : v) {
+            const uint32_t len = (uint32_t)x.size();
+            ok = ok && fwrite(&len, 4, 1, fp) == 1 && fwrite(x.data(), 1, len, fp) == len;
+        }
+    };
+    put_names(t->features);
+    put_names(t->samples);
+    ok = ok && fwrite(rp.data(), 8, n + 1, fp) == n + 1;
+    ok = ok && fwrite(cj.data(), 4, cj.size(), fp) == cj.size() && fwrite(cv.data(), 4, cv.size(), fp) == cv.size();
+    if (fclose(fp) != 0) ok = false;
+    if (!ok) { set_global_error(std::string("cms_table_save_csr: write failed: ") + path); return CMS_E_IO; }
+    return CMS_OK;
+}
+
+int cms_table_is_sparse(const cms_table *t) { return t ? (int)t->sparse : CMS_E_INVAL; }
+
 void cms_table_free(cms_table *t) { delete t; }
 int cms_table_n(const cms_table *t) { return t ? (int)t->samples.size() : CMS_E_INVAL; }
 int cms_table_f(const cms_table *t) { return t ? (int)t->features.size() : CMS_E_INVAL; }
@@ -397,19 +549,18 @@ int cms_table_abundance(const cms_db *db, const cms_table *t, int cp_correct, fl
     for (long s = 0; s < (long)N; s++) {
         float *out = abd + (size_t)s * L;
         memset(out, 0, L * sizeof(float));
-        const float *row = t->abd.data() + (size_t)s * F;
         std::vector<int32_t> touched;
         std::vector<char> seen;  // lazily sized
         float total = 0;
-        for (size_t j = 0; j < F; j++) {
-            if (row[j] > 0 && leaf_of[j] >= 0) {
-                float x = row[j] / cp_of[j];
+        t->for_row((size_t)s, [&](size_t j, float v) {
+            if (v > 0 && leaf_of[j] >= 0) {
+                float x = v / cp_of[j];
                 int32_t l = leaf_of[j];
                 out[l] = x;  // later duplicate overwrites
                 touched.push_back(l);
                 total += x;
             }
-        }
+        });
         total /= 100.0;
         int m = 0;
         if (seen.size() < L) seen.assign(L, 0);
@@ -448,16 +599,15 @@ int cms_table_abundance_csr(const cms_db *db, const cms_table *t, int cp_correct
     std::vector<std::vector<std::pair<int32_t, float>>> rows(N);
 #pragma omp parallel for schedule(dynamic, 8)
     for (long s = 0; s < (long)N; s++) {
-        const float *row = t->abd.data() + (size_t)s * F;
         std::vector<std::pair<int32_t, float>> &r = rows[(size_t)s];
         float total = 0;
-        for (size_t j = 0; j < F; j++) {
-            if (row[j] > 0 && leaf_of[j] >= 0) {
-                const float x = row[j] / cp_of[j];
+        t->for_row((size_t)s, [&](size_t j, float v) {
+            if (v > 0 && leaf_of[j] >= 0) {
+                const float x = v / cp_of[j];
                 r.push_back({leaf_of[j], x});
                 total += x;  // a duplicated leaf counts twice, as in Load_abd
             }
-        }
+        });
         total /= 100.0;
         // last column wins for a duplicated leaf: stable sort by leaf, keep the last
         std::stable_sort(r.begin(), r.end(), [](const std::pair<int32_t, float> &a, const std::pair<int32_t, float> &b) {
@@ -515,13 +665,19 @@ int cms_write_dist(const char *path, int n, const char *const *names, const floa
     for (int i = 0; i < n; i++) { hdr += '\t'; hdr += names[i]; }
     hdr += '\n';
     fwrite(hdr.data(), 1, hdr.size(), f);
+    // rows are formatted in parallel batches; a writer thread writes each
+    // finished batch while the next one is formatted
     const int batch = 256;
-    std::vector<std::string> rows(batch);
+    std::vector<std::string> rows[2] = {std::vector<std::string>(batch), std::vector<std::string>(batch)};
+    std::thread writer;
+    bool write_ok = true;
+    int cur = 0;
     for (int r0 = 0; r0 < n; r0 += batch) {
         int r1 = r0 + batch < n ? r0 + batch : n;
+        std::vector<std::string> &rb = rows[cur];
 #pragma omp parallel for schedule(dynamic, 1)
         for (int i = r0; i < r1; i++) {
-            std::string &s = rows[i - r0];
+            std::string &s = rb[i - r0];
             s.clear();
             s.reserve((size_t)n * 10 + 32);
             s += names[i];
@@ -538,9 +694,15 @@ int cms_write_dist(const char *path, int n, const char *const *names, const floa
             }
             s += '\n';
         }
-        for (int i = r0; i < r1; i++) fwrite(rows[i - r0].data(), 1, rows[i - r0].size(), f);
+        if (writer.joinable()) writer.join();
+        writer = std::thread([&rb, r1, r0, f, &write_ok]() {
+            for (int i = 0; i < r1 - r0; i++)
+                if (fwrite(rb[i].data(), 1, rb[i].size(), f) != rb[i].size()) write_ok = false;
+        });
+        cur ^= 1;
     }
-    if (fclose(f) != 0) {
+    if (writer.joinable()) writer.join();
+    if (fclose(f) != 0 || !write_ok) {
         set_global_error(std::string("write failed : ") + path);
         return CMS_E_IO;
     }

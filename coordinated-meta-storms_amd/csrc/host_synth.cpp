@@ -386,4 +386,35 @@ int cms_write_table(const char *path, int n, int leaf_n, const int64_t *rowptr, 
     return CMS_OK;
 }
 
+int cms_write_table_csr(const char *path, int n, int leaf_n, const int64_t *rowptr, const int32_t *col,
+                        const float *cnt) {
+    if (!path || n < 0 || leaf_n < 0 || (n && (!rowptr || !col || !cnt))) {
+        set_global_error("cms_write_table_csr: bad argument");
+        return CMS_E_INVAL;
+    }
+    std::vector<std::string> fn((size_t)leaf_n), sn((size_t)n);
+    std::vector<const char *> fp((size_t)leaf_n), sp((size_t)n);
+    for (int j = 0; j < leaf_n; j++) {
+        fn[j] = "T" + std::to_string(j);
+        fp[j] = fn[j].c_str();
+    }
+    for (int i = 0; i < n; i++) {
+        sn[i] = "S" + std::to_string(i);
+        sp[i] = sn[i].c_str();
+    }
+    // counts as the text table prints them (%g), so both forms hold one table
+    std::vector<float> c((size_t)std::max<int64_t>(n ? rowptr[n] : 0, 1));
+    for (int64_t k = 0; n && k < rowptr[n]; k++) {
+        char buf[64];
+        snprintf(buf, sizeof buf, "%g", (double)cnt[k]);
+        c[(size_t)k] = strtof(buf, nullptr);
+    }
+    cms_table *t = nullptr;
+    int rc = cms_table_from_csr(n, leaf_n, sp.data(), fp.data(), rowptr, col, c.data(), &t);
+    if (rc) return rc;
+    rc = cms_table_save_csr(t, path);
+    cms_table_free(t);
+    return rc;
+}
+
 }  // extern "C"

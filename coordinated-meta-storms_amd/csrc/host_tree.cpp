@@ -7,6 +7,8 @@
 // then writes register order_d + REG_SIZE. Registers are renamed here by value
 // liveness; the dataflow (which write each read sees) is unchanged.
 #include <algorithm>
+#include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 
@@ -18,10 +20,142 @@ static thread_local std::string g_err;
 void set_global_error(const std::string &msg) { g_err = msg; }
 const char *global_error() { return g_err.c_str(); }
 
+namespace {
+
+// floor(v * 10^s) and the rounding remainder of v = m * 2^k (m < 2^53,
+// -127 <= k < 0, 0 <= s <= 20) in 128-bit integers: m * 10^s < 2^127.
+struct Scaled {
+    unsigned __int128 q, rem, half;  // q = floor, rem = exact remainder, half = 2^(-k-1)
+};
+
+Scaled scale(uint64_t m, int k, int s) {
+    static const uint64_t p10[21] = {1ull,
+                                     10ull,
+                                     100ull,
+                                     1000ull,
+                                     10000ull,
+                                     100000ull,
+                                     1000000ull,
+                                     10000000ull,
+                                     100000000ull,
+                                     1000000000ull,
+                                     10000000000ull,
+                                     100000000000ull,
+                                     1000000000000ull,
+                                     10000000000000ull,
+                                     100000000000000ull,
+                                     1000000000000000ull,
+                                     10000000000000000ull,
+                                     100000000000000000ull,
+                                     1000000000000000000ull,
+                                     10000000000000000000ull,
+                                     0ull};
+    unsigned __int128 P = (unsigned __int128)m * (s <= 19 ? p10[s] : p10[19]);
+    if (s == 20) P *= 10u;
+    const int sh = -k;
+    Scaled r;
+    r.q = P >> sh;
+    r.rem = P - (r.q << sh);
+    r.half = (unsigned __int128)1 << (sh - 1);
+    return r;
+}
+
+// printf("%.6g", v) for v in [1e-15, 1e5): the six significant digits of
+// the exact binary value rounded half to even (what glibc's printf does),
+// then %g's choice of fixed or exponent notation with trailing zeros
+// removed. Returns false outside that range (the caller uses snprintf).
+bool fast_g6(double v, char *out, int &len) {
+    if (!(v >= 1e-15 && v < 1e5)) return false;
+    uint64_t bits;
+    memcpy(&bits, &v, 8);
+    const int be = (int)((bits >> 52) & 0x7ff);
+    if (be == 0) return false;  // subnormal: below the range anyway
+    const uint64_t m = (bits & ((1ull << 52) - 1)) | (1ull << 52);
+    const int k = be - 1075;  // v = m * 2^k
+    if (k >= 0 || k < -127) return false;
+    // decimal exponent from the binary one (floor(e2 * log10 2), off by at
+    // most one, corrected below)
+    const int e2 = be - 1023;
+    int e10 = (e2 * 78913) >> 18;
+    Scaled sc{};
+    for (int tries = 0; tries < 3; tries++) {
+        const int sp = 5 - e10;
+        if (sp < 0 || sp > 20) return false;
+        sc = scale(m, k, sp);
+        if (sc.q < 100000u) {
+            e10--;
+        } else if (sc.q >= 1000000u) {
+            e10++;
+        } else {
+            break;
+        }
+    }
+    if (sc.q < 100000u || sc.q >= 1000000u) return false;
+    uint64_t N = (uint64_t)sc.q;
+    if (sc.rem > sc.half || (sc.rem == sc.half && (N & 1u))) N++;
+    int X = e10;
+    if (N == 1000000u) {
+        N = 100000u;
+        X++;
+    }
+    char d[6];
+    {
+        const uint32_t hi = (uint32_t)(N / 1000u), lo = (uint32_t)(N % 1000u);
+        d[0] = (char)('0' + hi / 100u);
+        d[1] = (char)('0' + (hi / 10u) % 10u);
+        d[2] = (char)('0' + hi % 10u);
+        d[3] = (char)('0' + lo / 100u);
+        d[4] = (char)('0' + (lo / 10u) % 10u);
+        d[5] = (char)('0' + lo % 10u);
+    }
+    int nd = 6;
+    while (nd > 1 && d[nd - 1] == '0') nd--;
+    int n = 0;
+    if (X < -4 || X >= 6) {
+        out[n++] = d[0];
+        if (nd > 1) {
+            out[n++] = '.';
+            for (int i = 1; i < nd; i++) out[n++] = d[i];
+        }
+        out[n++] = 'e';
+        out[n++] = X < 0 ? '-' : '+';
+        const int ax = X < 0 ? -X : X;
+        if (ax >= 100) out[n++] = (char)('0' + ax / 100);
+        out[n++] = (char)('0' + (ax / 10) % 10);
+        out[n++] = (char)('0' + ax % 10);
+    } else if (X >= 0) {
+        for (int i = 0; i <= X; i++) out[n++] = i < nd ? d[i] : '0';
+        if (nd > X + 1) {
+            out[n++] = '.';
+            for (int i = X + 1; i < nd; i++) out[n++] = d[i];
+        }
+    } else {
+        out[n++] = '0';
+        out[n++] = '.';
+        for (int i = 0; i < -X - 1; i++) out[n++] = '0';
+        for (int i = 0; i < nd; i++) out[n++] = d[i];
+    }
+    len = n;
+    return true;
+}
+
+}  // namespace
+
 void append_g6(std::string &s, double v) {
     char buf[64];
-    int n = snprintf(buf, sizeof buf, "%.6g", v);
+    int n = 0;
+    if (!fast_g6(v, buf, n)) n = snprintf(buf, sizeof buf, "%.6g", v);
     s.append(buf, (size_t)n);
+}
+
+// The formatter's own check: the fast path's text and snprintf's, for the
+// tests (returns 1 when they differ).
+int g6_fast_differs(double v) {
+    char a[64], b[64];
+    int na = 0;
+    if (!fast_g6(v, a, na)) return 0;
+    const int nb = snprintf(b, sizeof b, "%.6g", v);
+    return na != nb || memcmp(a, b, (size_t)na) != 0;
 }
 
 // 1 - dist with the MIN_DIST clamp, FP32 exactly as comp.h:341-344.

@@ -136,7 +136,8 @@ constexpr SmVariant kSmVariantTable[kSmVariants] = {{8, 4, 0},  {16, 4, 0}, {32,
 // light bitmaps; crs >= the most light entries of any block.
 hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t *lbm, int64_t nblk, int64_t nwords,
                                 uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st);
-constexpr int kSmDefaultVariant = 1;  // without autotune: 16-entry chunk, 4 waves per workgroup, all staged
+constexpr int kSmCrossVariants = 6;   // the serving cross kernel: the plain tiles' variants 1..6
+constexpr int kSmDefaultVariant = 7;  // without autotune: split tiles, 8-entry chunk, 1 wave per workgroup
 hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
                                 unsigned long long *work, float *out, int64_t out_cap, int out_mode,
                                 int64_t out_base, int grid_waves, hipStream_t st);

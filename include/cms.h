@@ -231,7 +231,7 @@ typedef struct cms_server_opts {
     int slots;             /* query slots resident at once, multiple of 32 (default 1024) */
     int bands;             /* reference bands per cycle (default 16) */
     int64_t memory_budget; /* bytes of the per-request arena (0 = 8 GiB or less if the device has less free) */
-    int variant;           /* cross-kernel variant 1..6 as cms_opts.variant (0 = default) */
+    int variant;           /* cross-kernel variant 1..6 as cms_opts.variant 1..6 (0 = the context's if plain, else 2) */
     int reserved[3];
 } cms_server_opts;
 
@@ -326,6 +326,20 @@ int cms_table_f(const cms_table *t);
 const char *cms_table_sample_name(const cms_table *t, int i);
 const char *cms_table_feature_name(const cms_table *t, int j);
 int cms_table_skipped(const cms_table *t); /* rows rejected by the width check */
+/* Sparse table file: the same table (feature and sample names, counts) in
+ * CSR form, little-endian binary ("CMSTCSR1", u64 samples, features,
+ * entries; u32-length-prefixed names; i64 row offsets, i32 ascending column
+ * indices, f32 counts). No counterpart in the reference, whose only table is
+ * the dense text of table_format.h:110-165: a 10,000 x 99,322 text table is
+ * about 2 GB, its sparse file about 90 MB. cms_table_load reads either form
+ * (the magic selects it) and every cms_table_* function treats them alike. */
+int cms_table_save_csr(const cms_table *t, const char *path);
+int cms_table_is_sparse(const cms_table *t);
+/* Build a sparse table from memory: n samples, f features, row s =
+ * (col[k], counts[k]) for k in [rowptr[s], rowptr[s+1]), columns strictly
+ * ascending. */
+int cms_table_from_csr(int n, int f, const char *const *sample_names, const char *const *feature_names,
+                       const int64_t *rowptr, const int32_t *col, const float *counts, cms_table **out);
 
 /* Dense percent-scale abundances n x leaf_n, _Comp_Tree::Load_abd
  * (comp.h:243-279) with Check_OTU (utility.h:181-190) and copy-number
@@ -444,6 +458,9 @@ int cms_write_db(const char *root, const char *name, int leaf_n, int order_n,
 /* Write a dense OTU count table (text) from CSR counts over leaf_n columns. */
 int cms_write_table(const char *path, int n, int leaf_n, const int64_t *rowptr,
                     const int32_t *col, const float *cnt);
+/* The same table as a sparse table file (see cms_table_save_csr). */
+int cms_write_table_csr(const char *path, int n, int leaf_n, const int64_t *rowptr,
+                        const int32_t *col, const float *cnt);
 
 void cms_free(void *p);
 const char *cms_version(void);

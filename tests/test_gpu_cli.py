@@ -80,3 +80,21 @@ def test_cuda_comp_binary_output(cms, gpu_ok, golden, tmp_path, case, kernel):
         np.testing.assert_array_equal(got[p].view(np.uint32), exp.view(np.uint32))
     else:
         assert np.abs(got[p].astype(np.float64) - exp).max() <= 1e-5
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("tag,extra", [("dist_rT", []), ("sim_rT", ["-d", "F"])])
+def test_cuda_comp_sparse_table_input(cms, gpu_ok, golden, tmp_path, case, tag, extra):
+    """The reference's table saved as a sparse table file: -T reads it and
+    cuda-comp writes the reference's .dist bytes."""
+    print(f"cuda-comp sparse input {case} {tag}", flush=True)
+    g = golden(case)
+    sparse = str(tmp_path / "table.cmst")
+    cms.Table(g.table).save_csr(sparse)
+    exe = os.path.join(cms.BIN_DIR, "cuda-comp")
+    out = str(tmp_path / "o.dist")
+    env = dict(os.environ, ParallelMETA=g.root)
+    r = subprocess.run([exe, "-T", sparse, "-D", "G", "-o", out, "-M", "0"] + extra, env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert open(out, "rb").read() == g.read(tag + ".dist")

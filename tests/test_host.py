@@ -229,3 +229,89 @@ def test_abundance_csr_equals_dense(cms, golden, case):
         for s in range(csr.n):
             c = csr.col[csr.rowptr[s]:csr.rowptr[s + 1]]
             assert np.all(np.diff(c) > 0)
+
+
+@pytest.mark.parametrize("case", ["toy", "quirks", "config0", "core"])
+def test_sparse_table_file_same_abundances(cms, golden, tmp_path, case):
+    # the reference's text tables saved as sparse table files load to the
+    # same names and bit-identical abundances (dense and CSR, copy numbers on
+    # and off): duplicated, prefixed and unmapped columns and skipped rows of
+    # the quirks table included (skipped rows are not saved)
+    g = golden(case)
+    db = cms.Database(g.root, "G")
+    t = cms.Table(g.table)
+    p = str(tmp_path / "t.cmst")
+    t.save_csr(p)
+    s = cms.Table(p)
+    assert s.is_sparse and not t.is_sparse
+    assert s.sample_names() == t.sample_names() and s.feature_names() == t.feature_names()
+    assert s.skipped == 0
+    for cp in (True, False):
+        d0, m0 = t.abundance(db, cp)
+        d1, m1 = s.abundance(db, cp)
+        np.testing.assert_array_equal(d1.view(np.uint32), d0.view(np.uint32))
+        np.testing.assert_array_equal(m1, m0)
+        c0, c1 = t.abundance_csr(db, cp), s.abundance_csr(db, cp)
+        np.testing.assert_array_equal(c1.rowptr, c0.rowptr)
+        np.testing.assert_array_equal(c1.col, c0.col)
+        np.testing.assert_array_equal(c1.val.view(np.uint32), c0.val.view(np.uint32))
+    # saving the sparse table again gives the same bytes
+    p2 = str(tmp_path / "t2.cmst")
+    s.save_csr(p2)
+    assert open(p, "rb").read() == open(p2, "rb").read()
+
+
+def test_sparse_table_writer_matches_text_writer(cms, tmp_path):
+    counts = cms.synth_counts(77, 40, 300, 5, 60, 1.1, 30.0, 0.5)
+    tp, cp_ = str(tmp_path / "t.txt"), str(tmp_path / "t.cmst")
+    cms.write_table(tp, counts, 300)
+    cms.write_table_csr(cp_, counts, 300)
+    a, b = cms.Table(tp), cms.Table(cp_)
+    assert a.sample_names() == b.sample_names() and a.feature_names() == b.feature_names()
+    tree = cms.synth_tree(78, 300, 0.05)
+    root = str(tmp_path / "db")
+    cms.write_db(root, "x", tree, cms.synth_copy_numbers(79, 300))
+    db = cms.Database(root, "G")
+    np.testing.assert_array_equal(a.abundance(db)[0].view(np.uint32), b.abundance(db)[0].view(np.uint32))
+    assert os.path.getsize(cp_) < os.path.getsize(tp)
+
+
+def test_sparse_table_rejects_bad_files(cms, tmp_path):
+    counts = cms.synth_counts(80, 6, 50, 2, 10, 1.1, 30.0, 0.5)
+    good = str(tmp_path / "g.cmst")
+    cms.write_table_csr(good, counts, 50)
+    raw = open(good, "rb").read()
+    nnz = int(counts.rowptr[-1])
+    cases = {
+        "truncated": raw[:-3],
+        "trailing": raw + b"\0",
+        # the first column index (after the row offsets) set out of range
+        "column": raw[: len(raw) - 8 * nnz] + (1000).to_bytes(4, "little") + raw[len(raw) - 8 * nnz + 4:],
+        # the first row's columns made descending when it has two or more
+        "order": (raw[: len(raw) - 8 * nnz] + raw[len(raw) - 8 * nnz + 4: len(raw) - 8 * nnz + 8]
+                  + raw[len(raw) - 8 * nnz: len(raw) - 8 * nnz + 4] + raw[len(raw) - 8 * nnz + 8:]),
+    }
+    assert counts.rowptr[1] >= 2
+    for name, data in cases.items():
+        p = str(tmp_path / f"{name}.cmst")
+        open(p, "wb").write(data)
+        with pytest.raises(cms.CmsError) as e:
+            cms.Table(p)
+        assert e.value.code == cms.E_IO, name
+    # built from memory: bad rows are refused
+    with pytest.raises(cms.CmsError):
+        cms.Table(csr=(np.array([0, 2]), np.array([3, 1]), np.array([1.0, 2.0])), samples=["a"],
+                  features=[f"f{j}" for j in range(5)])
+
+
+def test_cms_gen_writes_sparse_table(cms, tmp_path):
+    import subprocess
+    exe = os.path.join(cms.BIN_DIR, "cms-gen")
+    t, c = str(tmp_path / "t.txt"), str(tmp_path / "t.cmst")
+    r = subprocess.run([exe, "--root", str(tmp_path), "--table", t, "--csr", c, "--leaves", "200", "--samples", "9"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    a, b = cms.Table(t), cms.Table(c)
+    assert b.is_sparse and a.n == b.n == 9 and a.f == b.f == 200
+    db = cms.Database(str(tmp_path), "G")
+    np.testing.assert_array_equal(a.abundance(db)[0].view(np.uint32), b.abundance(db)[0].view(np.uint32))
