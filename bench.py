@@ -561,7 +561,9 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (traffic-counter reference)")
     ap.add_argument("--bin", type=int, default=1, help="bin samples by sparsity (shared-minimum path)")
-    ap.add_argument("--autotune", type=int, default=1, help="autotune the persistent-kernel variant")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="autotune the persistent-kernel variant (default: the library's default variant, "
+                         "the one the committed profiles measure)")
     ap.add_argument("--variant", type=int, default=0, help="force a persistent-kernel variant (1..12; 7..12 take nearly empty entries out of the staged path)")
     ap.add_argument("--workload", type=int, default=1, choices=[1, 2, 3, 4],
                     help="1: headline all-pairs; 2: precision (16-bit walk storage, 50,000 samples); "

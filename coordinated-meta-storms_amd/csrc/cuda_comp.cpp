@@ -5,7 +5,7 @@
 //
 //   cuda-comp -T table -D G -o out.dist [-r T|F] [-d T|F] [-M 0..5] [-t n]
 //             [--kernel walk|sharedmin] [--storage f32|bf16|f16] [--device k]
-//             [--format dist|binary]
+//             [--format dist|binary] [--stream-bands P]
 //   cuda-comp -l list [-p prefix] -D G -o out.dist [...]   one sample per file
 //   cuda-comp -i file1 file2 -D G [-r T|F] [-d T|F] [-M 0..5]   one pair, printed
 //
@@ -16,8 +16,9 @@
 // --format binary writes the packed upper triangle (pair (i, j), i < j, at
 // i*n + j - (i+1)(i+2)/2; the values .dist would print: 1 - sim, or sim with
 // -d F) as little-endian float32 to the -o file and the sample names, one per
-// line, to <file>.names. With the shared-minimum kernel it is streamed to the
-// file in bands as the GPU computes them (no n x n buffer in host memory).
+// line, to <file>.names. With the shared-minimum kernel a triangle of more
+// than 2^28 pairs is streamed to the file in bands as the GPU computes them
+// (no n x n buffer in host memory).
 //
 // The database is read from $ParallelMETA/databases (db.config, tree/id.txt,
 // tree/order.txt, copy_number.txt). Errors exit with status 1 (the reference
@@ -58,6 +59,8 @@ namespace {
     printf("\t  --storage f32|bf16|f16   walk abundance storage, default f32\n");
     printf("\t  --device k               GPU index, default 0\n");
     printf("\t  --format dist|binary     text .dist (default) or packed float32 triangle + .names\n");
+    printf("\t  --stream-bands P         binary output with sharedmin: stream bands of P pairs (default: only\n");
+    printf("\t                           triangles above 2^28 pairs, in bands of 2^26)\n");
     printf("\t  -h Help\n");
     exit(0);
 }
@@ -98,6 +101,7 @@ int main(int argc, char **argv) {
     bool cp_correct = true, is_sim = false;
     int metric = 0, device = 0;
     bool binary = false;
+    int64_t band_pairs = 0;  // --stream-bands: stream binary output in bands of this many pairs
     cms_opts opts;
     cms_opts_default(&opts);
     if (argc == 1) printhelp();
@@ -125,6 +129,9 @@ int main(int argc, char **argv) {
             else die("unknown storage " + v);
         } else if (a == "--device") {
             device = atoi(need(1));
+        } else if (a == "--stream-bands") {
+            band_pairs = atoll(need(1));
+            if (band_pairs <= 0) die("--stream-bands needs a positive pair count");
         } else if (a == "--format") {
             std::string v = need(1);
             if (v == "dist") binary = false;
@@ -208,8 +215,13 @@ int main(int argc, char **argv) {
     const float none_f = 0.0f;
     if (on > 0) cms_db_order(db, &o1, &d1, &o2, &d2, &od);
     if (!o1) { o1 = o2 = od = &none_i; d1 = d2 = &none_f; }
-    const bool stream = binary && metric < 2 && opts.kernel == CMS_KERNEL_SHAREDMIN;
-    if (stream) opts.bin = 0;  // bands then arrive in the packed order of the table's samples
+    // binary output of a triangle above 1 GiB floats is streamed band by band
+    // (unbinned, so bands arrive in the packed order of the table's samples);
+    // a smaller one is computed whole with samples binned by sparsity
+    const int64_t npairs = (int64_t)n * (n > 0 ? n - 1 : 0) / 2;
+    const bool stream = binary && metric < 2 && opts.kernel == CMS_KERNEL_SHAREDMIN &&
+                        (band_pairs > 0 || npairs > ((int64_t)1 << 28));
+    if (opts.kernel == CMS_KERNEL_SHAREDMIN) opts.bin = stream ? 0 : 1;
     cms_ctx *ctx = nullptr;
     if (cms_create(device, &opts, &ctx) != CMS_OK) die(cms_last_error(nullptr));
     if (cms_set_tree(ctx, L, has_tree ? on : 0, o1, d1, o2, d2, od) != CMS_OK) die(cms_last_error(ctx));
@@ -237,7 +249,8 @@ int main(int argc, char **argv) {
         bo.is_sim = is_sim;
         if (!bo.f) die("Cannot open output file : " + out_path);
         if (stream) {
-            if (cms_all_pairs_stream(ctx, (int64_t)1 << 26, write_band, &bo) != CMS_OK) die(cms_last_error(ctx));
+            if (cms_all_pairs_stream(ctx, band_pairs > 0 ? band_pairs : (int64_t)1 << 26, write_band, &bo) != CMS_OK)
+                die(cms_last_error(ctx));
         } else {
             std::vector<float> sim((size_t)n * (n > 0 ? n - 1 : 0) / 2);
             if (cms_all_pairs(ctx, sim.data()) != CMS_OK) die(cms_last_error(ctx));

@@ -54,19 +54,20 @@ def test_cuda_comp_errors(cms, tmp_path):
 
 
 @pytest.mark.parametrize("case", ["config0", "core"])
-@pytest.mark.parametrize("kernel", ["walk", "sharedmin"])
-def test_cuda_comp_binary_output(cms, gpu_ok, golden, tmp_path, case, kernel):
+@pytest.mark.parametrize("kernel,extra", [("walk", []), ("sharedmin", []), ("sharedmin", ["--stream-bands", "1000"])])
+def test_cuda_comp_binary_output(cms, gpu_ok, golden, tmp_path, case, kernel, extra):
     """--format binary: packed float32 triangle of the values .dist prints.
     Walk: exactly (float)(1 - (double)sim) of the reference's Calc_sim; the
-    shared-minimum kernel streams bands to the file (<= 1e-5)."""
+    shared-minimum kernel whole (binned) or streamed to the file in bands
+    (<= 1e-5)."""
     import numpy as np
-    print(f"cuda-comp binary {case} {kernel}", flush=True)
+    print(f"cuda-comp binary {case} {kernel} {extra}", flush=True)
     g = golden(case)
     exe = os.path.join(cms.BIN_DIR, "cuda-comp")
     out = str(tmp_path / "o.bin")
     env = dict(os.environ, ParallelMETA=g.root)
-    r = subprocess.run([exe, "-T", g.table, "-D", "G", "-o", out, "--format", "binary", "--kernel", kernel], env=env,
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "-T", g.table, "-D", "G", "-o", out, "--format", "binary", "--kernel", kernel] + extra,
+                       env=env, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     names = open(out + ".names").read().split("\n")[:-1]
     assert names == [ln.split("\t")[0] for ln in g.read("dist_rT.dist").decode().splitlines()[1:]]
