@@ -195,7 +195,7 @@ typedef struct cms_stats {
     int nslots;                /* register slots after re-allocation */
     int scale_bits;            /* SHAREDMIN fixed-point scale (2^scale_bits per percent) */
     int entries;               /* SHAREDMIN active embedding entries */
-    int variant;               /* SHAREDMIN persistent-kernel variant in use (1..6) */
+    int variant;               /* SHAREDMIN persistent-kernel variant in use (1..12) */
     double tune_ms[12];        /* autotuner timings per variant (0 = not timed) */
     int64_t bands;             /* bands of the last streamed run */
     int64_t entries_visited;   /* SHAREDMIN: entries set in both blocks' bitmaps, summed over tiles */
