@@ -774,7 +774,7 @@ namespace {
 int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out_cap, int out_mode,
                  hipStream_t st, int64_t *done) {
     *done = 0;
-    if (!c->d_work) HIPCALL(c, dalloc(c->d_work, 1));
+    if (!c->d_work) HIPCALL(c, dalloc(c->d_work, kWorkWords));
     if (c->grid_waves == 0) {
         int cus = 0;
         HIPCALL(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -888,7 +888,7 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
     const double pairs = (double)c->n * (c->n - 1) / 2.0;
     c->stats.dense_updates = pairs * c->tree.order_n;
     if (is_leaf_metric(c)) {
-        if (!c->d_work) HIPCALL(c, dalloc(c->d_work, 1));
+        if (!c->d_work) HIPCALL(c, dalloc(c->d_work, kWorkWords));
         if (c->grid_waves == 0) {
             int cus = 0;
             HIPCALL(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));

@@ -654,7 +654,7 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     CREATE_HIP(hipMemset(s->qbm, 0, sizeof(uint32_t) * nqbm));
     CREATE_HIP(hipMalloc((void **)&s->qcolsum, sizeof(unsigned long long) * opts.slots));
     CREATE_HIP(hipMemset(s->qcolsum, 0, sizeof(unsigned long long) * opts.slots));
-    CREATE_HIP(hipMalloc((void **)&s->work, sizeof(unsigned long long)));
+    CREATE_HIP(hipMalloc((void **)&s->work, sizeof(unsigned long long) * kWorkWords));
     CREATE_HIP(hipMalloc((void **)&s->counter, sizeof(unsigned long long)));
     CREATE_HIP(hipMemset(s->counter, 0, sizeof(unsigned long long)));
     s->part_tiles = (int64_t)(opts.slots / kSmBlock) * std::max<int64_t>(s->band_blocks, 1);

@@ -114,6 +114,12 @@ constexpr int kCntSparsePairs = 3;  // pairs with mass on both sides in those en
 constexpr int kCntHalfRows = 4;     // entries light in one block only (half path)
 constexpr int kCntHalfPairs = 5;    // pairs with mass on both sides in those entries
 constexpr int kCounters = 8;
+// Tile counters of the persistent shared-minimum kernels: one per XCD, 64
+// bytes apart (kWorkWords words zeroed before each launch).
+constexpr int kXcds = 8;
+constexpr int kWorkStride = 8;
+constexpr int kWorkWords = kXcds * kWorkStride;
+constexpr unsigned kXcdRun = 256;  // consecutive tiles dealt to one XCD
 
 // Persistent all-pairs launch over upper-triangle block tiles
 // [t_begin, t_begin + t_count) of an nb x nb block grid. variant selects the
