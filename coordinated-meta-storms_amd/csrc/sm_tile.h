@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace cms {
 
@@ -257,15 +258,15 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
 // ACC is 32 x 33 uint32 (row = row-block sample, padded against bank
 // conflicts). Windows of 64 bitmap
 // words (2,048 entries). L (2,048 uint16) holds the staged entries' rows from
-// the front, then the half entries' rows (bit 15: the light block is B), and
+// the front, then the half entries light in A, then those light in B, and
 // the sparse entries' rows from the back (disjoint subsets of the window's
 // entries); W (64 uint4) holds each word's light bits and the ranks of the
 // light entries before it, from which record positions follow. Integer sums:
 // the pair's shared sum is (sa + sb - sum |a - b|) / 2 over the staged
 // entries plus ACC.
 struct SplitCounts {
-    // wave totals: entries, sparse entries and their pairs with mass, half
-    // entries and their pairs with mass
+    // tile totals: entries, sparse entries and their pairs with mass, half
+    // entries and the pairs they evaluate (32 per light sample)
     uint32_t visited = 0, sparse = 0, pairs = 0, half = 0, half_pairs = 0;
 };
 
@@ -284,10 +285,13 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0;
     for (int k = lane; k < 32 * kAccStride; k += 64) ACC[k] = 0u;
+    // the lane's discard word for the sparse and half loops: the staging
+    // buffer, which only the staged loop after them uses
+    uint32_t *const trash = S + lane;
     uint32_t cs = 0;                // this lane's column summed over the staged rows
     uint32_t baseA = 0, baseB = 0;  // light entries of each block before the window (ranks)
     uint32_t pairs_lane = 0;        // this lane's sparse pairs with mass in the window (<= 32 x 9)
-    uint32_t hp_lane = 0;           // this lane's half pairs with mass in the window (<= 32 x 96)
+    uint32_t hp_lane = 0;           // this lane's half pairs evaluated in the window (<= 32 x 96)
     for (int64_t wb = 0; wb < nwords; wb += 64) {
         const int64_t wi = wb + lane;
         uint32_t a = 0u, b = 0u, la = 0u, lb = 0u;
@@ -297,10 +301,12 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
             la = LA[wi];
             lb = LB[wi];
         }
-        const uint32_t both = a & b, sw = both & la & lb, hw = both & (la ^ lb), dw = both & ~(la | lb);
-        // 16-bit counts packed two per scan (totals <= 2,048): (staged | half << 16),
-        // (light A | light B << 16), sparse
-        const int c1 = __popc(dw) | (__popc(hw) << 16), c2 = __popc(la) | (__popc(lb) << 16), c3 = __popc(sw);
+        const uint32_t both = a & b, sw = both & la & lb, hwa = both & la & ~lb, hwb = both & lb & ~la,
+                       dw = both & ~(la | lb);
+        // 16-bit counts packed two per scan (totals <= 2,048): (staged | half
+        // light in A << 16), (light A | light B << 16), (sparse | half light in B << 16)
+        const int c1 = __popc(dw) | (__popc(hwa) << 16), c2 = __popc(la) | (__popc(lb) << 16),
+                  c3 = __popc(sw) | (__popc(hwb) << 16);
         int i1 = c1, i2 = c2, i3 = c3;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -313,8 +319,8 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         }
         const int t1 = __builtin_amdgcn_readfirstlane(__shfl(i1, 63, 64));
         const int t2 = __builtin_amdgcn_readfirstlane(__shfl(i2, 63, 64));
-        const int tots = __builtin_amdgcn_readfirstlane(__shfl(i3, 63, 64));
-        const int totd = t1 & 0xffff, toth = t1 >> 16;
+        const int t3 = __builtin_amdgcn_readfirstlane(__shfl(i3, 63, 64));
+        const int totd = t1 & 0xffff, tha = t1 >> 16, tots = t3 & 0xffff, thb = t3 >> 16, toth = tha + thb;
         const uint32_t winA = baseA, winB = baseB;
         baseA += (uint32_t)(t2 & 0xffff);
         baseB += (uint32_t)(t2 >> 16);
@@ -322,15 +328,13 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         cnt.visited += (uint32_t)(totd + toth + tots);
         cnt.sparse += (uint32_t)tots;
         cnt.half += (uint32_t)toth;
-        const int e1 = i1 - c1, e2 = i2 - c2;  // this lane's exclusive offsets
-        int posd = e1 & 0xffff, posh = totd + (e1 >> 16), poss = i3 - c3;
+        const int e1 = i1 - c1, e2 = i2 - c2, e3 = i3 - c3;  // this lane's exclusive offsets
+        int posd = e1 & 0xffff, posa = totd + (e1 >> 16), posb = totd + tha + (e3 >> 16), poss = e3 & 0xffff;
         __builtin_amdgcn_wave_barrier();
         if (tots + toth) W[lane] = make_uint4(la, lb, (uint32_t)(e2 & 0xffff), (uint32_t)(e2 >> 16));
         for (uint32_t m = dw; m; m &= m - 1u) L[posd++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
-        for (uint32_t m = hw; m; m &= m - 1u) {
-            const int bit = __ffs(m) - 1;
-            L[posh++] = (uint16_t)((lane * 32 + bit) | (((lb >> bit) & 1u) << 15));
-        }
+        for (uint32_t m = hwa; m; m &= m - 1u) L[posa++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
+        for (uint32_t m = hwb; m; m &= m - 1u) L[posb++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
         for (uint32_t m = sw; m; m &= m - 1u) L[2047 - poss++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -355,9 +359,10 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
             }
         };
         if (totd) load_chunk(0);
-        // sparse entries: one per lane
-        for (int j = 0; j < tots; j += 64) {
-            const int k = j + lane;
+        // sparse entries: one per lane, lane l taking entries l * niter + j
+        const int sniter = (tots + 63) >> 6;
+        for (int j = 0; j < sniter; j++) {
+            const int k = lane * sniter + j;
             if (k < tots) {
                 const uint32_t r = L[2047 - k];
                 const uint4 wd = W[r >> 5];
@@ -369,61 +374,79 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
                 uint32_t ia[3], ib[3], ma = qa.x, mb = qb.x;
 #pragma unroll
                 for (int t = 0; t < 3; t++) {
-                    ia[t] = ma ? (uint32_t)(__ffs(ma) - 1) : 0u;
+                    ia[t] = (uint32_t)(__ffs(ma) - 1) * kAccStride;
                     ma &= ma - 1u;
-                    ib[t] = mb ? (uint32_t)(__ffs(mb) - 1) : 0u;
+                    ib[t] = (uint32_t)(__ffs(mb) - 1);
                     mb &= mb - 1u;
                 }
+                // every lane adds 9 values; the pairs it does not have go
+                // to its own discard word
 #pragma unroll
                 for (int t = 0; t < 3; t++)
 #pragma unroll
                     for (int u = 0; u < 3; u++)
-                        if (t < na && u < nb) atomicAdd(ACC + ia[t] * kAccStride + ib[u], min(va[t], vb[u]));
+                        atomicAdd((t < na && u < nb) ? ACC + ia[t] + ib[u] : trash, min(va[t], vb[u]));
                 pairs_lane += (uint32_t)(na * nb);
             }
         }
         // half entries: one per lane, which reads the heavy block's 32 values
-        // of the row (one 128-byte line) and the light block's record
-        for (int j = 0; j < toth; j += 64) {
-            const int k = j + lane;
-            if (k < toth) {
-                const uint32_t e = L[totd + k];
-                const uint32_t r = e & 0x7ffu;
-                const bool blight = (e >> 15) != 0u;  // the light block is B, the heavy one A
-                const uint4 wd = W[r >> 5];
-                const uint32_t below = (1u << (r & 31u)) - 1u;
-                const uint4 q = blight ? RB[winB + wd.w + __popc(wd.y & below)] : RA[winA + wd.z + __popc(wd.x & below)];
-                const uint4 *hrow = reinterpret_cast<const uint4 *>(Y + (wb * 32 + r) * ystride + (blight ? abase : bbase));
-                uint4 h[8];
+        // of the row (one 128-byte line) and the light block's record; first
+        // those light in A (ACC row = the light sample), then those light in
+        // B (ACC column = the light sample). Lanes whose entries share a light
+        // sample add into the same cells (light samples cluster at the
+        // block's densest, first, samples), which serialises the atomics.
+        auto half = [&](int first, int count, auto light_b) {
+            constexpr bool BL = decltype(light_b)::value;
+            constexpr uint32_t kStep = BL ? (uint32_t)kAccStride : 1u;  // ACC words from heavy sample c to c + 1
+            // lane l takes entries l * niter + j: the entries of one step
+            // come from all over the window (different subtrees, so fewer
+            // lanes share a light sample), each lane's from one stretch
+            const int niter = (count + 63) >> 6;
+            for (int j = 0; j < niter; j++) {
+                const int k = lane * niter + j;
+                if (k < count) {
+                    const uint32_t r = L[first + k];
+                    const uint4 wd = W[r >> 5];
+                    const uint32_t below = (1u << (r & 31u)) - 1u;
+                    const uint4 q = BL ? RB[winB + wd.w + __popc(wd.y & below)] : RA[winA + wd.z + __popc(wd.x & below)];
+                    const uint4 *hrow =
+                        reinterpret_cast<const uint4 *>(Y + (wb * 32 + r) * ystride + (BL ? abase : bbase));
+                    uint4 h[8];  // heavy samples 4g .. 4g + 3
 #pragma unroll
-                for (int c = 0; c < 8; c++) h[c] = hrow[c];
-                const uint32_t vl[3] = {q.y, q.z, q.w};
-                const int nl = __popc(q.x);
-                uint32_t ml = q.x, il[3];
+                    for (int g = 0; g < 8; g++) h[g] = hrow[g];
+                    const uint32_t vl[3] = {q.y, q.z, q.w};
+                    const int nl = __popc(q.x);
+                    uint32_t ml = q.x, il[3];
 #pragma unroll
-                for (int t = 0; t < 3; t++) {
-                    il[t] = ml ? (uint32_t)(__ffs(ml) - 1) : 0u;
-                    ml &= ml - 1u;
-                }
-                // ACC index of (light sample il, heavy sample c)
-                const uint32_t step = blight ? (uint32_t)kAccStride : 1u;
-                uint32_t hn = 0u;
-#pragma unroll
-                for (int c = 0; c < 32; c++) {
-                    const uint4 hq = h[c >> 2];
-                    const uint32_t hv = (c & 3) == 0 ? hq.x : (c & 3) == 1 ? hq.y : (c & 3) == 2 ? hq.z : hq.w;
-                    if (hv) {
-                        hn++;
-#pragma unroll
-                        for (int t = 0; t < 3; t++)
-                            if (t < nl)
-                                atomicAdd(ACC + (blight ? il[t] : il[t] * kAccStride) + (uint32_t)c * step,
-                                          min(vl[t], hv));
+                    for (int t = 0; t < 3; t++) {
+                        il[t] = ml ? (uint32_t)(__ffs(ml) - 1) : 0u;
+                        ml &= ml - 1u;
                     }
+                    // per light sample: 32 minima, those with a zero heavy
+                    // value added to the lane's discard word
+#pragma unroll
+                    for (int t = 0; t < 3; t++)
+                        if (t < nl) {
+                            // the cell index is opaque to the optimiser, so the
+                            // 32 cells are formed by adds of constants instead
+                            // of 32 registers of precomputed offsets
+                            uint32_t ci = BL ? il[t] : il[t] * kAccStride;
+                            asm volatile("" : "+v"(ci));
+                            uint32_t *cell = ACC + ci;
+#pragma unroll
+                            for (int g = 0; g < 8; g++) {
+                                const uint32_t hq[4] = {h[g].x, h[g].y, h[g].z, h[g].w};
+#pragma unroll
+                                for (int e = 0; e < 4; e++)
+                                    atomicAdd(hq[e] ? cell + (g * 4 + e) * kStep : trash, min(vl[t], hq[e]));
+                            }
+                        }
+                    hp_lane += 32u * (uint32_t)nl;
                 }
-                hp_lane += hn * (uint32_t)nl;
             }
-        }
+        };
+        half(totd, tha, std::false_type{});
+        half(totd + tha, thb, std::true_type{});
         if (tots | toth) {
             uint32_t p = pairs_lane, h = hp_lane;
 #pragma unroll
