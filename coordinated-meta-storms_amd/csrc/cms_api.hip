@@ -302,7 +302,11 @@ int ensure_sharedmin(cms_ctx *c) {
     c->scale_bits = K;
     const int64_t nblk = c->npad / kSmBlock;
     const size_t nbm = (size_t)std::max<int64_t>(nblk, 1) * std::max<int64_t>(c->nwords, 1);
-    HIPCALL(c, dreserve(c->d_Y, c->cap_Y, (size_t)std::max<int64_t>(nrows, 1) * std::max<int64_t>(c->npad, 1)));
+    // Y holds one more row than the entries: an all-zero row the split tiles
+    // stage in place of missing rows (cleared here; the embedding writes
+    // only the entries' rows)
+    HIPCALL(c, dreserve(c->d_Y, c->cap_Y, (size_t)(std::max<int64_t>(nrows, 0) + 1) * std::max<int64_t>(c->npad, 1)));
+    HIPCALL(c, hipMemsetAsync(c->d_Y + (size_t)nrows * c->npad, 0, sizeof(uint32_t) * c->npad, c->stream));
     HIPCALL(c, dreserve(c->d_bm, c->cap_bm, nbm));
     HIPCALL(c, hipMemsetAsync(c->d_bm, 0, sizeof(uint32_t) * nbm, c->stream));
     // light-entry bitmaps of the split tiles (filled by the embedding)
@@ -341,6 +345,7 @@ SmData sm_data(const cms_ctx *c) {
     d.lbm = c->d_lbm;
     d.crl = c->d_crl;
     d.crs = c->crs;
+    d.zrow = c->sm_rows;  // the zero row after the entries' rows
     return d;
 }
 

@@ -384,7 +384,7 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
             SplitCounts sc;
             sm_tile_split<CHUNK, SquareRows<false>>(d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords,
                                                     d.lbm + cb * d.nwords, d.crl + ra * d.crs, d.crl + cb * d.crs,
-                                                    d.nwords, rows, d.Y, d.npad, abase, bbase, s_list[wave],
+                                                    d.nwords, d.zrow, rows, d.Y, d.npad, abase, bbase, s_list[wave],
                                                     s_words[wave], s_stage[wave], s_acc[wave], lane, acc, sc);
             // the tile's counts, added once per tile (wave totals over
             // thousands of tiles would not fit 32 bits)
@@ -645,7 +645,8 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
             sm_persistent_kernel<CH, WV, OUT><<<dim3((unsigned)wg), 64 * WV, 0, st>>>(d, nb, t_begin, t_count,   \
                                                                                    work, out, out_base);          \
     }
-    if (variant >= 6 && (!d.lbm || !d.crl)) return hipErrorInvalidValue;  // the split tiles read the light records
+    // the split tiles read the light records and the zero row
+    if (variant >= 6 && (!d.lbm || !d.crl || d.zrow < d.nrows)) return hipErrorInvalidValue;
     switch (variant) {
     case 0: CMS_PERSIST(8, 4, false); break;
     case 1: CMS_PERSIST(16, 4, false); break;

@@ -104,6 +104,10 @@ struct SmData {
     const uint32_t *lbm;          // null = none (the split tiles need them)
     const uint4 *crl;
     int64_t crs;                  // record stride per block (>= its light entries)
+    // An all-zero row of Y (index >= nrows; -1 = none): the split tiles
+    // stage it in the slots of a short last chunk, so every chunk runs the
+    // same unrolled accumulation.
+    int64_t zrow;
 };
 constexpr int kSmRecVals = 3;
 // Counters of a shared-minimum all-pairs run (SmData.counter[k]):

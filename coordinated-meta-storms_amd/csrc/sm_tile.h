@@ -86,9 +86,7 @@ struct SadOp {
     typedef uint32_t Acc;
     static constexpr bool kColSum = true;
     __device__ __forceinline__ static uint32_t sad(uint32_t a, uint32_t b, uint32_t c) {
-        uint32_t d;
-        asm("v_sad_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
-        return d;
+        return (a > b ? a - b : b - a) + c;
     }
     __device__ __forceinline__ static void add2(Acc &acc, uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1) {
         acc = sad(a1, b1, sad(a0, b0, acc));
@@ -275,6 +273,7 @@ constexpr int kAccStride = 33;
 template <int CHUNK, class Rows>
 __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t *B0, const uint32_t *LA,
                                               const uint32_t *LB, const uint4 *RA, const uint4 *RB, int64_t nwords,
+                                              int64_t zrow,
                                               const Rows &rows, const uint32_t *Y, int64_t ystride, int64_t abase,
                                               int64_t bbase, uint16_t *L, uint4 *W, uint32_t *S, uint32_t *ACC,
                                               int lane, uint32_t (&acc)[4][4], SplitCounts &cnt) {
@@ -345,15 +344,15 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         Rows win = rows;
         win.window(wb * 32);
         uint32_t v[CHUNK];
+        // slots past the list end stage the zero row (relative to the window)
+        const uint32_t zr = (uint32_t)(zrow - wb * 32);
         auto load_chunk = [&](int j) {
-            const uint32_t first =
-                (uint32_t)__builtin_amdgcn_readfirstlane((int)*reinterpret_cast<const uint32_t *>(L + j)) & 0xffffu;
 #pragma unroll
             for (int k = 0; k < CHUNK; k += 2) {
                 const uint32_t pr =
                     (uint32_t)__builtin_amdgcn_readfirstlane((int)*reinterpret_cast<const uint32_t *>(L + j + k));
-                const uint32_t r0 = (j + k < totd) ? (pr & 0xffffu) : first;
-                const uint32_t r1 = (j + k + 1 < totd) ? (pr >> 16) : first;
+                const uint32_t r0 = (j + k < totd) ? (pr & 0xffffu) : zr;
+                const uint32_t r1 = (j + k + 1 < totd) ? (pr >> 16) : zr;
                 v[k] = win.load(r0);
                 v[k + 1] = win.load(r1);
             }
@@ -414,33 +413,39 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
                     uint4 h[8];  // heavy samples 4g .. 4g + 3
 #pragma unroll
                     for (int g = 0; g < 8; g++) h[g] = hrow[g];
-                    const uint32_t vl[3] = {q.y, q.z, q.w};
+                    const uint32_t vl[3] = {q.y, q.z, q.w};  // zero past the record's samples
                     const int nl = __popc(q.x);
+                    // the record's light samples; slots past nl repeat the
+                    // first (their values are zero, so their minima add 0)
                     uint32_t ml = q.x, il[3];
+                    il[0] = (uint32_t)(__ffs(ml) - 1);
+                    ml &= ml - 1u;
 #pragma unroll
-                    for (int t = 0; t < 3; t++) {
-                        il[t] = ml ? (uint32_t)(__ffs(ml) - 1) : 0u;
+                    for (int t = 1; t < 3; t++) {
+                        il[t] = ml ? (uint32_t)(__ffs(ml) - 1) : il[0];
                         ml &= ml - 1u;
                     }
-                    // per light sample: 32 minima, those with a zero heavy
-                    // value added to the lane's discard word
+                    // the cells are opaque to the optimiser, so the 3 x 32
+                    // cells are the three bases plus immediate offsets
+                    // instead of registers of precomputed addresses
+                    uint32_t ci[3];
 #pragma unroll
-                    for (int t = 0; t < 3; t++)
-                        if (t < nl) {
-                            // the cell index is opaque to the optimiser, so the
-                            // 32 cells are formed by adds of constants instead
-                            // of 32 registers of precomputed offsets
-                            uint32_t ci = BL ? il[t] : il[t] * kAccStride;
-                            asm volatile("" : "+v"(ci));
-                            uint32_t *cell = ACC + ci;
+                    for (int t = 0; t < 3; t++) ci[t] = BL ? il[t] : il[t] * kAccStride;
+                    asm volatile("" : "+v"(ci[0]), "+v"(ci[1]), "+v"(ci[2]));
+                    // per heavy sample with mass: its minimum with each light
+                    // sample; lanes whose heavy value is zero sit out (exec
+                    // mask), which also keeps them off cells other lanes add to
 #pragma unroll
-                            for (int g = 0; g < 8; g++) {
-                                const uint32_t hq[4] = {h[g].x, h[g].y, h[g].z, h[g].w};
+                    for (int g = 0; g < 8; g++) {
+                        const uint32_t hq[4] = {h[g].x, h[g].y, h[g].z, h[g].w};
 #pragma unroll
-                                for (int e = 0; e < 4; e++)
-                                    atomicAdd(hq[e] ? cell + (g * 4 + e) * kStep : trash, min(vl[t], hq[e]));
+                        for (int e = 0; e < 4; e++)
+                            if (hq[e]) {
+#pragma unroll
+                                for (int t = 0; t < 3; t++)
+                                    atomicAdd(ACC + ci[t] + (g * 4 + e) * kStep, min(vl[t], hq[e]));
                             }
-                        }
+                    }
                     hp_lane += 32u * (uint32_t)nl;
                 }
             }
@@ -460,23 +465,25 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         }
         if (totd == 0) continue;
         for (int j = 0; j < totd; j += CHUNK) {
-            const int kn = min(CHUNK, totd - j);
-            if (kn == CHUNK) {
+            // a short last chunk holds zero rows past the list end (|0 - 0|
+            // and the column sums gain nothing), so every chunk runs the same
+            // unrolled accumulation
 #pragma unroll
-                for (int k = 0; k < CHUNK; k++) cs += v[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < CHUNK; k++) cs += (k < kn) ? v[k] : 0u;
-            }
+            for (int k = 0; k < CHUNK; k++) cs += v[k];
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int k = 0; k < CHUNK; k++) S[k * 64 + lane] = v[k];
-            if (kn & 1) S[kn * 64 + lane] = 0u;  // rows are accumulated in pairs
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // the column sum is complete here, before the next chunk's loads
+            // reuse this chunk's registers (otherwise the sum is sunk below
+            // the loads and the loaded values are copied back at the end of
+            // every chunk)
+            asm volatile("" ::"v"(cs));
             if (j + CHUNK < totd) load_chunk(j + CHUNK);
-            auto rows2 = [&](int k) {
+#pragma unroll
+            for (int k = 0; k < CHUNK; k += 2) {
                 const uint4 a0 = *reinterpret_cast<const uint4 *>(S + k * 64 + ty * 4);
                 const uint4 b0 = *reinterpret_cast<const uint4 *>(S + k * 64 + 32 + tx * 4);
                 const uint4 a1 = *reinterpret_cast<const uint4 *>(S + (k + 1) * 64 + ty * 4);
@@ -487,12 +494,6 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
                 for (int ii = 0; ii < 4; ii++)
 #pragma unroll
                     for (int jj = 0; jj < 4; jj++) SmOp::add2(acc[ii][jj], av0[ii], bv0[jj], av1[ii], bv1[jj]);
-            };
-            if (kn == CHUNK) {
-#pragma unroll
-                for (int k = 0; k < CHUNK; k += 2) rows2(k);
-            } else {
-                for (int k = 0; k < kn; k += 2) rows2(k);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
