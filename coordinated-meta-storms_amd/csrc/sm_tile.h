@@ -9,6 +9,11 @@
 
 namespace cms {
 
+// Word 3 of a raw buffer resource on gfx9-family parts (32-bit data format,
+// no swizzle, no index stride): buffer loads through it address base +
+// offset bytes like a global load.
+constexpr int kRawBufferFlags = 0x00020000;
+
 // Row loaders: value of this lane's sample at row rbase + r of Y, with
 // window(rbase) called once per 64-word bitmap window and r < 2048. Row
 // addresses come from the wave-uniform row index in scalar registers (64-bit
@@ -29,8 +34,13 @@ struct SquareRows {
     const uint32_t *win = nullptr;
     __device__ __forceinline__ void window(int64_t rbase) { win = Y + rbase * stride; }
     __device__ __forceinline__ uint32_t load(uint32_t r) const {
-        const char *row = reinterpret_cast<const char *>(win + (uint64_t)r * (uint64_t)stride);
-        const uint32_t v = *reinterpret_cast<const uint32_t *>(row + col4);
+        // a raw buffer whose base is the row (scalar registers) and whose
+        // offset is the lane's column: the address costs scalar
+        // instructions only (a global load of the same row would add a 64-bit
+        // vector address per load)
+        const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint32_t *>(win + (uint64_t)r * (uint64_t)stride), (short)0, (int)0xffffffff, kRawBufferFlags);
+        const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(row, (int)col4, 0, 0);
         return (!CHECK || ok) ? v : 0u;
     }
 };
@@ -291,14 +301,24 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
     uint32_t baseA = 0, baseB = 0;  // light entries of each block before the window (ranks)
     uint32_t pairs_lane = 0;        // this lane's sparse pairs with mass in the window (<= 32 x 9)
     uint32_t hp_lane = 0;           // this lane's half pairs evaluated in the window (<= 32 x 96)
+    // the four bitmap words of the lane in the next window, loaded one window
+    // ahead so their latency hides behind the current window's work
+    uint32_t na = 0u, nb = 0u, nla = 0u, nlb = 0u;
+    if (lane < nwords) {
+        na = A0[lane];
+        nb = B0[lane];
+        nla = LA[lane];
+        nlb = LB[lane];
+    }
     for (int64_t wb = 0; wb < nwords; wb += 64) {
-        const int64_t wi = wb + lane;
-        uint32_t a = 0u, b = 0u, la = 0u, lb = 0u;
-        if (wi < nwords) {
-            a = A0[wi];
-            b = B0[wi];
-            la = LA[wi];
-            lb = LB[wi];
+        const uint32_t a = na, b = nb, la = nla, lb = nlb;
+        const int64_t wn = wb + 64 + lane;
+        na = nb = nla = nlb = 0u;
+        if (wn < nwords) {
+            na = A0[wn];
+            nb = B0[wn];
+            nla = LA[wn];
+            nlb = LB[wn];
         }
         const uint32_t both = a & b, sw = both & la & lb, hwa = both & la & ~lb, hwb = both & lb & ~la,
                        dw = both & ~(la | lb);
