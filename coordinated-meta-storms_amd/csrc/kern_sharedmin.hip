@@ -342,11 +342,11 @@ template <int CHUNK, int WAVES, int OUT, bool SPLIT>
 __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, int64_t t_begin, int64_t t_count,
                                                    unsigned long long *__restrict__ work, float *__restrict__ out,
                                                    int64_t out_base) {
-    // 2,048 entry rows (uint16) per window; split tiles: the window's word
-    // records and the sparse accumulator
-    __shared__ __align__(16) uint16_t s_list[WAVES][2048];
+    // entry rows (uint16) of one window: 2,048 (split tiles: kSplitList);
+    // split tiles also hold the window's word records and the accumulator
+    __shared__ __align__(16) uint16_t s_list[WAVES][SPLIT ? kSplitList : 2048];
     __shared__ uint32_t s_stage[WAVES][CHUNK * 64];
-    __shared__ __align__(16) uint4 s_words[SPLIT ? WAVES : 1][SPLIT ? 64 : 1];
+    __shared__ __align__(16) uint4 s_words[SPLIT ? WAVES : 1][SPLIT ? kSplitWords : 1];
     __shared__ __align__(16) uint32_t s_acc[SPLIT ? WAVES : 1][SPLIT ? 32 * kAccStride : 4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ty = lane >> 3, tx = lane & 7;

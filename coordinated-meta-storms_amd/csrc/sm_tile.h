@@ -264,12 +264,13 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
 //   - staged entries, the rest: the |a - b| path of sm_tile_accumulate.
 //
 // ACC is 32 x 33 uint32 (row = row-block sample, padded against bank
-// conflicts). Windows of 64 bitmap
-// words (2,048 entries). L (2,048 uint16) holds the staged entries' rows from
-// the front, then the half entries light in A, then those light in B, and
-// the sparse entries' rows from the back (disjoint subsets of the window's
-// entries); W (64 uint4) holds each word's light bits and the ranks of the
-// light entries before it, from which record positions follow. Integer sums:
+// conflicts). Windows of kSplitWords bitmap words (kSplitList entries; lanes
+// past kSplitWords hold empty words). L (kSplitList uint16) holds the staged
+// entries' rows from the front, then the half entries light in A, then those
+// light in B, and the sparse entries' rows from the back (disjoint subsets of
+// the window's entries); W (kSplitWords uint4) holds each word's light bits
+// and the ranks of the light entries before it, from which record positions
+// follow. Integer sums:
 // the pair's shared sum is (sa + sb - sum |a - b|) / 2 over the staged
 // entries plus ACC.
 struct SplitCounts {
@@ -279,6 +280,12 @@ struct SplitCounts {
 };
 
 constexpr int kAccStride = 33;
+// Split tiles walk the bitmaps in windows of 48 words (1,536 entries): the
+// window's entry list, word records, staging rows and accumulator then take
+// 10,112 bytes of LDS per wave, so 16 waves fit in a CU's 160 KB (4 per SIMD,
+// as many as 128 VGPRs allow).
+constexpr int kSplitWords = 48;
+constexpr int kSplitList = kSplitWords * 32;
 
 template <int CHUNK, class Rows>
 __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t *B0, const uint32_t *LA,
@@ -304,17 +311,17 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
     // the four bitmap words of the lane in the next window, loaded one window
     // ahead so their latency hides behind the current window's work
     uint32_t na = 0u, nb = 0u, nla = 0u, nlb = 0u;
-    if (lane < nwords) {
+    if (lane < kSplitWords && lane < nwords) {
         na = A0[lane];
         nb = B0[lane];
         nla = LA[lane];
         nlb = LB[lane];
     }
-    for (int64_t wb = 0; wb < nwords; wb += 64) {
+    for (int64_t wb = 0; wb < nwords; wb += kSplitWords) {
         const uint32_t a = na, b = nb, la = nla, lb = nlb;
-        const int64_t wn = wb + 64 + lane;
+        const int64_t wn = wb + kSplitWords + lane;
         na = nb = nla = nlb = 0u;
-        if (wn < nwords) {
+        if (lane < kSplitWords && wn < nwords) {
             na = A0[wn];
             nb = B0[wn];
             nla = LA[wn];
@@ -350,11 +357,11 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         const int e1 = i1 - c1, e2 = i2 - c2, e3 = i3 - c3;  // this lane's exclusive offsets
         int posd = e1 & 0xffff, posa = totd + (e1 >> 16), posb = totd + tha + (e3 >> 16), poss = e3 & 0xffff;
         __builtin_amdgcn_wave_barrier();
-        if (tots + toth) W[lane] = make_uint4(la, lb, (uint32_t)(e2 & 0xffff), (uint32_t)(e2 >> 16));
+        if ((tots + toth) && lane < kSplitWords) W[lane] = make_uint4(la, lb, (uint32_t)(e2 & 0xffff), (uint32_t)(e2 >> 16));
         for (uint32_t m = dw; m; m &= m - 1u) L[posd++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
         for (uint32_t m = hwa; m; m &= m - 1u) L[posa++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
         for (uint32_t m = hwb; m; m &= m - 1u) L[posb++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
-        for (uint32_t m = sw; m; m &= m - 1u) L[2047 - poss++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
+        for (uint32_t m = sw; m; m &= m - 1u) L[kSplitList - 1 - poss++] = (uint16_t)(lane * 32 + __ffs(m) - 1);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -378,12 +385,13 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
             }
         };
         if (totd) load_chunk(0);
-        // sparse entries: one per lane, lane l taking entries l * niter + j
+        // sparse entries: one per lane, lane l taking entries 64 j + l
+        // (neighbouring lanes read neighbouring records, which share lines)
         const int sniter = (tots + 63) >> 6;
         for (int j = 0; j < sniter; j++) {
-            const int k = lane * sniter + j;
+            const int k = j * 64 + lane;
             if (k < tots) {
-                const uint32_t r = L[2047 - k];
+                const uint32_t r = L[kSplitList - 1 - k];
                 const uint4 wd = W[r >> 5];
                 const uint32_t below = (1u << (r & 31u)) - 1u;
                 const uint4 qa = RA[winA + wd.z + __popc(wd.x & below)];
