@@ -564,7 +564,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=0,
                     help="autotune the persistent-kernel variant (default: the library's default variant, "
                          "the one the committed profiles measure)")
-    ap.add_argument("--variant", type=int, default=0, help="force a persistent-kernel variant (1..12; 7..12 take nearly empty entries out of the staged path)")
+    ap.add_argument("--variant", type=int, default=0, help="force a persistent-kernel variant (1..16; 7..16 take nearly empty entries out of the staged path, "
+                         "13..16 share each tile among 2 or 4 waves)")
     ap.add_argument("--workload", type=int, default=1, choices=[1, 2, 3, 4],
                     help="1: headline all-pairs; 2: precision (16-bit walk storage, 50,000 samples); "
                          "3: batched stream (100,000 samples of uneven sparsity); "

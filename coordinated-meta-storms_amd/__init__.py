@@ -70,7 +70,7 @@ class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("prep_ms", C.c_double), ("pair_node_updates", C.c_double),
                 ("dense_updates", C.c_double), ("valu_ops", C.c_double), ("bytes_moved", C.c_double),
                 ("tiles", C.c_int64), ("nslots", C.c_int), ("scale_bits", C.c_int), ("entries", C.c_int),
-                ("variant", C.c_int), ("tune_ms", C.c_double * 12), ("bands", C.c_int64),
+                ("variant", C.c_int), ("tune_ms", C.c_double * 16), ("bands", C.c_int64),
                 ("entries_visited", C.c_int64), ("entries_staged", C.c_int64), ("entries_sparse", C.c_int64),
                 ("sparse_pairs", C.c_int64), ("entries_half", C.c_int64), ("half_pairs", C.c_int64)]
 

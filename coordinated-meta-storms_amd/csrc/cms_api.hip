@@ -95,6 +95,7 @@ void drop_layouts(cms_ctx *c) {
     dfree(c->d_lmstat);
     dfree(c->d_lmperm);
     c->lm_layout = -1;
+    c->variant = -1;  // chosen again for the new samples or tree
 }
 
 // -M 2..5 run on the leaf-metric kernels: with CMS_KERNEL_WALK the merge
@@ -345,6 +346,7 @@ SmData sm_data(const cms_ctx *c) {
     d.lbm = c->d_lbm;
     d.crl = c->d_crl;
     d.crs = c->crs;
+    d.pl = c->d_pl;
     d.zrow = c->sm_rows;  // the zero row after the entries' rows
     return d;
 }
@@ -505,7 +507,7 @@ const char *opts_error(const cms_opts *o) {
     if ((o->tile_a != 0 && o->tile_a != kSmBlock) || (o->tile_b != 0 && o->tile_b != kSmBlock))
         return "tile_a / tile_b: the shared-minimum tiles are 32 x 32 samples per wave (0 or 32)";
     if (o->chunk != 0 && o->chunk != 8 && o->chunk != 16 && o->chunk != 32) return "chunk must be 0, 8, 16 or 32";
-    if (o->variant < 0 || o->variant > kSmVariants) return "variant must be 0..12";
+    if (o->variant < 0 || o->variant > kSmVariants) return "variant must be 0..16";
     if (o->variant > 0 && o->chunk != 0 && kSmVariantTable[o->variant - 1].chunk != o->chunk)
         return "variant and chunk disagree";
     if (o->metric < CMS_METRIC_WEIGHTED || o->metric > CMS_METRIC_BC) return "unknown metric";
@@ -605,6 +607,7 @@ int cms_set_opts(cms_ctx *c, const cms_opts *o) {
     if (c->servers > 0) return fail(c, CMS_E_STATE, "the context is serving (destroy its server first)");
     if (opts_error(o)) return fail(c, CMS_E_INVAL, opts_error(o));
     c->opts = *o;
+    c->variant = -1;  // chosen again under the new options
     return CMS_OK;
 }
 
@@ -760,8 +763,10 @@ int cms_set_samples(cms_ctx *c, int n, const float *abd) {
 
 namespace {
 
-// Persistent-kernel variant: forced by opts.variant, else the default, or the
-// fastest of all variants when opts.autotune is set. Tiles [0, ntiles) are
+// Persistent-kernel variant: forced by opts.variant, else the default (split
+// tiles shared by four waves when the whole triangle has fewer than
+// kSmSharedTileMaxTiles tiles, one wave per tile otherwise), or the fastest
+// of all variants when opts.autotune is set. Tiles [0, ntiles) are
 // the range the caller is about to compute into `out` (out_mode as in
 // launch_sm_persistent, out_base 0; with out_mode 1 the caller passes only
 // tiles whose packed stored-position indices fit `out`).
@@ -778,6 +783,7 @@ namespace {
 // *done = 0).
 int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out_cap, int out_mode,
                  hipStream_t st, int64_t *done) {
+    const int64_t all_tiles = nb * (nb + 1) / 2;  // the whole triangle (the default depends on it)
     *done = 0;
     if (!c->d_work) HIPCALL(c, dalloc(c->d_work, kWorkWords));
     if (c->grid_waves == 0) {
@@ -793,7 +799,10 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
     // opts.chunk (if set) restricts the candidates to that LDS chunk
     auto allowed = [&](int v) { return c->opts.chunk == 0 || kSmVariantTable[v].chunk == c->opts.chunk; };
     if (!c->opts.autotune || ntiles <= 0) {
-        c->variant = kSmDefaultVariant;
+        // few tiles: four waves share each tile (at 2,048 samples, 2,080
+        // tiles: 3.2 ms against 4.5 ms with one wave per tile; at 4,096
+        // samples one wave per tile is faster)
+        c->variant = all_tiles < kSmSharedTileMaxTiles ? kSmSharedTileVariant : kSmDefaultVariant;
         if (!allowed(c->variant))
             for (int v = 0; v < kSmVariants; v++)
                 if (allowed(v)) {

@@ -328,6 +328,69 @@ __global__ __launch_bounds__(64 * kSmWaves) void sm_rect_kernel(SmData d, int64_
         }
 }
 
+// The similarity of stored pair (pa, pb) of an upper-triangle block tile
+// (pairs at or below the diagonal and past the last sample skipped).
+//   OUT 0: the packed upper triangle in the caller's sample order (stored
+//          position -> sample id through d.perm, if any);
+//   OUT 1: the packed triangle of stored positions, minus out_base.
+// (Per pair: a helper taking the lane's 4 x 4 sums by reference put them in
+// scratch memory.)
+template <int OUT>
+__device__ __forceinline__ void sm_store_pair(const SmData &d, uint32_t acc, int64_t pa, int64_t pb,
+                                              float *__restrict__ out, int64_t out_base) {
+    if (pa >= pb || pb >= d.n) return;
+    const float sim = sm_finish(d, acc, pa, pb);
+    if constexpr (OUT == 0) {
+        int64_t a = d.perm ? d.perm[pa] : pa, b = d.perm ? d.perm[pb] : pb;
+        if (a > b) {
+            const int64_t x = a;
+            a = b;
+            b = x;
+        }
+        out[a * (int64_t)d.n + b - (a + 1) * (a + 2) / 2] = sim;
+    } else {
+        out[pa * (int64_t)d.n + pb - (pa + 1) * (pa + 2) / 2 - out_base] = sim;
+    }
+}
+
+// A split tile's counts (one wave's share), added once per tile: wave totals
+// over thousands of tiles would not fit 32 bits.
+__device__ __forceinline__ void sm_add_counts(const SmData &d, const SplitCounts &sc) {
+    if (!d.counter || !sc.visited) return;
+    atomicAdd(d.counter + kCntVisited, (unsigned long long)sc.visited);
+    atomicAdd(d.counter + kCntStaged, (unsigned long long)(sc.visited - sc.sparse - sc.half));
+    if (sc.sparse) {
+        atomicAdd(d.counter + kCntSparseRows, (unsigned long long)sc.sparse);
+        atomicAdd(d.counter + kCntSparsePairs, (unsigned long long)sc.pairs);
+    }
+    if (sc.half) {
+        atomicAdd(d.counter + kCntHalfRows, (unsigned long long)sc.half);
+        atomicAdd(d.counter + kCntHalfPairs, (unsigned long long)sc.half_pairs);
+    }
+}
+
+// Next tile of the persistent launches: tiles are dealt to the XCDs in runs
+// of kXcdRun consecutive tiles (run q to XCD q mod 8; workgroups are
+// dispatched to the XCDs round-robin), so the waves of one XCD work on few
+// row blocks at a time and share those blocks' Y lines in its L2 (at 10,000
+// samples: L2 hits 15 % -> 40 % of requests, HBM reads -30 %). The calling
+// wave (every lane; src and left stay wave-uniform) drains its own XCD's
+// counter (src, initially blockIdx.x mod 8), then each other XCD's; -1 when
+// all eight are past the range (left counts the counters not yet found
+// empty).
+__device__ __forceinline__ int64_t sm_next_tile(unsigned long long *__restrict__ work, int64_t t_count, int &src,
+                                                int &left, int lane) {
+    while (left > 0) {
+        unsigned long long k = 0;
+        if (lane == 0) k = atomicAdd(work + src * kWorkStride, 1ull);
+        k = __shfl(k, 0, 64);
+        const unsigned long long t = ((k / kXcdRun) * kXcds + (unsigned)src) * kXcdRun + k % kXcdRun;
+        if (t < (unsigned long long)t_count) return (int64_t)t;
+        if (--left > 0) src = (src + 1) & (kXcds - 1);
+    }
+    return -1;
+}
+
 // All pairs, persistent: a fixed grid of waves pulls upper-triangle block
 // tiles [t_begin, t_begin + t_count) from a global counter (tiles are
 // row-major over blocks; with samples binned densest-first the heaviest tiles
@@ -349,24 +412,11 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
     __shared__ __align__(16) uint4 s_words[SPLIT ? WAVES : 1][SPLIT ? kSplitWords : 1];
     __shared__ __align__(16) uint32_t s_acc[SPLIT ? WAVES : 1][SPLIT ? 32 * kAccStride : 4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ty = lane >> 3, tx = lane & 7;
-    // Tiles are dealt to the XCDs in runs of kXcdRun consecutive tiles (run
-    // q to XCD q mod 8; workgroups are dispatched to the XCDs round-robin),
-    // so the waves of one XCD work on few row blocks at a time and share
-    // those blocks' Y lines in its L2 (at 10,000 samples: L2 hits 15 % -> 40 %
-    // of requests, HBM reads -30 %). A wave drains its own XCD's counter,
-    // then each other XCD's; it exits when all eight are past the range.
+    // each wave takes its own tiles
     int src = (int)(blockIdx.x & (kXcds - 1)), left = kXcds;
     for (;;) {
-        unsigned long long k = 0;
-        if (lane == 0) k = atomicAdd(work + src * kWorkStride, 1ull);
-        k = __shfl(k, 0, 64);
-        const unsigned long long t = ((k / kXcdRun) * kXcds + (unsigned)src) * kXcdRun + k % kXcdRun;
-        if (t >= (unsigned long long)t_count) {
-            if (--left == 0) break;
-            src = (src + 1) & (kXcds - 1);
-            continue;
-        }
+        const int64_t t = sm_next_tile(work, t_count, src, left, lane);
+        if (t < 0) break;
         int64_t ra, cb;
         sm_decode_upper(t_begin + (int64_t)t, nb, ra, cb);
         // block indices are wave-uniform: keep them (and every address
@@ -386,20 +436,7 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
                                                     d.lbm + cb * d.nwords, d.crl + ra * d.crs, d.crl + cb * d.crs,
                                                     d.nwords, d.zrow, rows, d.Y, d.npad, abase, bbase, s_list[wave],
                                                     s_words[wave], s_stage[wave], s_acc[wave], lane, acc, sc);
-            // the tile's counts, added once per tile (wave totals over
-            // thousands of tiles would not fit 32 bits)
-            if (d.counter && lane == 0 && sc.visited) {
-                atomicAdd(d.counter + kCntVisited, (unsigned long long)sc.visited);
-                atomicAdd(d.counter + kCntStaged, (unsigned long long)(sc.visited - sc.sparse - sc.half));
-                if (sc.sparse) {
-                    atomicAdd(d.counter + kCntSparseRows, (unsigned long long)sc.sparse);
-                    atomicAdd(d.counter + kCntSparsePairs, (unsigned long long)sc.pairs);
-                }
-                if (sc.half) {
-                    atomicAdd(d.counter + kCntHalfRows, (unsigned long long)sc.half);
-                    atomicAdd(d.counter + kCntHalfPairs, (unsigned long long)sc.half_pairs);
-                }
-            }
+            if (lane == 0) sm_add_counts(d, sc);
         } else {
             const uint32_t visited = sm_square_tile<CHUNK, false>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords,
                                                                   nullptr, abase, bbase, s_list[wave], s_stage[wave],
@@ -409,25 +446,12 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
                 atomicAdd(d.counter + kCntStaged, (unsigned long long)visited);
             }
         }
+        const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
         for (int ii = 0; ii < 4; ii++)
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
-                const int64_t pa = abase + ty * 4 + ii, pb = bbase + tx * 4 + jj;
-                if (pa >= pb || pb >= d.n) continue;
-                const float sim = sm_finish(d, acc[ii][jj], pa, pb);
-                if constexpr (OUT == 0) {
-                    int64_t a = d.perm ? d.perm[pa] : pa, b = d.perm ? d.perm[pb] : pb;
-                    if (a > b) {
-                        const int64_t x = a;
-                        a = b;
-                        b = x;
-                    }
-                    out[a * (int64_t)d.n + b - (a + 1) * (a + 2) / 2] = sim;
-                } else {
-                    out[pa * (int64_t)d.n + pb - (pa + 1) * (pa + 2) / 2 - out_base] = sim;
-                }
-            }
+            for (int jj = 0; jj < 4; jj++)
+                sm_store_pair<OUT>(d, acc[ii][jj], abase + ty * 4 + ii, bbase + tx * 4 + jj, out, out_base);
     }
 }
 
@@ -446,6 +470,70 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(CHUN
     SmData d, int64_t nb, int64_t t_begin, int64_t t_count, unsigned long long *__restrict__ work,
     float *__restrict__ out, int64_t out_base) {
     sm_persistent_body<CHUNK, WAVES, OUT, true>(d, nb, t_begin, t_count, work, out, out_base);
+}
+
+// Split tiles with PARTS waves per tile (1024 / PARTS pairs per wave): a
+// workgroup of PARTS waves takes one tile at a time, each wave walks every
+// PARTS-th bitmap window, and the waves meet in the workgroup's accumulator:
+// more waves per tile fill the device when the sample set has few tiles.
+// Integer sums: every PARTS gives the same bits. Up to 168 VGPRs (3 waves per
+// SIMD) for the 8-row chunks: at 128 their registers spill.
+template <int CHUNK, int PARTS, int OUT>
+__global__ __launch_bounds__(64 * PARTS) __attribute__((amdgpu_waves_per_eu(CHUNK <= 8 ? 3 : 1))) void sm_persistent_parts_kernel(
+    SmData d, int64_t nb, int64_t t_begin, int64_t t_count, unsigned long long *__restrict__ work,
+    float *__restrict__ out, int64_t out_base) {
+    __shared__ __align__(16) uint16_t s_list[PARTS][kSplitList];
+    __shared__ uint32_t s_stage[PARTS][CHUNK * 64];
+    __shared__ __align__(16) uint4 s_words[PARTS][kSplitWords];
+    __shared__ __align__(16) uint32_t s_acc[32 * kAccStride];
+    __shared__ long long s_tile;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ty = lane >> 3, tx = lane & 7;
+    int src = (int)(blockIdx.x & (kXcds - 1)), left = kXcds;  // wave 0 deals the workgroup's tiles
+    for (;;) {
+        if (wave == 0) {
+            const int64_t t = sm_next_tile(work, t_count, src, left, lane);
+            if (lane == 0) s_tile = t;
+        }
+        for (int k = threadIdx.x; k < 32 * kAccStride; k += 64 * PARTS) s_acc[k] = 0u;
+        __syncthreads();
+        const long long t = s_tile;
+        if (t < 0) break;
+        int64_t ra, cb;
+        sm_decode_upper(t_begin + (int64_t)t, nb, ra, cb);
+        ra = __builtin_amdgcn_readfirstlane((int)ra);
+        cb = __builtin_amdgcn_readfirstlane((int)cb);
+        const int64_t abase = ra * kSmBlock, bbase = cb * kSmBlock;
+        SquareRows<false> rows;
+        rows.Y = d.Y;
+        rows.stride = d.npad;
+        rows.ok = true;
+        rows.col4 = (uint32_t)((lane < 32) ? abase + lane : bbase + (lane - 32)) * 4u;
+        SplitCounts sc;
+        uint32_t acc[4][4];
+        sm_tile_split<CHUNK, SquareRows<false>, PARTS>(
+            d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords, d.lbm + cb * d.nwords,
+            d.crl + ra * d.crs, d.crl + cb * d.crs, d.nwords, d.zrow, rows, d.Y, d.npad, abase, bbase, s_list[wave],
+            s_words[wave], s_stage[wave], s_acc, lane, acc, sc, wave, d.pl + ra * d.nwords, d.pl + cb * d.nwords);
+        if (lane == 0) sm_add_counts(d, sc);
+        // every wave's staged sums join the sparse and half sums
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (acc[i][j]) atomicAdd(s_acc + (ty * 4 + i) * kAccStride + tx * 4 + j, acc[i][j]);
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    sm_store_pair<OUT>(d, s_acc[(ty * 4 + i) * kAccStride + tx * 4 + j], abase + ty * 4 + i,
+                                       bbase + tx * 4 + j, out, out_base);
+        }
+        // the accumulator and the tile id are rewritten for the next tile
+        __syncthreads();
+    }
 }
 
 // Explicit pair list (caller's sample ids; d.pos maps them to stored
@@ -645,8 +733,16 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
             sm_persistent_kernel<CH, WV, OUT><<<dim3((unsigned)wg), 64 * WV, 0, st>>>(d, nb, t_begin, t_count,   \
                                                                                    work, out, out_base);          \
     }
-    // the split tiles read the light records and the zero row
+#define CMS_PARTS(CH, PT)                                                                                        \
+    {                                                                                                            \
+        const int64_t wg = std::max<int64_t>(1, std::min<int64_t>(t_count, grid_waves / PT));                    \
+        sm_persistent_parts_kernel<CH, PT, OUT><<<dim3((unsigned)wg), 64 * PT, 0, st>>>(d, nb, t_begin, t_count,  \
+                                                                                      work, out, out_base);       \
+    }
+    // the split tiles read the light records and the zero row; tiles shared
+    // by several waves also the light prefixes
     if (variant >= 6 && (!d.lbm || !d.crl || d.zrow < d.nrows)) return hipErrorInvalidValue;
+    if (variant >= 12 && !d.pl) return hipErrorInvalidValue;
     switch (variant) {
     case 0: CMS_PERSIST(8, 4, false); break;
     case 1: CMS_PERSIST(16, 4, false); break;
@@ -659,9 +755,14 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
     case 8: CMS_PERSIST(8, 4, true); break;
     case 9: CMS_PERSIST(16, 2, true); break;
     case 10: CMS_PERSIST(16, 1, true); break;
-    default: CMS_PERSIST(16, 4, true); break;
+    case 11: CMS_PERSIST(16, 4, true); break;
+    case 12: CMS_PARTS(8, 2); break;
+    case 13: CMS_PARTS(8, 4); break;
+    case 14: CMS_PARTS(16, 2); break;
+    default: CMS_PARTS(16, 4); break;
     }
 #undef CMS_PERSIST
+#undef CMS_PARTS
     return hipGetLastError();
 }
 

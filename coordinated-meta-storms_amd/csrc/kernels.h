@@ -104,6 +104,7 @@ struct SmData {
     const uint32_t *lbm;          // null = none (the split tiles need them)
     const uint4 *crl;
     int64_t crs;                  // record stride per block (>= its light entries)
+    const uint32_t *pl;           // pl[b * nwords + w] = light entries of block b before word w
     // An all-zero row of Y (index >= nrows; -1 = none): the split tiles
     // stage it in the slots of a short last chunk, so every chunk runs the
     // same unrolled accumulation.
@@ -133,14 +134,18 @@ constexpr unsigned kXcdRun = 256;  // consecutive tiles dealt to one XCD
 // floats; a launch that could write outside them is refused on the host
 // (hipErrorInvalidValue) before anything is queued.
 // Variants: CHUNK x WAVES of the plain tiles (1..6) and of the split tiles
-// (7..12), whose entries light in both blocks take the sparse path.
-constexpr int kSmVariants = 12;
+// (7..12), whose entries light in both blocks take the sparse path, one wave
+// per tile (1,024 pairs per wave); and split tiles shared by PARTS waves of
+// a workgroup (13..16: 512 or 256 pairs per wave), for sample sets with
+// fewer tiles than the device has wave slots.
+constexpr int kSmVariants = 16;
 struct SmVariant {
-    int chunk, waves, split;
+    int chunk, waves, split, parts;  // parts: waves per tile
 };
-constexpr SmVariant kSmVariantTable[kSmVariants] = {{8, 4, 0},  {16, 4, 0}, {32, 4, 0}, {16, 2, 0},
-                                                    {32, 2, 0}, {16, 8, 0}, {8, 2, 1},  {8, 1, 1},
-                                                    {8, 4, 1},  {16, 2, 1}, {16, 1, 1}, {16, 4, 1}};
+constexpr SmVariant kSmVariantTable[kSmVariants] = {
+    {8, 4, 0, 1},  {16, 4, 0, 1}, {32, 4, 0, 1}, {16, 2, 0, 1}, {32, 2, 0, 1}, {16, 8, 0, 1},
+    {8, 2, 1, 1},  {8, 1, 1, 1},  {8, 4, 1, 1},  {16, 2, 1, 1}, {16, 1, 1, 1}, {16, 4, 1, 1},
+    {8, 2, 1, 2},  {8, 4, 1, 4},  {16, 2, 1, 2}, {16, 4, 1, 4}};
 // Light-entry prefix (PL[b][w] = light entries of block b before word w) and
 // the records of every block's light entries in rank order, from Y and the
 // light bitmaps; crs >= the most light entries of any block.
@@ -148,6 +153,8 @@ hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t 
                                 uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st);
 constexpr int kSmCrossVariants = 6;   // the serving cross kernel: the plain tiles' variants 1..6
 constexpr int kSmDefaultVariant = 7;  // without autotune: split tiles, 8-entry chunk, 1 wave per workgroup
+constexpr int kSmSharedTileVariant = 13;       // without autotune, below kSmSharedTileMaxTiles tiles: split
+constexpr int64_t kSmSharedTileMaxTiles = 4096;  // tiles shared by 4 waves, 8-entry chunk
 hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
                                 unsigned long long *work, float *out, int64_t out_cap, int out_mode,
                                 int64_t out_base, int grid_waves, hipStream_t st);

@@ -273,6 +273,14 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
 // follow. Integer sums:
 // the pair's shared sum is (sa + sb - sum |a - b|) / 2 over the staged
 // entries plus ACC.
+//
+// PARTS > 1: the tile is shared by PARTS waves of a workgroup; wave `part`
+// takes windows part, part + PARTS, ... (rank bases from the light prefixes
+// PLA, PLB: light entries of each block before a word), adds into the
+// workgroup's ACC, which the caller zeroes before and adds after (with
+// barriers), and returns only its staged rows' shared sums, exact on their
+// own (|a - b| and a + b have the same parity). PARTS == 1: the wave zeroes
+// ACC and returns the tile's shared sums.
 struct SplitCounts {
     // tile totals: entries, sparse entries and their pairs with mass, half
     // entries and the pairs they evaluate (32 per light sample)
@@ -287,20 +295,23 @@ constexpr int kAccStride = 33;
 constexpr int kSplitWords = 48;
 constexpr int kSplitList = kSplitWords * 32;
 
-template <int CHUNK, class Rows>
+template <int CHUNK, class Rows, int PARTS = 1>
 __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t *B0, const uint32_t *LA,
                                               const uint32_t *LB, const uint4 *RA, const uint4 *RB, int64_t nwords,
                                               int64_t zrow,
                                               const Rows &rows, const uint32_t *Y, int64_t ystride, int64_t abase,
                                               int64_t bbase, uint16_t *L, uint4 *W, uint32_t *S, uint32_t *ACC,
-                                              int lane, uint32_t (&acc)[4][4], SplitCounts &cnt) {
+                                              int lane, uint32_t (&acc)[4][4], SplitCounts &cnt, int part = 0,
+                                              const uint32_t *PLA = nullptr, const uint32_t *PLB = nullptr) {
     static_assert(CHUNK % 2 == 0, "rows are staged in pairs");
+    constexpr int64_t kStep = (int64_t)PARTS * kSplitWords;  // words from one of the wave's windows to the next
     const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0;
-    for (int k = lane; k < 32 * kAccStride; k += 64) ACC[k] = 0u;
+    if constexpr (PARTS == 1)
+        for (int k = lane; k < 32 * kAccStride; k += 64) ACC[k] = 0u;
     // the lane's discard word for the sparse and half loops: the staging
     // buffer, which only the staged loop after them uses
     uint32_t *const trash = S + lane;
@@ -311,15 +322,16 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
     // the four bitmap words of the lane in the next window, loaded one window
     // ahead so their latency hides behind the current window's work
     uint32_t na = 0u, nb = 0u, nla = 0u, nlb = 0u;
-    if (lane < kSplitWords && lane < nwords) {
-        na = A0[lane];
-        nb = B0[lane];
-        nla = LA[lane];
-        nlb = LB[lane];
+    const int64_t w0 = (int64_t)part * kSplitWords;
+    if (lane < kSplitWords && w0 + lane < nwords) {
+        na = A0[w0 + lane];
+        nb = B0[w0 + lane];
+        nla = LA[w0 + lane];
+        nlb = LB[w0 + lane];
     }
-    for (int64_t wb = 0; wb < nwords; wb += kSplitWords) {
+    for (int64_t wb = w0; wb < nwords; wb += kStep) {
         const uint32_t a = na, b = nb, la = nla, lb = nlb;
-        const int64_t wn = wb + kSplitWords + lane;
+        const int64_t wn = wb + kStep + lane;
         na = nb = nla = nlb = 0u;
         if (lane < kSplitWords && wn < nwords) {
             na = A0[wn];
@@ -347,9 +359,16 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         const int t2 = __builtin_amdgcn_readfirstlane(__shfl(i2, 63, 64));
         const int t3 = __builtin_amdgcn_readfirstlane(__shfl(i3, 63, 64));
         const int totd = t1 & 0xffff, tha = t1 >> 16, tots = t3 & 0xffff, thb = t3 >> 16, toth = tha + thb;
-        const uint32_t winA = baseA, winB = baseB;
-        baseA += (uint32_t)(t2 & 0xffff);
-        baseB += (uint32_t)(t2 >> 16);
+        uint32_t winA, winB;
+        if constexpr (PARTS == 1) {
+            winA = baseA;
+            winB = baseB;
+            baseA += (uint32_t)(t2 & 0xffff);
+            baseB += (uint32_t)(t2 >> 16);
+        } else {
+            winA = PLA[wb];
+            winB = PLB[wb];
+        }
         if (totd + toth + tots == 0) continue;
         cnt.visited += (uint32_t)(totd + toth + tots);
         cnt.sparse += (uint32_t)tots;
@@ -539,8 +558,10 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            acc[i][j] = SmOp::finish(acc[i][j], sa[i], sb[j]) + ACC[(ty * 4 + i) * kAccStride + tx * 4 + j];
+        for (int j = 0; j < 4; j++) {
+            acc[i][j] = SmOp::finish(acc[i][j], sa[i], sb[j]);
+            if constexpr (PARTS == 1) acc[i][j] += ACC[(ty * 4 + i) * kAccStride + tx * 4 + j];
+        }
 }
 
 // Similarity from a pair's shared sum: weighted, acc / (100 * 2^K)

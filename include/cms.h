@@ -82,7 +82,7 @@ typedef struct cms_opts {
     int metric;       /* CMS_METRIC_* (default WEIGHTED) */
     int bin;          /* SHAREDMIN: store samples binned by sparsity (densest first) */
     int autotune;     /* SHAREDMIN: time the persistent-kernel variants once, keep the fastest */
-    int variant;      /* SHAREDMIN: 1..12 force a persistent-kernel variant (0 = default/tuned) */
+    int variant;      /* SHAREDMIN: 1..16 force a persistent-kernel variant (0 = default/tuned) */
     int embed_slice;  /* SHAREDMIN: samples per embedding slice, multiple of 64 (0 = by a 16 GiB layout) */
     int reserved[3];
 } cms_opts;
@@ -195,8 +195,8 @@ typedef struct cms_stats {
     int nslots;                /* register slots after re-allocation */
     int scale_bits;            /* SHAREDMIN fixed-point scale (2^scale_bits per percent) */
     int entries;               /* SHAREDMIN active embedding entries */
-    int variant;               /* SHAREDMIN persistent-kernel variant in use (1..12) */
-    double tune_ms[12];        /* autotuner timings per variant (0 = not timed) */
+    int variant;               /* SHAREDMIN persistent-kernel variant in use (1..16) */
+    double tune_ms[16];        /* autotuner timings per variant (0 = not timed) */
     int64_t bands;             /* bands of the last streamed run */
     int64_t entries_visited;   /* SHAREDMIN: entries set in both blocks' bitmaps, summed over tiles */
     int64_t entries_staged;    /*   of those, accumulated for all 32 x 32 pairs of the tile */

@@ -31,7 +31,9 @@ def test_binned_and_variants_identical(cms, data, metric):
     tree, abd = data
     ref = _plain(cms, tree, abd, metric)
     visited = None
-    for variant in range(1, 13):  # 7..12 take the nearly empty entries out of the staged path
+    # 7..16 take the nearly empty entries out of the staged path; 13..16 share
+    # each tile among 2 or 4 waves
+    for variant in range(1, 17):
         with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric, bin=1, variant=variant) as ctx:
             ctx.set_tree(tree)
             ctx.set_samples_csr(abd)
@@ -59,7 +61,7 @@ def test_autotune_picks_a_variant(cms, data):
         ctx.set_samples_csr(abd)
         got = ctx.all_pairs()
         st = ctx.stats()
-        assert 1 <= st.variant <= 12
+        assert 1 <= st.variant <= 16
         assert all(t > 0 for t in st.tune_ms)
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
 
@@ -179,3 +181,21 @@ def test_autotune_on_real_tiles(cms, wide):
         lo, hi = np.minimum(a, b), np.maximum(a, b)
         np.testing.assert_array_equal(stored.view(np.uint32), ref[cms.packed_index(lo, hi, n)].view(np.uint32))
         del pi, qi, a, b, lo, hi
+
+
+def test_default_shares_few_tiles(cms, data):
+    """Without autotuning, a triangle of fewer than 4,096 block tiles runs the
+    split tiles shared by four waves (variant 14); the result is the integers
+    of one wave per tile (variant 8), bit for bit."""
+    print("default variant with few tiles", flush=True)
+    tree, abd = data
+    outs = {}
+    for variant in (0, 8):
+        with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, variant=variant) as ctx:
+            ctx.set_tree(tree)
+            ctx.set_samples_csr(abd)
+            outs[variant] = ctx.all_pairs()
+            st = ctx.stats()
+            assert st.variant == (14 if variant == 0 else 8)
+            assert st.entries_sparse > 0 and st.entries_half > 0
+    np.testing.assert_array_equal(outs[0].view(np.uint32), outs[8].view(np.uint32))
