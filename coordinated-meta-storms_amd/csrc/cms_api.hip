@@ -766,25 +766,33 @@ namespace {
 // Persistent-kernel variant: forced by opts.variant, else the default (split
 // tiles shared by four waves when the whole triangle has fewer than
 // kSmSharedTileMaxTiles tiles, one wave per tile otherwise), or the fastest
-// of all variants when opts.autotune is set. Tiles [0, ntiles) are
-// the range the caller is about to compute into `out` (out_mode as in
-// launch_sm_persistent, out_base 0; with out_mode 1 the caller passes only
-// tiles whose packed stored-position indices fit `out`).
+// of all variants when opts.autotune is set.
 //
-// With at least 12 segments of 8,192 tiles (about one per resident wave, so
-// each launch runs at full occupancy) the tuning launches are real work:
-// every variant computes two consecutive segments from the start of the
-// range, in forward then reverse variant order (binned tiles get sparser
-// along the range, and the two passes cancel that to first order), timed and
-// normalised by the rows they visited. The results stay in `out` and
-// *done = the number of leading tiles already computed; they are counted in
-// c->d_counter, which the caller zeroes before calling. Smaller ranges are
-// timed on three segments at the start, middle and end (results discarded,
-// *done = 0).
+// The tuner times every candidate on the same segments of consecutive tiles
+// spread over the whole triangle (the centres of equal strata): binned tiles
+// get sparser along the triangle, so segments from its start alone mislead.
+// A segment holds about one tile per wave slot, which the variants with one
+// wave per tile finish in one round of tiles while those that share tiles
+// among waves finish it in several: short segments favour the latter, which
+// lose over a long triangle. So the candidates are one family, the one the
+// default picks by the triangle's size: tiles shared by waves below
+// kSmSharedTileMaxTiles tiles, one wave per tile above.
+//
+// Real work (out_mode 0, `out` the whole packed triangle, and at least two
+// segments per candidate): 2 x candidates segments at evenly spaced
+// positions, candidate k timing segments k and 2 nc - 1 - k (one from each
+// half of the triangle), its time normalised by the rows it visited. The
+// tiles stay computed: *done receives the computed ranges [t0, t1) in
+// ascending order, and their rows are counted in c->d_counter, which the
+// caller zeroes before. Otherwise the timings are discarded (*done empty):
+// three segments per candidate at the centres of three strata, the total
+// time compared; with out_mode 1 (a band buffer) each segment's results are
+// written from the start of `out` (its first packed index as out_base), and
+// a segment whose rows do not fit `out` is shortened.
 int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out_cap, int out_mode,
-                 hipStream_t st, int64_t *done) {
+                 hipStream_t st, std::vector<std::pair<int64_t, int64_t>> *done) {
+    done->clear();
     const int64_t all_tiles = nb * (nb + 1) / 2;  // the whole triangle (the default depends on it)
-    *done = 0;
     if (!c->d_work) HIPCALL(c, dalloc(c->d_work, kWorkWords));
     if (c->grid_waves == 0) {
         int cus = 0;
@@ -811,38 +819,62 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
                 }
         return CMS_OK;
     }
-    constexpr int64_t kSeg = 8192;
+    const int64_t n = c->n;
+    auto row_start = [n](int64_t r) { return r * n - r * (r + 1) / 2; };
+    // packed-index range [lo, hi) of the stored-position pairs of tiles [t0, t1)
+    auto span = [&](int64_t t0, int64_t t1, int64_t &lo, int64_t &hi) {
+        int64_t ra, ca, rb, cbb;
+        host_decode_upper(t0, nb, ra, ca);
+        host_decode_upper(t1 - 1, nb, rb, cbb);
+        lo = row_start(std::min(ra * kSmBlock, n));
+        hi = row_start(std::min((rb + 1) * kSmBlock, n));
+    };
     auto timed = [&](int v, int64_t t0, int64_t len, float &ms) -> int {
+        int64_t base = 0;
+        if (out_mode == 1) {
+            int64_t lo, hi;
+            span(t0, t0 + len, lo, hi);
+            base = lo;
+        }
         HIPCALL(c, hipEventRecord(c->evp0, st));
         HIPCALL(c, launch_zero_counter(c->d_work, st));
-        HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, t0, len, c->d_work, out, out_cap, out_mode, 0,
+        HIPCALL(c, launch_sm_persistent(sm_data(c), v, nb, t0, len, c->d_work, out, out_cap, out_mode, base,
                                         c->grid_waves, st));
         HIPCALL(c, hipEventRecord(c->evp1, st));
         HIPCALL(c, hipEventSynchronize(c->evp1));
         HIPCALL(c, hipEventElapsedTime(&ms, c->evp0, c->evp1));
         return CMS_OK;
     };
+    // tiles per segment: about one per wave slot (grid_waves / 2, 16 per CU)
+    const int64_t kSeg = std::max<int64_t>(1024, (int64_t)c->grid_waves / 2);
     float best = 1e30f;
     int best_v = -1;
     std::vector<int> cand;
+    const bool shared = all_tiles < kSmSharedTileMaxTiles;
     for (int v = 0; v < kSmVariants; v++)
-        if (allowed(v)) cand.push_back(v);
+        if (allowed(v) && (kSmVariantTable[v].parts > 1) == shared) cand.push_back(v);
+    if (cand.empty())
+        for (int v = 0; v < kSmVariants; v++)
+            if (allowed(v)) cand.push_back(v);
     const int nc = (int)cand.size();
     for (int v = 0; v < kSmVariants; v++) c->stats.tune_ms[v] = 0.0;
-    if (ntiles >= 2 * (int64_t)nc * kSeg) {
+    if (out_mode == 0 && ntiles == all_tiles && ntiles >= 2 * (int64_t)nc * kSeg) {
         std::vector<double> rows(kSmVariants, 0.0);
         std::vector<float> msum(kSmVariants, 0.0f);
         unsigned long long before = 0, after = 0;
         HIPCALL(c, hipMemcpy(&before, c->d_counter, sizeof before, hipMemcpyDeviceToHost));
-        for (int k = 0; k < 2 * nc; k++) {
-            const int v = cand[k < nc ? k : 2 * nc - 1 - k];
+        const int nseg = 2 * nc;
+        for (int k = 0; k < nseg; k++) {
+            const int v = cand[k < nc ? k : nseg - 1 - k];
+            const int64_t t0 = (ntiles - kSeg) * k / (nseg - 1);
             float ms = 0;
-            int rc = timed(v, (int64_t)k * kSeg, kSeg, ms);
+            int rc = timed(v, t0, kSeg, ms);
             if (rc) return rc;
             HIPCALL(c, hipMemcpy(&after, c->d_counter, sizeof after, hipMemcpyDeviceToHost));
             msum[v] += ms;
             rows[v] += (double)(after - before);
             before = after;
+            done->push_back({t0, t0 + kSeg});
         }
         for (int v : cand) {
             c->stats.tune_ms[v] = msum[v];
@@ -852,20 +884,30 @@ int pick_variant(cms_ctx *c, int64_t nb, int64_t ntiles, float *out, int64_t out
                 best_v = v;
             }
         }
-        *done = 2 * (int64_t)nc * kSeg;
     } else {
-        // three segments at the start, middle and end of the range: with
-        // binned samples the first tiles are the densest, the last the
-        // sparsest; one timing per variant (the instances differ by a few
-        // per cent)
-        const int nseg = ntiles >= 3 * kSeg ? 3 : 1;
-        const int64_t seg_len = std::min<int64_t>(ntiles, kSeg);
+        const int nseg = all_tiles >= 3 * kSeg ? 3 : 1;
+        std::vector<std::pair<int64_t, int64_t>> segs;
+        for (int k = 0; k < nseg; k++) {
+            const int64_t len0 = std::min<int64_t>(all_tiles, kSeg);
+            const int64_t t0 = nseg == 1 ? 0 : all_tiles * (2 * k + 1) / (2 * nseg) - len0 / 2;
+            int64_t len = len0;
+            if (out_mode == 1) {
+                // the segment's rows must fit the buffer: shorten it
+                int64_t lo, hi;
+                span(t0, t0 + len, lo, hi);
+                while (len > 1 && hi - lo > out_cap) {
+                    len /= 2;
+                    span(t0, t0 + len, lo, hi);
+                }
+                if (hi - lo > out_cap) continue;
+            }
+            segs.push_back({t0, len});
+        }
         for (int v : cand) {
             float total = 0;
-            for (int k = 0; k < nseg; k++) {
-                const int64_t t0 = nseg == 1 ? 0 : (ntiles - seg_len) * k / (nseg - 1);
+            for (const auto &sg : segs) {
                 float ms = 0;
-                int rc = timed(v, t0, seg_len, ms);
+                int rc = timed(v, sg.first, sg.second, ms);
                 if (rc) return rc;
                 total += ms;
             }
@@ -932,16 +974,24 @@ int cms_all_pairs_device(cms_ctx *c, float *d_sim, void *stream) {
         // the kernel time and visited rows include tuning launches that computed real tiles
         HIPCALL(c, hipEventRecord(c->ev0, st));
         HIPCALL(c, launch_zero_counters(c->d_counter, kCounters, st));
-        int64_t done = 0;
+        std::vector<std::pair<int64_t, int64_t>> done;  // tile ranges the tuner computed for real
         rc = pick_variant(c, nb, ntiles, d_sim, npairs, 0, st, &done);
         if (rc) return rc;
-        if (done == 0) {
+        if (done.empty()) {
             HIPCALL(c, hipEventRecord(c->ev0, st));  // discarded tuning launches are not counted
             HIPCALL(c, launch_zero_counters(c->d_counter, kCounters, st));
         }
-        HIPCALL(c, launch_zero_counter(c->d_work, st));
-        HIPCALL(c, launch_sm_persistent(sm_data(c), c->variant, nb, done, ntiles - done, c->d_work, d_sim, npairs, 0,
-                                        0, c->grid_waves, st));
+        // the tiles between the tuned ranges
+        int64_t t0 = 0;
+        done.push_back({ntiles, ntiles});
+        for (const auto &r : done) {
+            if (r.first > t0) {
+                HIPCALL(c, launch_zero_counter(c->d_work, st));
+                HIPCALL(c, launch_sm_persistent(sm_data(c), c->variant, nb, t0, r.first - t0, c->d_work, d_sim, npairs,
+                                                0, 0, c->grid_waves, st));
+            }
+            t0 = std::max(t0, r.second);
+        }
         c->stats.tiles = ntiles;
     } else {
         HIPCALL(c, launch_walk_all_pairs(walk_data(c), d_sim, st));
@@ -1194,21 +1244,16 @@ int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *u
         STREAMCALL(hipEventCreateWithFlags(&done_c[i], hipEventDisableTiming));
     }
     if (!c->copy_stream) STREAMCALL(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
-    // tune on the first band's tiles: their stored-position pairs are the
-    // first count[0] <= cap entries of the packed triangle; tiles the tuner
-    // computed for real are skipped by the first band's launch
-    STREAMCALL(hipEventRecord(c->ev0, c->stream));
-    STREAMCALL(launch_zero_counters(c->d_counter, kCounters, c->stream));
-    int64_t tuned = 0;
-    rc = pick_variant(c, nb, tile_start(cuts[1]), dbuf[0], cap, 1, c->stream, &tuned);
+    // choose the variant; the tuner's timings over the whole triangle are
+    // written to the first band buffer and discarded (it is filled again)
+    std::vector<std::pair<int64_t, int64_t>> tuned_ranges;
+    rc = pick_variant(c, nb, tile_start(cuts[1]), dbuf[0], cap, 1, c->stream, &tuned_ranges);
     if (rc) {
         cleanup();
         return rc;
     }
-    if (tuned == 0) {
-        STREAMCALL(hipEventRecord(c->ev0, c->stream));
-        STREAMCALL(launch_zero_counters(c->d_counter, kCounters, c->stream));
-    }
+    STREAMCALL(hipEventRecord(c->ev0, c->stream));
+    STREAMCALL(launch_zero_counters(c->d_counter, kCounters, c->stream));
     const SmData d = sm_data(c);
     const int nbands = (int)cuts.size() - 1;
     std::vector<int64_t> first(nbands), count(nbands);
@@ -1218,7 +1263,7 @@ int cms_all_pairs_stream(cms_ctx *c, int64_t band_pairs, cms_band_fn fn, void *u
             const int64_t r0 = cuts[k] * kSmBlock, r1 = std::min(cuts[k + 1] * kSmBlock, n);
             first[k] = row_start(r0);
             count[k] = row_start(r1) - first[k];
-            const int64_t t0 = tile_start(cuts[k]) + (k == 0 ? tuned : 0), t1 = tile_start(cuts[k + 1]);
+            const int64_t t0 = tile_start(cuts[k]), t1 = tile_start(cuts[k + 1]);
             if (k >= 2) STREAMCALL(hipStreamWaitEvent(c->stream, done_c[b], 0));  // device buffer free again
             if (c->opts.use_graph) {
                 // capture this band's launches and replay them as a graph; later

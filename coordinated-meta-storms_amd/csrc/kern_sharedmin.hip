@@ -766,13 +766,6 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
     return hipGetLastError();
 }
 
-// Host decode of upper-triangle block tile t (row-major over an nb x nb grid).
-static void host_decode_upper(int64_t t, int64_t nb, int64_t &rb, int64_t &cb) {
-    int64_t r = 0;
-    while ((r + 1) * nb - (r + 1) * r / 2 <= t) r++;
-    rb = r;
-    cb = r + (t - (r * nb - r * (r - 1) / 2));
-}
 
 hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
                                 unsigned long long *work, float *out, int64_t out_cap, int out_mode,
