@@ -90,7 +90,9 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("queued", C.c_int64),
                 ("active", C.c_int64), ("steps", C.c_int64), ("kernel_ms", C.c_double), ("pairs", C.c_double),
                 ("pair_entries", C.c_double), ("arena_bytes", C.c_int64), ("arena_peak", C.c_int64),
-                ("slots", C.c_int), ("slots_peak", C.c_int), ("bands", C.c_int), ("uptime_s", C.c_double)]
+                ("slots", C.c_int), ("slots_peak", C.c_int), ("bands", C.c_int), ("uptime_s", C.c_double),
+                ("entries_staged", C.c_double), ("entries_sparse", C.c_double), ("sparse_pairs", C.c_double),
+                ("entries_half", C.c_double), ("half_pairs", C.c_double)]
 
 
 class SynthOpts(C.Structure):

@@ -108,9 +108,13 @@ struct cms_server {
     // device
     uint8_t *arena = nullptr;
     uint32_t *QY = nullptr, *qbm = nullptr;
+    // split cross tiles: the query blocks' light bitmaps, prefixes and records
+    uint32_t *qlbm = nullptr, *qpl = nullptr;
+    uint4 *qcrl = nullptr;
     unsigned long long *qcolsum = nullptr, *work = nullptr, *counter = nullptr;
     uint32_t *part_acc = nullptr;  // split-tile partial sums, part_tiles x 1024 (kept zeroed)
     int64_t part_tiles = 0;
+    int64_t qcrs = 0;  // record slots per query block (nwords * 32)
     float **d_slot_out[kRing] = {};
     int32_t *d_blocks[kRing] = {}, *d_dirty[kRing] = {};
     uint32_t *d_masks[kRing] = {};
@@ -266,7 +270,10 @@ std::string launch_step(cms_server *s, StepLaunch &L) {
         SRV_HIP(hipMemcpyAsync(s->d_dirty[k], s->h_dirty[k], sizeof(int32_t) * L.ndirty, hipMemcpyHostToDevice, st));
         SRV_HIP(hipMemcpyAsync(s->d_masks[k], s->h_masks[k], sizeof(uint32_t) * L.ndirty, hipMemcpyHostToDevice, st));
         SRV_HIP(launch_qbm_build(s->QY, s->opts.slots, s->nrows, s->nwords, s->d_dirty[k], s->d_masks[k], L.ndirty,
-                                 s->qbm, st));
+                                 s->qbm, s->qlbm, st));
+        if (s->qlbm)
+            SRV_HIP(launch_light_records_listed(s->QY, s->opts.slots, s->d_dirty[k], s->d_masks[k], L.ndirty, s->qlbm,
+                                                s->nwords, s->qpl, s->qcrl, s->qcrs, st));
     }
     SRV_HIP(hipMemcpyAsync(s->d_slot_out[k], s->h_slot_out[k], sizeof(float *) * s->opts.slots, hipMemcpyHostToDevice,
                            st));
@@ -548,6 +555,9 @@ void free_server(cms_server *s) {
     if (s->arena) (void)hipFree(s->arena);
     if (s->QY) (void)hipFree(s->QY);
     if (s->qbm) (void)hipFree(s->qbm);
+    if (s->qlbm) (void)hipFree(s->qlbm);
+    if (s->qpl) (void)hipFree(s->qpl);
+    if (s->qcrl) (void)hipFree(s->qcrl);
     if (s->qcolsum) (void)hipFree(s->qcolsum);
     if (s->work) (void)hipFree(s->work);
     if (s->counter) (void)hipFree(s->counter);
@@ -594,7 +604,7 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     if (opts.slots % kSmBlock) return sfail(CMS_E_INVAL, "cms_server_create: slots must be a multiple of 32");
     if (opts.memory_budget < 0) return sfail(CMS_E_INVAL, "cms_server_create: negative memory budget");
     if (opts.variant < 0 || opts.variant > kSmCrossVariants)
-        return sfail(CMS_E_INVAL, "cms_server_create: bad variant (the cross kernel has variants 1..6)");
+        return sfail(CMS_E_INVAL, "cms_server_create: bad variant (the cross kernel has variants 1..8)");
     int rc = ctx_prepare(ref);
     if (rc) return rc;
     if (ref->n <= 0) return ctx_fail(ref, CMS_E_STATE, "cms_server_create: no reference samples");
@@ -629,10 +639,14 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
             return sfail(CMS_E_RANGE, "unweighted edge coefficient outside the references' fixed-point range");
         }
     }
-    // the cross kernel has the plain tiles' variants only: a context tuned to
-    // a split variant serves with the default plain one
-    s->variant = opts.variant > 0 ? opts.variant - 1
-                                  : (ref->variant >= 0 && ref->variant < kSmCrossVariants ? ref->variant : 1);
+    // default: split tiles (8-row chunk, one wave per workgroup) when the
+    // references carry light records, else the plain (16, 4)
+    const bool ref_light = d.lbm && d.crl && d.pl && d.zrow >= d.nrows;
+    s->variant = opts.variant > 0 ? opts.variant - 1 : (ref_light ? 6 : 1);
+    if (s->variant >= 6 && !ref_light) {
+        free_server(s);
+        return sfail(CMS_E_INVAL, "cms_server_create: the split cross tiles need the references' light records");
+    }
     CREATE_HIP(hipSetDevice(s->device));
     int cus = 0;
     CREATE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
@@ -647,16 +661,27 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     s->opts.memory_budget = budget;
     s->sched = std::make_unique<Scheduler>(opts.slots, budget, opts.bands);
     CREATE_HIP(hipMalloc((void **)&s->arena, (size_t)std::max<int64_t>(budget, 256)));
-    CREATE_HIP(hipMalloc((void **)&s->QY, sizeof(uint32_t) * (size_t)std::max<int64_t>(s->nrows, 1) * opts.slots));
-    CREATE_HIP(hipMemset(s->QY, 0, sizeof(uint32_t) * (size_t)std::max<int64_t>(s->nrows, 1) * opts.slots));
+    // QY: the entries' rows plus an all-zero row (row nrows, never packed),
+    // which the split tiles stage in place of missing rows as they do Y's
+    const size_t nqy = (size_t)(std::max<int64_t>(s->nrows, 0) + 1) * opts.slots;
+    CREATE_HIP(hipMalloc((void **)&s->QY, sizeof(uint32_t) * nqy));
+    CREATE_HIP(hipMemset(s->QY, 0, sizeof(uint32_t) * nqy));
     const size_t nqbm = (size_t)(opts.slots / kSmBlock) * (size_t)std::max<int64_t>(s->nwords, 1);
     CREATE_HIP(hipMalloc((void **)&s->qbm, sizeof(uint32_t) * nqbm));
     CREATE_HIP(hipMemset(s->qbm, 0, sizeof(uint32_t) * nqbm));
+    s->qcrs = std::max<int64_t>(s->nwords, 1) * 32;
+    if (s->variant >= 6) {
+        CREATE_HIP(hipMalloc((void **)&s->qlbm, sizeof(uint32_t) * nqbm));
+        CREATE_HIP(hipMemset(s->qlbm, 0, sizeof(uint32_t) * nqbm));
+        CREATE_HIP(hipMalloc((void **)&s->qpl, sizeof(uint32_t) * nqbm));
+        CREATE_HIP(hipMemset(s->qpl, 0, sizeof(uint32_t) * nqbm));
+        CREATE_HIP(hipMalloc((void **)&s->qcrl, sizeof(uint4) * (size_t)(opts.slots / kSmBlock) * (size_t)s->qcrs));
+    }
     CREATE_HIP(hipMalloc((void **)&s->qcolsum, sizeof(unsigned long long) * opts.slots));
     CREATE_HIP(hipMemset(s->qcolsum, 0, sizeof(unsigned long long) * opts.slots));
     CREATE_HIP(hipMalloc((void **)&s->work, sizeof(unsigned long long) * kWorkWords));
-    CREATE_HIP(hipMalloc((void **)&s->counter, sizeof(unsigned long long)));
-    CREATE_HIP(hipMemset(s->counter, 0, sizeof(unsigned long long)));
+    CREATE_HIP(hipMalloc((void **)&s->counter, sizeof(unsigned long long) * kCounters));
+    CREATE_HIP(hipMemset(s->counter, 0, sizeof(unsigned long long) * kCounters));
     s->part_tiles = (int64_t)(opts.slots / kSmBlock) * std::max<int64_t>(s->band_blocks, 1);
     CREATE_HIP(hipMalloc((void **)&s->part_acc, sizeof(uint32_t) * 1024 * (size_t)s->part_tiles));
     CREATE_HIP(hipMemset(s->part_acc, 0, sizeof(uint32_t) * 1024 * (size_t)s->part_tiles));
@@ -698,6 +723,15 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     x.unweighted = d.unweighted;
     x.inv_scale = d.inv_scale;
     x.counter = s->counter;
+    x.qlbm = s->qlbm;
+    x.qpl = s->qpl;
+    x.qcrl = s->qcrl;
+    x.qcrs = s->qcrs;
+    x.lbm = d.lbm;
+    x.pl = d.pl;
+    x.crl = d.crl;
+    x.crs = d.crs;
+    x.zrow = d.zrow;  // the references' zero row; QY's is the same row index
 
     s->worker = std::thread(worker_main, s);
     s->deliverer = std::thread(delivery_main, s);
@@ -842,9 +876,15 @@ int cms_server_get_stats(cms_server *s, cms_server_stats *st) {
     st->steps = s->steps;
     st->kernel_ms = s->kernel_ms;
     st->pairs = s->pairs;
-    unsigned long long visited = 0;
-    if (s->counter && hipMemcpy(&visited, s->counter, sizeof(visited), hipMemcpyDeviceToHost) == hipSuccess)
-        st->pair_entries = (double)visited * 1024.0;
+    unsigned long long cnt[kCounters] = {};
+    if (s->counter && hipMemcpy(cnt, s->counter, sizeof(cnt), hipMemcpyDeviceToHost) == hipSuccess) {
+        st->pair_entries = (double)cnt[kCntVisited] * 1024.0;
+        st->entries_staged = (double)cnt[kCntStaged];
+        st->entries_sparse = (double)cnt[kCntSparseRows];
+        st->sparse_pairs = (double)cnt[kCntSparsePairs];
+        st->entries_half = (double)cnt[kCntHalfRows];
+        st->half_pairs = (double)cnt[kCntHalfPairs];
+    }
     st->arena_bytes = s->opts.memory_budget;
     st->arena_peak = s->arena_peak;
     st->slots = s->opts.slots;

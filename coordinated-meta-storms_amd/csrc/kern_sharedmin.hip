@@ -353,22 +353,6 @@ __device__ __forceinline__ void sm_store_pair(const SmData &d, uint32_t acc, int
     }
 }
 
-// A split tile's counts (one wave's share), added once per tile: wave totals
-// over thousands of tiles would not fit 32 bits.
-__device__ __forceinline__ void sm_add_counts(const SmData &d, const SplitCounts &sc) {
-    if (!d.counter || !sc.visited) return;
-    atomicAdd(d.counter + kCntVisited, (unsigned long long)sc.visited);
-    atomicAdd(d.counter + kCntStaged, (unsigned long long)(sc.visited - sc.sparse - sc.half));
-    if (sc.sparse) {
-        atomicAdd(d.counter + kCntSparseRows, (unsigned long long)sc.sparse);
-        atomicAdd(d.counter + kCntSparsePairs, (unsigned long long)sc.pairs);
-    }
-    if (sc.half) {
-        atomicAdd(d.counter + kCntHalfRows, (unsigned long long)sc.half);
-        atomicAdd(d.counter + kCntHalfPairs, (unsigned long long)sc.half_pairs);
-    }
-}
-
 // Next tile of the persistent launches: tiles are dealt to the XCDs in runs
 // of kXcdRun consecutive tiles (run q to XCD q mod 8; workgroups are
 // dispatched to the XCDs round-robin), so the waves of one XCD work on few
@@ -432,11 +416,12 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
             rows.ok = true;
             rows.col4 = (uint32_t)((lane < 32) ? abase + lane : bbase + (lane - 32)) * 4u;
             SplitCounts sc;
+            const HeavyRows H{d.Y + abase, d.npad, d.Y + bbase, d.npad};
             sm_tile_split<CHUNK, SquareRows<false>>(d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords,
                                                     d.lbm + cb * d.nwords, d.crl + ra * d.crs, d.crl + cb * d.crs,
-                                                    d.nwords, d.zrow, rows, d.Y, d.npad, abase, bbase, s_list[wave],
-                                                    s_words[wave], s_stage[wave], s_acc[wave], lane, acc, sc);
-            if (lane == 0) sm_add_counts(d, sc);
+                                                    d.nwords, d.zrow, rows, H, s_list[wave], s_words[wave],
+                                                    s_stage[wave], s_acc[wave], lane, acc, sc);
+            if (lane == 0) split_add_counts(d.counter, sc);
         } else {
             const uint32_t visited = sm_square_tile<CHUNK, false>(d, d.bm + ra * d.nwords, nullptr, d.bm + cb * d.nwords,
                                                                   nullptr, abase, bbase, s_list[wave], s_stage[wave],
@@ -511,11 +496,12 @@ __global__ __launch_bounds__(64 * PARTS) __attribute__((amdgpu_waves_per_eu(CHUN
         rows.col4 = (uint32_t)((lane < 32) ? abase + lane : bbase + (lane - 32)) * 4u;
         SplitCounts sc;
         uint32_t acc[4][4];
-        sm_tile_split<CHUNK, SquareRows<false>, PARTS>(
+        const HeavyRows H{d.Y + abase, d.npad, d.Y + bbase, d.npad};
+        sm_tile_split<CHUNK, SquareRows<false>, true>(
             d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords, d.lbm + cb * d.nwords,
-            d.crl + ra * d.crs, d.crl + cb * d.crs, d.nwords, d.zrow, rows, d.Y, d.npad, abase, bbase, s_list[wave],
-            s_words[wave], s_stage[wave], s_acc, lane, acc, sc, wave, d.pl + ra * d.nwords, d.pl + cb * d.nwords);
-        if (lane == 0) sm_add_counts(d, sc);
+            d.crl + ra * d.crs, d.crl + cb * d.crs, d.nwords, d.zrow, rows, H, s_list[wave], s_words[wave],
+            s_stage[wave], s_acc, lane, acc, sc, wave, PARTS, d.pl + ra * d.nwords, d.pl + cb * d.nwords);
+        if (lane == 0) split_add_counts(d.counter, sc);
         // every wave's staged sums join the sparse and half sums
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -566,10 +552,12 @@ __global__ __launch_bounds__(256) void sm_pairs_kernel(SmData d, int64_t n_pairs
 
 // ------------------------------------------------------------ light records
 
-// PL[b][w] = light entries of block b in words before w (one wave per block).
+// PL[b][w] = light entries of block b in words before w (one wave per block;
+// block b = blocks[blockIdx.x], or blockIdx.x without a list).
 __global__ __launch_bounds__(64) void light_prefix_kernel(const uint32_t *__restrict__ lbm, int64_t nwords,
+                                                          const int32_t *__restrict__ blocks,
                                                           uint32_t *__restrict__ PL) {
-    const int64_t b = blockIdx.x;
+    const int64_t b = blocks ? blocks[blockIdx.x] : blockIdx.x;
     const int lane = threadIdx.x;
     const uint32_t *Lw = lbm + b * nwords;
     uint32_t *P = PL + b * nwords;
@@ -590,15 +578,21 @@ __global__ __launch_bounds__(64) void light_prefix_kernel(const uint32_t *__rest
 
 // One wave per (block, 64-word window): the window's light entries are
 // listed in LDS, then each half wave reads one entry's 32 values of Y (one
-// 128-byte line) and writes its record {mask, values of the set bits in
-// ascending sample order, zeros after them} at the entry's rank.
-__global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__restrict__ Y, int64_t npad,
+// 128-byte line; with a mask, the columns whose bit is clear read as zero)
+// and writes its record {mask, values of the set bits in ascending sample
+// order, zeros after them} at the entry's rank. Y has row stride `stride`;
+// block b = blocks[blockIdx.x] (mask masks[blockIdx.x]), or blockIdx.x
+// without a list.
+__global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__restrict__ Y, int64_t stride,
+                                                           const int32_t *__restrict__ blocks,
+                                                           const uint32_t *__restrict__ masks,
                                                            const uint32_t *__restrict__ lbm,
                                                            const uint32_t *__restrict__ PL, int64_t nwords,
                                                            uint4 *__restrict__ crl, int64_t crs) {
     __shared__ uint16_t rows[2048];
     __shared__ uint32_t wrank[64];
-    const int64_t b = blockIdx.x, wb = (int64_t)blockIdx.y * 64;
+    const int64_t b = blocks ? blocks[blockIdx.x] : blockIdx.x, wb = (int64_t)blockIdx.y * 64;
+    const uint32_t cols = masks ? masks[blockIdx.x] : 0xffffffffu;
     const int lane = threadIdx.x;
     const int64_t w = wb + lane;
     const uint32_t bits = (w < nwords) ? lbm[b * nwords + w] : 0u;
@@ -623,7 +617,7 @@ __global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__res
         const int k = k0 + half;
         const bool in = k < tot;
         const int64_t row = wb * 32 + (in ? rows[k] : rows[k0]);
-        const uint32_t v = Y[row * npad + b * kSmBlock + col];
+        const uint32_t v = ((cols >> col) & 1u) ? Y[row * stride + b * kSmBlock + col] : 0u;
         const uint64_t bal = __ballot(in && v != 0u);
         const uint32_t mask = half ? (uint32_t)(bal >> 32) : (uint32_t)bal;
         const uint32_t below = mask & ((1u << col) - 1u);
@@ -643,11 +637,24 @@ hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t 
                                 uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st) {
     if (nblk <= 0 || nwords <= 0) return hipSuccess;
     if (crs < nwords * 32 || npad < nblk * kSmBlock) return hipErrorInvalidValue;
-    light_prefix_kernel<<<dim3((unsigned)nblk), 64, 0, st>>>(lbm, nwords, PL);
+    light_prefix_kernel<<<dim3((unsigned)nblk), 64, 0, st>>>(lbm, nwords, nullptr, PL);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    light_records_kernel<<<dim3((unsigned)nblk, (unsigned)((nwords + 63) / 64)), 64, 0, st>>>(Y, npad, lbm, PL, nwords,
-                                                                                             crl, crs);
+    light_records_kernel<<<dim3((unsigned)nblk, (unsigned)((nwords + 63) / 64)), 64, 0, st>>>(
+        Y, npad, nullptr, nullptr, lbm, PL, nwords, crl, crs);
+    return hipGetLastError();
+}
+
+hipError_t launch_light_records_listed(const uint32_t *Y, int64_t stride, const int32_t *blocks,
+                                       const uint32_t *masks, int nblocks, const uint32_t *lbm, int64_t nwords,
+                                       uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st) {
+    if (nblocks <= 0 || nwords <= 0) return hipSuccess;
+    if (crs < nwords * 32 || stride % kSmBlock) return hipErrorInvalidValue;
+    light_prefix_kernel<<<dim3((unsigned)nblocks), 64, 0, st>>>(lbm, nwords, blocks, PL);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    light_records_kernel<<<dim3((unsigned)nblocks, (unsigned)((nwords + 63) / 64)), 64, 0, st>>>(
+        Y, stride, blocks, masks, lbm, PL, nwords, crl, crs);
     return hipGetLastError();
 }
 

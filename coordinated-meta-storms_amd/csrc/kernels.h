@@ -151,7 +151,13 @@ constexpr SmVariant kSmVariantTable[kSmVariants] = {
 // light bitmaps; crs >= the most light entries of any block.
 hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t *lbm, int64_t nblk, int64_t nwords,
                                 uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st);
-constexpr int kSmCrossVariants = 6;   // the serving cross kernel: the plain tiles' variants 1..6
+// The same for the listed blocks of a matrix of row stride `stride` (a
+// multiple of 32), the columns whose bit of masks[k] is clear reading as
+// zero (the serving query matrix: inactive slots).
+hipError_t launch_light_records_listed(const uint32_t *Y, int64_t stride, const int32_t *blocks,
+                                       const uint32_t *masks, int nblocks, const uint32_t *lbm, int64_t nwords,
+                                       uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st);
+constexpr int kSmCrossVariants = 8;   // the serving cross kernel: plain tiles 1..6, split tiles 7..8
 constexpr int kSmDefaultVariant = 7;  // without autotune: split tiles, 8-entry chunk, 1 wave per workgroup
 constexpr int kSmSharedTileVariant = 13;       // without autotune, below kSmSharedTileMaxTiles tiles: split
 constexpr int64_t kSmSharedTileMaxTiles = 4096;  // tiles shared by 4 waves, 8-entry chunk
@@ -290,23 +296,35 @@ struct CrossData {
     int n_ref;
     int unweighted;
     double inv_scale;                   // 1 / (100 * 2^K)
-    unsigned long long *counter;        // optional: entries visited
+    unsigned long long *counter;        // optional: counters as SmData.counter (kCnt*)
+    // split tiles (cross variants 7, 8): light bitmaps, prefixes and records
+    // of both sides (as SmData: lbm / pl / crl, crs), and a row index that
+    // is all zero in QY and Y
+    const uint32_t *qlbm, *qpl;
+    const uint4 *qcrl;
+    int64_t qcrs;
+    const uint32_t *lbm, *pl;
+    const uint4 *crl;
+    int64_t crs;
+    int64_t zrow;
 };
 
 // Step: tiles (active query block qblocks[k]) x (reference blocks
 // [rb0, rb0 + nrb)); slot_out[slot] = the output row of the query in that slot
 // (indexed by caller's reference id), null for an inactive slot. `work` is a
-// zeroed device counter. Variants as launch_sm_persistent. When the step has
-// too few tiles to fill the device, tile rows are split over several waves
-// whose sums meet in part_acc (part_tiles x 1024 zeroed uint32, left zeroed)
-// and a second kernel writes the similarities.
+// zeroed device counter. Variants 0..5 as launch_sm_persistent's plain tiles;
+// 6 and 7 split tiles (8-row chunk, 1 or 2 waves per workgroup), which need
+// the light structures of CrossData. When the step has too few tiles to fill
+// the device, tile rows are split over several waves whose sums meet in
+// part_acc (part_tiles x 1024 zeroed uint32, left zeroed) and a second kernel
+// writes the similarities.
 hipError_t launch_sm_cross(const CrossData &x, int variant, const int32_t *qblocks, int nqa, int64_t rb0, int64_t nrb,
                            float *const *slot_out, uint32_t *part_acc, int64_t part_tiles, unsigned long long *work,
                            int grid_waves, hipStream_t st);
 // Bitmaps of the listed query blocks from their active slots (masks[k] bit j
-// = slot 32 blocks[k] + j is active).
+// = slot 32 blocks[k] + j is active), and their light bitmaps if qlbm is given.
 hipError_t launch_qbm_build(const uint32_t *QY, int64_t qstride, int64_t nrows, int64_t nwords, const int32_t *blocks,
-                            const uint32_t *masks, int nblocks, uint32_t *qbm, hipStream_t st);
+                            const uint32_t *masks, int nblocks, uint32_t *qbm, uint32_t *qlbm, hipStream_t st);
 // QY[:, slots[q]] = Ys[:, q] and qcs[slots[q]] = cs[q] for q < nq.
 hipError_t launch_pack(const uint32_t *Ys, int64_t sstride, int nq, const int32_t *slots, int64_t nrows, uint32_t *QY,
                        int64_t qstride, const unsigned long long *cs, unsigned long long *qcs, hipStream_t st);

@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "kernels.h"
+
 namespace cms {
 
 // Word 3 of a raw buffer resource on gfx9-family parts (32-bit data format,
@@ -274,18 +276,39 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
 // the pair's shared sum is (sa + sb - sum |a - b|) / 2 over the staged
 // entries plus ACC.
 //
-// PARTS > 1: the tile is shared by PARTS waves of a workgroup; wave `part`
-// takes windows part, part + PARTS, ... (rank bases from the light prefixes
-// PLA, PLB: light entries of each block before a word), adds into the
-// workgroup's ACC, which the caller zeroes before and adds after (with
-// barriers), and returns only its staged rows' shared sums, exact on their
-// own (|a - b| and a + b have the same parity). PARTS == 1: the wave zeroes
-// ACC and returns the tile's shared sums.
+// STRIDED: the tile's windows are shared by nparts waves (of a workgroup, or
+// separate work units); this one takes windows part, part + nparts, ...
+// (rank bases from the light prefixes PLA, PLB: light entries of each block
+// before a word), adds into ACC, which the caller zeroes before and reads
+// after, and returns only its staged rows' shared sums, exact on their own
+// (|a - b| and a + b have the same parity). Otherwise the wave walks every
+// window, zeroes ACC and returns the tile's shared sums.
+//
+// H locates the two blocks' rows for the half entries' heavy side: row r of
+// block A at H.A + r * H.as, of block B at H.B + r * H.bs (the all-pairs
+// tiles read both from Y; the serving tiles A from the query matrix).
 struct SplitCounts {
     // tile totals: entries, sparse entries and their pairs with mass, half
     // entries and the pairs they evaluate (32 per light sample)
     uint32_t visited = 0, sparse = 0, pairs = 0, half = 0, half_pairs = 0;
 };
+
+// A split tile's counts (one wave's share) into counter[kCnt*] (if given),
+// added once per tile: wave totals over thousands of tiles would not fit 32
+// bits.
+__device__ __forceinline__ void split_add_counts(unsigned long long *counter, const SplitCounts &sc) {
+    if (!counter || !sc.visited) return;
+    atomicAdd(counter + kCntVisited, (unsigned long long)sc.visited);
+    atomicAdd(counter + kCntStaged, (unsigned long long)(sc.visited - sc.sparse - sc.half));
+    if (sc.sparse) {
+        atomicAdd(counter + kCntSparseRows, (unsigned long long)sc.sparse);
+        atomicAdd(counter + kCntSparsePairs, (unsigned long long)sc.pairs);
+    }
+    if (sc.half) {
+        atomicAdd(counter + kCntHalfRows, (unsigned long long)sc.half);
+        atomicAdd(counter + kCntHalfPairs, (unsigned long long)sc.half_pairs);
+    }
+}
 
 constexpr int kAccStride = 33;
 // Split tiles walk the bitmaps in windows of 48 words (1,536 entries): the
@@ -295,22 +318,29 @@ constexpr int kAccStride = 33;
 constexpr int kSplitWords = 48;
 constexpr int kSplitList = kSplitWords * 32;
 
-template <int CHUNK, class Rows, int PARTS = 1>
+struct HeavyRows {
+    const uint32_t *A;  // block A's first column
+    int64_t as;         // its row stride (words)
+    const uint32_t *B;
+    int64_t bs;
+};
+
+template <int CHUNK, class Rows, bool STRIDED = false>
 __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t *B0, const uint32_t *LA,
                                               const uint32_t *LB, const uint4 *RA, const uint4 *RB, int64_t nwords,
-                                              int64_t zrow,
-                                              const Rows &rows, const uint32_t *Y, int64_t ystride, int64_t abase,
-                                              int64_t bbase, uint16_t *L, uint4 *W, uint32_t *S, uint32_t *ACC,
-                                              int lane, uint32_t (&acc)[4][4], SplitCounts &cnt, int part = 0,
+                                              int64_t zrow, const Rows &rows, const HeavyRows &H, uint16_t *L,
+                                              uint4 *W, uint32_t *S, uint32_t *ACC, int lane, uint32_t (&acc)[4][4],
+                                              SplitCounts &cnt, int part = 0, int nparts = 1,
                                               const uint32_t *PLA = nullptr, const uint32_t *PLB = nullptr) {
     static_assert(CHUNK % 2 == 0, "rows are staged in pairs");
-    constexpr int64_t kStep = (int64_t)PARTS * kSplitWords;  // words from one of the wave's windows to the next
+    // words from one of the wave's windows to the next
+    const int64_t kStep = STRIDED ? (int64_t)nparts * kSplitWords : (int64_t)kSplitWords;
     const int ty = lane >> 3, tx = lane & 7;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0;
-    if constexpr (PARTS == 1)
+    if constexpr (!STRIDED)
         for (int k = lane; k < 32 * kAccStride; k += 64) ACC[k] = 0u;
     // the lane's discard word for the sparse and half loops: the staging
     // buffer, which only the staged loop after them uses
@@ -322,7 +352,7 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
     // the four bitmap words of the lane in the next window, loaded one window
     // ahead so their latency hides behind the current window's work
     uint32_t na = 0u, nb = 0u, nla = 0u, nlb = 0u;
-    const int64_t w0 = (int64_t)part * kSplitWords;
+    const int64_t w0 = STRIDED ? (int64_t)part * kSplitWords : 0;
     if (lane < kSplitWords && w0 + lane < nwords) {
         na = A0[w0 + lane];
         nb = B0[w0 + lane];
@@ -360,7 +390,7 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         const int t3 = __builtin_amdgcn_readfirstlane(__shfl(i3, 63, 64));
         const int totd = t1 & 0xffff, tha = t1 >> 16, tots = t3 & 0xffff, thb = t3 >> 16, toth = tha + thb;
         uint32_t winA, winB;
-        if constexpr (PARTS == 1) {
+        if constexpr (!STRIDED) {
             winA = baseA;
             winB = baseB;
             baseA += (uint32_t)(t2 & 0xffff);
@@ -455,8 +485,8 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
                     const uint4 wd = W[r >> 5];
                     const uint32_t below = (1u << (r & 31u)) - 1u;
                     const uint4 q = BL ? RB[winB + wd.w + __popc(wd.y & below)] : RA[winA + wd.z + __popc(wd.x & below)];
-                    const uint4 *hrow =
-                        reinterpret_cast<const uint4 *>(Y + (wb * 32 + r) * ystride + (BL ? abase : bbase));
+                    const int64_t hr = wb * 32 + r;
+                    const uint4 *hrow = reinterpret_cast<const uint4 *>(BL ? H.A + hr * H.as : H.B + hr * H.bs);
                     uint4 h[8];  // heavy samples 4g .. 4g + 3
 #pragma unroll
                     for (int g = 0; g < 8; g++) h[g] = hrow[g];
@@ -560,7 +590,7 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             acc[i][j] = SmOp::finish(acc[i][j], sa[i], sb[j]);
-            if constexpr (PARTS == 1) acc[i][j] += ACC[(ty * 4 + i) * kAccStride + tx * 4 + j];
+            if constexpr (!STRIDED) acc[i][j] += ACC[(ty * 4 + i) * kAccStride + tx * 4 + j];
         }
 }
 

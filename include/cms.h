@@ -231,7 +231,8 @@ typedef struct cms_server_opts {
     int slots;             /* query slots resident at once, multiple of 32 (default 1024) */
     int bands;             /* reference bands per cycle (default 16) */
     int64_t memory_budget; /* bytes of the per-request arena (0 = 8 GiB or less if the device has less free) */
-    int variant;           /* cross-kernel variant 1..6 as cms_opts.variant 1..6 (0 = the context's if plain, else 2) */
+    int variant;           /* cross-kernel variant: 1..6 plain tiles as cms_opts.variant 1..6, 7..8 split tiles
+                              (8-row chunk, 1 or 2 waves per workgroup); 0 = 7 */
     int reserved[3];
 } cms_server_opts;
 
@@ -257,6 +258,9 @@ typedef struct cms_server_stats {
     int64_t arena_bytes, arena_peak;
     int slots, slots_peak, bands;
     double uptime_s;
+    double entries_staged;         /* of the visited entries: staged for all 32 x 32 pairs */
+    double entries_sparse, sparse_pairs;  /* split cross tiles: light in both blocks, pairs with mass there */
+    double entries_half, half_pairs;      /*   light in one block, pairs evaluated there */
 } cms_server_stats;
 
 void cms_server_opts_default(cms_server_opts *o);

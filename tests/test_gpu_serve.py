@@ -65,6 +65,28 @@ def test_serve_matches_fp64_oracle(cms, orc, data, metric, bin_):
     assert st.completed == 1 and st.steps >= 3
 
 
+@pytest.mark.parametrize("metric", [0, 1])
+def test_cross_variants_identical(cms, data, metric):
+    """Every cross-kernel variant (plain tiles 1..6, split tiles 7..8, which
+    read the light entries of the query blocks and the references from
+    records) gives the same integers: bit-identical outputs. Two query sets
+    share the slots, so some query blocks hold inactive slots with stale
+    data; bands=3 and 64 slots leave few tiles per step, so tile rows are
+    split over several waves."""
+    print(f"serve cross variants metric={metric}", flush=True)
+    tree, refs, qs = data
+    outs = {}
+    with _ctx(cms, tree, refs, metric) as ctx:
+        for variant in range(1, 9):
+            with cms.Server(ctx, slots=64, bands=3, variant=variant) as srv:
+                a = srv.submit(qs.subset(np.arange(45)))
+                b = srv.submit(qs.subset(np.arange(45, 70)))
+                outs[variant] = (a.wait(timeout=60).copy(), b.wait(timeout=60).copy())
+    for variant in range(2, 9):
+        for k in range(2):
+            np.testing.assert_array_equal(outs[variant][k].view(np.uint32), outs[1][k].view(np.uint32))
+
+
 def test_binned_and_unbinned_references_identical(cms, data):
     print("serve binned vs unbinned", flush=True)
     tree, refs, qs = data
