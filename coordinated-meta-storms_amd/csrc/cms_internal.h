@@ -114,6 +114,8 @@ int compile_tree(int leaf_n, int order_n, const int32_t *o1, const float *d1,
 // an integer path (exact six-digit rounding of the binary value), the rest
 // snprintf.
 void append_g6(std::string &s, double v);
+// The same text written at out (room for 32 bytes); returns its length.
+int write_g6(char *out, double v);
 // 1 when the integer path and snprintf("%.6g") disagree on v (tests).
 int g6_fast_differs(double v);
 

@@ -669,30 +669,57 @@ int cms_write_dist(const char *path, int n, const char *const *names, const floa
     // finished batch while the next one is formatted
     const int batch = 256;
     std::vector<std::string> rows[2] = {std::vector<std::string>(batch), std::vector<std::string>(batch)};
+    // the batch's entries below the diagonal, row i's at low[(i - r0) * r1 +
+    // j] for j < min(i, r1): they sit in column i of the packed rows j, so
+    // they are gathered by 64 x 64 blocks (each packed row's segment read
+    // contiguously, written transposed) instead of one strided read per entry
+    std::vector<float> low((size_t)batch * n);
     std::thread writer;
     bool write_ok = true;
     int cur = 0;
+    auto packed = [n](long a, long b) { return a * (long)n + b - (1 + a + 1) * (a + 1) / 2; };  // a < b
     for (int r0 = 0; r0 < n; r0 += batch) {
         int r1 = r0 + batch < n ? r0 + batch : n;
         std::vector<std::string> &rb = rows[cur];
+        const int nj = (r1 + 63) / 64;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int jb = 0; jb < nj; jb++) {
+            const int j0 = jb * 64, j1 = std::min(j0 + 64, r1);
+            for (int i0 = r0; i0 < r1; i0 += 64) {
+                const int i1 = std::min(i0 + 64, r1);
+                for (int j = j0; j < j1; j++) {
+                    const int ib = std::max(i0, j + 1);  // entries of packed row j at columns ib..i1-1
+                    if (ib >= i1) continue;
+                    const float *src = sim + packed(j, ib);
+                    for (int i = ib; i < i1; i++) low[(size_t)(i - r0) * r1 + j] = src[i - ib];
+                }
+            }
+        }
 #pragma omp parallel for schedule(dynamic, 1)
         for (int i = r0; i < r1; i++) {
+            // the row is written straight into its buffer: a tab and at most
+            // 13 bytes per value (%.6g of a double, e.g. -1.23457e+308), and
+            // a margin for the formatter's 32-byte scratch at the end
             std::string &s = rb[i - r0];
-            s.clear();
-            s.reserve((size_t)n * 10 + 32);
-            s += names[i];
-            for (int j = 0; j < n; j++) {
-                long ii = i <= j ? i : j, jj = i >= j ? i : j;
-                s += '\t';
-                if (ii == jj) {
-                    s += is_sim ? "1" : "0";
-                    continue;
-                }
-                long p = ii * (long)n + jj - (1 + ii + 1) * (ii + 1) / 2;
-                double v = is_sim ? (double)sim[p] : 1.0 - (double)sim[p];
-                cms::append_g6(s, v);
+            const size_t nl = strlen(names[i]);
+            s.resize(nl + (size_t)n * 14 + 40);
+            char *o = &s[0];
+            memcpy(o, names[i], nl);
+            o += nl;
+            const float *lrow = low.data() + (size_t)(i - r0) * r1;
+            for (int j = 0; j < i; j++) {
+                *o++ = '\t';
+                o += cms::write_g6(o, is_sim ? (double)lrow[j] : 1.0 - (double)lrow[j]);
             }
-            s += '\n';
+            *o++ = '\t';
+            *o++ = is_sim ? '1' : '0';
+            const float *urow = i + 1 < n ? sim + packed(i, i + 1) : nullptr;
+            for (int j = i + 1; j < n; j++) {
+                *o++ = '\t';
+                o += cms::write_g6(o, is_sim ? (double)urow[j - i - 1] : 1.0 - (double)urow[j - i - 1]);
+            }
+            *o++ = '\n';
+            s.resize((size_t)(o - s.data()));
         }
         if (writer.joinable()) writer.join();
         writer = std::thread([&rb, r1, r0, f, &write_ok]() {

@@ -141,10 +141,15 @@ bool fast_g6(double v, char *out, int &len) {
 
 }  // namespace
 
+int write_g6(char *out, double v) {
+    int n = 0;
+    if (!fast_g6(v, out, n)) n = snprintf(out, 32, "%.6g", v);
+    return n;
+}
+
 void append_g6(std::string &s, double v) {
     char buf[64];
-    int n = 0;
-    if (!fast_g6(v, buf, n)) n = snprintf(buf, sizeof buf, "%.6g", v);
+    const int n = write_g6(buf, v);
     s.append(buf, (size_t)n);
 }
 
