@@ -78,11 +78,9 @@ __global__ void sample_mass_kernel(const int64_t *__restrict__ rowptr, const int
 //                     from xroot[segment][sample].
 // Each node writes Y[row][s] for its active child edges and ORs the
 // per-32-sample-block bits of those rows into the bitmaps.
-#ifndef CMS_EMBED_NODE_CHUNK
-#define CMS_EMBED_NODE_CHUNK 8  // node records (and 2x leaf positions) staged in LDS per step
-#endif
-constexpr int kEmbedNodeChunk = CMS_EMBED_NODE_CHUNK;
+constexpr int kEmbedNodeChunk = 8;  // node records (and 2x leaf positions) staged in LDS per step
 constexpr int kEmbedLeafChunk = 2 * kEmbedNodeChunk;
+static_assert(kEmbedLeafChunk % 8 == 0, "leaf values are staged in passes of 8");
 
 struct BitmapWriter {
     uint32_t bits0 = 0, bits1 = 0;    // entries with mass in the wave's first / second block
@@ -113,6 +111,63 @@ struct BitmapWriter {
     }
 };
 
+// The same bitmaps with the per-entry work in vector registers: each lane
+// keeps the bits of the current word where its own sample has mass (two
+// vector ops per entry, no scalar ones), and at a word change the 32 lanes
+// of each block sum their bits into per-bit counters saturating at 4 (bit
+// planes c0, c1 and c2 = "4 or more", five shuffle-and-add steps), which
+// give the block's word (count > 0) and its light word (count 1 to 3). Used
+// where entries come in runs of consecutive rows (a subtree segment): there
+// the scalar writer's ballot, popcounts and selects per entry kept the
+// scalar unit busier than all the vector work (3.5x the scalar
+// instructions of this form).
+static_assert(kSmRecVals == 3, "the light counters saturate at 4");
+// Value of lane (lane ^ X) within each 32-lane half: ds_swizzle in bit-mask
+// mode (and-mask 0x1f, xor-mask X), no address register.
+template <int X>
+__device__ __forceinline__ uint32_t swizzle_xor(uint32_t v) {
+    static_assert(X > 0 && X < 32, "xor within 32 lanes");
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (X << 10) | 0x1f);
+}
+// Counters of lanes lane and lane ^ X added (saturating at 4).
+template <int X>
+__device__ __forceinline__ void count_step(uint32_t &c0, uint32_t &c1, uint32_t &c2) {
+    const uint32_t b0 = swizzle_xor<X>(c0), b1 = swizzle_xor<X>(c1), b2 = swizzle_xor<X>(c2);
+    const uint32_t k0 = c0 & b0, x1 = c1 ^ b1;
+    c2 = c2 | b2 | (c1 & b1) | (k0 & x1);
+    c1 = x1 ^ k0;
+    c0 ^= b0;
+}
+struct LaneBitmapWriter {
+    uint32_t m = 0;    // this lane's sample has mass at entry cur * 32 + bit
+    int64_t cur = -1;  // current bitmap word (wave-uniform)
+    __device__ __forceinline__ void flush(uint32_t *bm, uint32_t *lbm, int64_t blk0, int64_t nwords, int lane) {
+        if (cur < 0) return;
+        uint32_t c0 = m, c1 = 0, c2 = 0;
+        count_step<1>(c0, c1, c2);
+        count_step<2>(c0, c1, c2);
+        count_step<4>(c0, c1, c2);
+        count_step<8>(c0, c1, c2);
+        count_step<16>(c0, c1, c2);
+        if ((lane & 31) == 0) {
+            const int64_t w = (blk0 + (lane >> 5)) * nwords + cur;
+            const uint32_t any = c0 | c1 | c2, light = (c0 | c1) & ~c2;
+            if (any) atomicOr(bm + w, any);
+            if (lbm && light) atomicOr(lbm + w, light);
+        }
+        m = 0;
+    }
+    __device__ __forceinline__ void add(int32_t row, uint32_t q, uint32_t *bm, uint32_t *lbm, int64_t blk0,
+                                        int64_t nwords, int lane) {
+        const int64_t w = row >> 5;
+        if (w != cur) {
+            flush(bm, lbm, blk0, nwords, lane);
+            cur = w;
+        }
+        m |= min(q, 1u) << (row & 31);
+    }
+};
+
 // UNW: unweighted form (comp.h:378-446): x is presence (leaf a > 0, parent =
 // either child present) and the edge coefficient is w, so the pair sum is
 // the weight shared by both samples. SUMS: only accumulate each sample's
@@ -136,7 +191,7 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
     const float *abs_ = ab + (valid ? (s >> 3) * nlp * 8 + (s & 7) : 0);
     const int64_t blk0 = (int64_t)blockIdx.x * 2;  // 32-sample blocks of this wave
     const int first = seg_first[blockIdx.y], last = seg_last[blockIdx.y];
-    BitmapWriter bw;
+    LaneBitmapWriter bw;
     unsigned long long qsum = 0;
     double ysum = 0.0;
 
@@ -147,13 +202,23 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
         const int lb = nodes[base].leafpos;
         const int le = (base + cnt < order_n) ? nodes[base + cnt].leafpos : (int)nlp;
         const int nl = le - lb;  // <= 2 * cnt
-        for (int j0 = 0; j0 < nl; j0 += 16) {
-            float v[16];
+        // leaf values in passes of 8 without per-value branches: a pass reads
+        // 8 consecutive positions (the next chunk's leaves past this chunk's
+        // last, into stage slots no node reads), clamped to the last
+        // position only at the end of the leaf stream
+        for (int j0 = 0; j0 < nl; j0 += 8) {
+            const float *pa = abs_ + (int64_t)(lb + j0) * 8;
+            float v[8];
+            if (lb + j0 + 8 <= nlp) {
 #pragma unroll
-            for (int k = 0; k < 16; k++) v[k] = (valid && j0 + k < nl) ? abs_[(int64_t)(lb + j0 + k) * 8] : 0.0f;
+                for (int k = 0; k < 8; k++) v[k] = pa[k * 8];
+            } else {
+                const int lim = (int)(nlp - 1 - lb - j0);
 #pragma unroll
-            for (int k = 0; k < 16; k++)
-                if (j0 + k < nl) stage[(j0 + k) * 64 + lane] = v[k];
+                for (int k = 0; k < 8; k++) v[k] = pa[min(k, lim) * 8];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) stage[(j0 + k) * 64 + lane] = valid ? v[k] : 0.0f;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -699,15 +764,16 @@ hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const f
 
 template <bool UNW, bool SUMS>
 static hipError_t launch_embed_t(const EmbedData &e, hipStream_t st) {
-    const size_t lds = (size_t)(e.nslots > 0 ? e.nslots : 1) * 64 * sizeof(double);
+    const int nslots = e.nslots > 0 ? e.nslots : 1;
+    const size_t lds = (size_t)nslots * 64 * sizeof(double);
     if (lds + sizeof(float) * kEmbedLeafChunk * 64 + sizeof(EmbedNode) * kEmbedNodeChunk > 65536)
         return hipErrorInvalidConfiguration;
     const unsigned waves = (unsigned)(e.npad / 64);
     const int64_t ystride = e.ystride ? e.ystride : e.npad;
     if (e.nseg > 0) {
         embed_seg_kernel<UNW, SUMS><<<dim3(waves, (unsigned)e.nseg), 64, lds, st>>>(
-            e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.nlp, e.n, e.npad, e.nwords, ystride, e.scale, e.Y,
-            e.bm, e.lbm, e.xroot, e.colsum, e.colsum_f64);
+            e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.nlp, e.n, e.npad, e.nwords, ystride, e.scale,
+            e.Y, e.bm, e.lbm, e.xroot, e.colsum, e.colsum_f64);
         hipError_t r = hipGetLastError();
         if (r != hipSuccess) return r;
     }
