@@ -252,8 +252,7 @@ int ensure_sharedmin(cms_ctx *c) {
     ed.seg_first = c->d_seg_first;
     ed.seg_last = c->d_seg_last;
     ed.nseg = (int)c->tree.seg_first.size();
-    ed.top_nodes = c->d_top_nodes;
-    ed.top_src = c->d_top_src;
+    ed.top = unw ? c->d_toprecs_u : c->d_toprecs;
     ed.ntop = (int)c->tree.top_nodes.size();
     ed.ab = (const float *)c->d_ab;
     ed.nlp = nlp;
@@ -571,8 +570,8 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_embnodes_u);
     dfree(c->d_seg_first);
     dfree(c->d_seg_last);
-    dfree(c->d_top_nodes);
-    dfree(c->d_top_src);
+    dfree(c->d_toprecs);
+    dfree(c->d_toprecs_u);
     dfree(c->d_rowptr);
     dfree(c->d_col);
     dfree(c->d_val);
@@ -682,8 +681,22 @@ int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const f
     };
     HIPCALL(c, upload_i32(c->d_seg_first, t.seg_first));
     HIPCALL(c, upload_i32(c->d_seg_last, t.seg_last));
-    HIPCALL(c, upload_i32(c->d_top_nodes, t.top_nodes));
-    HIPCALL(c, upload_i32(c->d_top_src, t.top_src));
+    {
+        // the top nodes' records, contiguous, with their children's segments
+        std::vector<EmbedNode> top(t.top_nodes.size()), top_u(t.top_nodes.size());
+        for (size_t k = 0; k < t.top_nodes.size(); k++) {
+            top[k] = emb[t.top_nodes[k]];
+            top_u[k] = emb_u[t.top_nodes[k]];
+            top[k].pad0 = top_u[k].pad0 = t.top_src[2 * k];
+            top[k].pad1 = top_u[k].pad1 = t.top_src[2 * k + 1];
+        }
+        HIPCALL(c, dalloc(c->d_toprecs, std::max<size_t>(top.size(), 1)));
+        HIPCALL(c, dalloc(c->d_toprecs_u, std::max<size_t>(top.size(), 1)));
+        if (!top.empty()) {
+            HIPCALL(c, hipMemcpy(c->d_toprecs, top.data(), sizeof(EmbedNode) * top.size(), hipMemcpyHostToDevice));
+            HIPCALL(c, hipMemcpy(c->d_toprecs_u, top_u.data(), sizeof(EmbedNode) * top.size(), hipMemcpyHostToDevice));
+        }
+    }
     if (!t.nodes.empty()) {
         HIPCALL(c, hipMemcpy(c->d_nodes, t.nodes.data(), sizeof(WalkNode) * t.nodes.size(), hipMemcpyHostToDevice));
         HIPCALL(c, hipMemcpy(c->d_chunks, t.chunks.data(), sizeof(WalkChunk) * t.chunks.size(), hipMemcpyHostToDevice));

@@ -29,7 +29,8 @@ struct cms_ctx {
     int32_t *d_leaf_lp_ptr = nullptr, *d_leaf_lp = nullptr, *d_leaf_uses = nullptr;
     EmbedNode *d_embnodes = nullptr, *d_embnodes_u = nullptr;
     int64_t nrows_u = 0;
-    int32_t *d_seg_first = nullptr, *d_seg_last = nullptr, *d_top_nodes = nullptr, *d_top_src = nullptr;
+    int32_t *d_seg_first = nullptr, *d_seg_last = nullptr;
+    EmbedNode *d_toprecs = nullptr, *d_toprecs_u = nullptr;  // records of the nodes above the segments
     int64_t nrows = 0;
 
     int n = -1;

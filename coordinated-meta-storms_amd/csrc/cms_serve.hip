@@ -99,7 +99,7 @@ struct cms_server {
 
     // resident references
     CrossData cross{};
-    const EmbedNode *embnodes = nullptr;
+    const EmbedNode *embnodes = nullptr, *toprecs = nullptr;
     int metric = CMS_METRIC_WEIGHTED;
     int64_t nrows = 0, nwords = 0, nlp = 0, nseg = 0, nrb = 0, band_blocks = 0;
     int n_ref = 0, scale_bits = 0, variant = 1, grid_waves = 0;
@@ -218,8 +218,7 @@ std::string launch_embedding(cms_server *s, ReqHost &r, int64_t scratch_off) {
     e.seg_first = c->d_seg_first;
     e.seg_last = c->d_seg_last;
     e.nseg = (int)s->nseg;
-    e.top_nodes = c->d_top_nodes;
-    e.top_src = c->d_top_src;
+    e.top = s->toprecs;
     e.ntop = (int)c->tree.top_nodes.size();
     e.ab = ab;
     e.nlp = s->nlp;
@@ -618,6 +617,7 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     s->t0 = std::chrono::steady_clock::now();
     s->metric = ref->sm_metric;
     s->embnodes = s->metric == CMS_METRIC_UNWEIGHTED ? ref->d_embnodes_u : ref->d_embnodes;
+    s->toprecs = s->metric == CMS_METRIC_UNWEIGHTED ? ref->d_toprecs_u : ref->d_toprecs;
     const SmData d = ctx_sm_data(ref);
     s->nrows = d.nrows;
     s->nwords = d.nwords;

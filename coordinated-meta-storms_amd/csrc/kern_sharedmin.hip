@@ -274,9 +274,7 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
 }
 
 template <bool UNW, bool SUMS>
-__global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restrict__ nodes,
-                                                       const int32_t *__restrict__ top_nodes,
-                                                       const int32_t *__restrict__ top_src, int ntop,
+__global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restrict__ top, int ntop,
                                                        const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
                                                        int64_t nwords, int64_t ystride, double scale,
                                                        uint32_t *__restrict__ Y,
@@ -284,6 +282,7 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
                                                        unsigned long long *__restrict__ colsum,
                                                        double *__restrict__ colsum_f64) {
     extern __shared__ double eslots[];
+    __shared__ EmbedNode tnodes[kEmbedNodeChunk];
     const int lane = threadIdx.x;
     const int64_t s = (int64_t)blockIdx.x * 64 + lane;
     const bool valid = s < n;
@@ -292,44 +291,90 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
     BitmapWriter bw;
     unsigned long long qsum = 0;
     double ysum = 0.0;
-    for (int t = 0; t < ntop; t++) {
-        const int i = top_nodes[t];
-        const EmbedNode nd = nodes[i];
-        const uint32_t code = __builtin_amdgcn_readfirstlane(nd.code);
-        int64_t lp = __builtin_amdgcn_readfirstlane(nd.leafpos);
-        const int32_t src[2] = {top_src[2 * t], top_src[2 * t + 1]};
-        double x[2];
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const bool leaf = j == 0 ? (code & kC1Leaf) : (code & kC2Leaf);
-            if (leaf) {
-                x[j] = valid ? (double)abs_[lp * 8] : 0.0;
-                if (UNW) x[j] = x[j] > 0.0 ? 1.0 : 0.0;
-                lp++;
-            } else if (src[j] >= 0) {
-                x[j] = xroot[(int64_t)src[j] * npad + s];
-            } else {
-                x[j] = eslots[((code >> (8 * j)) & 0xffu) * 64 + lane];
-            }
-        }
-        const int32_t rows[2] = {__builtin_amdgcn_readfirstlane(nd.row1), __builtin_amdgcn_readfirstlane(nd.row2)};
-        const double betas[2] = {nd.beta1, nd.beta2};
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            if (rows[j] < 0) continue;
-            if (SUMS) {
-                ysum += betas[j] * x[j];
-                continue;
-            }
-            const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
-            Y[(int64_t)rows[j] * ystride + s] = q;
-            qsum += q;
-            bw.add(rows[j], q, bm, lbm, blk0, nwords, lane);
-        }
-        if (code & kDstLive)
-            eslots[((code >> 16) & 0xffu) * 64 + lane] =
-                UNW ? fmax(x[0], x[1]) : (double)nd.w1 * x[0] + (double)nd.w2 * x[1];
+    // Chunks of kEmbedNodeChunk nodes: their records staged in LDS (the next
+    // chunk's fetched into registers while this one is walked), then the
+    // loads of every child value held in memory (leaf abundances, segment
+    // outputs) issued before the first node uses one, so a chunk waits on
+    // memory about once instead of three times per node.
+    const uint4 *recs = reinterpret_cast<const uint4 *>(top);  // 3 per record
+    uint4 r0{}, r1{}, r2{};
+    if (lane < min(kEmbedNodeChunk, ntop)) {
+        r0 = recs[3 * lane];
+        r1 = recs[3 * lane + 1];
+        r2 = recs[3 * lane + 2];
+    }
+    for (int t0 = 0; t0 < ntop; t0 += kEmbedNodeChunk) {
+        const int cnt = min(kEmbedNodeChunk, ntop - t0);
         __builtin_amdgcn_wave_barrier();
+        if (lane < cnt) {
+            uint4 *q = reinterpret_cast<uint4 *>(tnodes) + 3 * lane;
+            q[0] = r0;
+            q[1] = r1;
+            q[2] = r2;
+        }
+        if (lane < min(kEmbedNodeChunk, ntop - t0 - kEmbedNodeChunk)) {
+            const int64_t r = 3 * (int64_t)(t0 + kEmbedNodeChunk + lane);
+            r0 = recs[r];
+            r1 = recs[r + 1];
+            r2 = recs[r + 2];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double xin[kEmbedNodeChunk][2];
+#pragma unroll
+        for (int k = 0; k < kEmbedNodeChunk; k++) {
+            xin[k][0] = xin[k][1] = 0.0;
+            if (k >= cnt) continue;
+            const uint32_t code = __builtin_amdgcn_readfirstlane(tnodes[k].code);
+            int64_t lp = __builtin_amdgcn_readfirstlane(tnodes[k].leafpos);
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int32_t src = __builtin_amdgcn_readfirstlane(j == 0 ? tnodes[k].pad0 : tnodes[k].pad1);
+                if (code & (j == 0 ? kC1Leaf : kC2Leaf)) {
+                    const float a = abs_[lp * 8];
+                    xin[k][j] = valid ? (double)a : 0.0;
+                    lp++;
+                } else if (src >= 0) {
+                    xin[k][j] = xroot[(int64_t)src * npad + s];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kEmbedNodeChunk; k++) {
+            if (k >= cnt) break;
+            const uint32_t code = __builtin_amdgcn_readfirstlane(tnodes[k].code);
+            double x[2];
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int32_t src = __builtin_amdgcn_readfirstlane(j == 0 ? tnodes[k].pad0 : tnodes[k].pad1);
+                if (code & (j == 0 ? kC1Leaf : kC2Leaf)) {
+                    x[j] = UNW ? (xin[k][j] > 0.0 ? 1.0 : 0.0) : xin[k][j];
+                } else if (src >= 0) {
+                    x[j] = xin[k][j];
+                } else {
+                    x[j] = eslots[((code >> (8 * j)) & 0xffu) * 64 + lane];
+                }
+            }
+            const int32_t rows[2] = {__builtin_amdgcn_readfirstlane(tnodes[k].row1),
+                                     __builtin_amdgcn_readfirstlane(tnodes[k].row2)};
+            const double betas[2] = {tnodes[k].beta1, tnodes[k].beta2};
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                if (rows[j] < 0) continue;
+                if (SUMS) {
+                    ysum += betas[j] * x[j];
+                    continue;
+                }
+                const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
+                Y[(int64_t)rows[j] * ystride + s] = q;
+                qsum += q;
+                bw.add(rows[j], q, bm, lbm, blk0, nwords, lane);
+            }
+            if (code & kDstLive)
+                eslots[((code >> 16) & 0xffu) * 64 + lane] =
+                    UNW ? fmax(x[0], x[1]) : (double)tnodes[k].w1 * x[0] + (double)tnodes[k].w2 * x[1];
+        }
     }
     if (SUMS) {
         if (valid) atomicAdd(colsum_f64 + s, ysum);
@@ -778,7 +823,7 @@ static hipError_t launch_embed_t(const EmbedData &e, hipStream_t st) {
         if (r != hipSuccess) return r;
     }
     if (e.ntop > 0)
-        embed_top_kernel<UNW, SUMS><<<waves, 64, lds, st>>>(e.nodes, e.top_nodes, e.top_src, e.ntop, e.ab, e.nlp,
+        embed_top_kernel<UNW, SUMS><<<waves, 64, lds, st>>>(e.top, e.ntop, e.ab, e.nlp,
                                                             e.n, e.npad, e.nwords, ystride, e.scale, e.Y, e.bm,
                                                             e.lbm, e.xroot, e.colsum, e.colsum_f64);
     return hipGetLastError();

@@ -209,7 +209,10 @@ struct EmbedData {
     int nslots;
     const int32_t *seg_first, *seg_last;  // nseg subtree segments
     int nseg;
-    const int32_t *top_nodes, *top_src;   // ntop nodes above the segments
+    // records of the ntop nodes above the segments, in schedule order, each
+    // with pad0 / pad1 = the segment whose output is its child 1 / 2 (-1:
+    // the child is its leaf or register)
+    const EmbedNode *top;
     int ntop;
     const float *ab;  // F32 walk layout
     int64_t nlp;
