@@ -84,6 +84,12 @@ void drop_layouts(cms_ctx *c) {
     dfree(c->d_xroot);
     dfree(c->d_colsum);
     c->cap_xroot = c->cap_colsum = 0;
+    dfree(c->d_ll_pos);
+    dfree(c->d_ll_val);
+    dfree(c->d_ll_top);
+    dfree(c->d_ll_start);
+    dfree(c->d_ll_end);
+    c->cap_ll_pos = c->cap_ll_val = c->cap_ll_top = c->cap_ll_start = c->cap_ll_end = 0;
     c->cap_Y = c->cap_bm = c->cap_mass = 0;
     c->sm_ready = false;
     dfree(c->d_leafP);
@@ -224,18 +230,35 @@ int ensure_sharedmin(cms_ctx *c) {
     const int64_t nrows = unw ? c->nrows_u : c->nrows;
     c->npad = ((int64_t)c->n + 63) / 64 * 64;
     c->nwords = (nrows + 31) / 32;
-    // The embedding reads the F32 walk layout. Above kEmbedSliceBytes it is
-    // built and embedded in slices of stored positions (multiples of 64
-    // samples), so the transient layout stays bounded beside Y.
+    // The embedding reads the samples' leaf lists (their nonzero leaves in
+    // leaf-stream order, built here) where the leaf stream fits the list
+    // builder, else the F32 walk layout. The walk layout is built and
+    // embedded in slices of stored positions (multiples of 64 samples) above
+    // kEmbedSliceBytes, so the transient layout stays bounded beside Y;
+    // cms_opts.embed_slice forces slices on either input.
     const int64_t nlp = (int64_t)c->tree.leaf_stream.size();
+    const int nseg = (int)c->tree.seg_first.size();
+    const bool lists = c->opts.embed_input == 0 && nlp >= 1 && nlp <= kLeafListMaxPositions;
     const size_t per_sample = (size_t)std::max<int64_t>(nlp, 1) * sizeof(float);
     int64_t slice = c->npad;
     if (c->opts.embed_slice > 0)
         slice = std::max<int64_t>(64, (int64_t)c->opts.embed_slice / 64 * 64);
-    else if ((size_t)c->npad * per_sample > kEmbedSliceBytes)
+    else if (!lists && (size_t)c->npad * per_sample > kEmbedSliceBytes)
         slice = std::max<int64_t>(64, (int64_t)(kEmbedSliceBytes / per_sample) / 64 * 64);
     const int64_t nslices = c->n > 0 ? (c->n + slice - 1) / slice : 1;
-    if (nslices == 1) {
+    if (lists) {
+        const int64_t ntl = c->ntop_leaves;
+        HIPCALL(c, dreserve(c->d_ll_pos, c->cap_ll_pos, (size_t)std::max<int64_t>(c->nnz, 1)));
+        HIPCALL(c, dreserve(c->d_ll_val, c->cap_ll_val, (size_t)std::max<int64_t>(c->nnz, 1)));
+        HIPCALL(c, dreserve(c->d_ll_start, c->cap_ll_start, (size_t)std::max<int64_t>((int64_t)c->n * nseg, 1)));
+        HIPCALL(c, dreserve(c->d_ll_end, c->cap_ll_end, (size_t)std::max(c->n, 1)));
+        HIPCALL(c, dreserve(c->d_ll_top, c->cap_ll_top, (size_t)std::max<int64_t>(ntl * c->npad, 1)));
+        if (ntl > 0) HIPCALL(c, hipMemsetAsync(c->d_ll_top, 0, sizeof(float) * ntl * c->npad, c->stream));
+        HIPCALL(c, launch_leaf_lists(c->d_rowptr, c->d_col, c->d_val, binned ? c->d_perm : nullptr, c->n,
+                                     c->d_leaf_lp_ptr, c->d_leaf_lp, nlp, c->d_seg_lp, nseg, c->d_topidx,
+                                     c->d_ll_pos, c->d_ll_val, c->d_ll_start, c->d_ll_end, c->d_ll_top, c->npad,
+                                     c->stream));
+    } else if (nslices == 1) {
         rc = ensure_walk_layout(c, CMS_STORE_F32, binned);
         if (rc) return rc;
     } else {
@@ -255,6 +278,7 @@ int ensure_sharedmin(cms_ctx *c) {
     ed.top = unw ? c->d_toprecs_u : c->d_toprecs;
     ed.ntop = (int)c->tree.top_nodes.size();
     ed.ab = (const float *)c->d_ab;
+    ed.sparse = lists ? 1 : 0;
     ed.nlp = nlp;
     ed.nwords = c->nwords;
     ed.ystride = c->npad;
@@ -264,12 +288,15 @@ int ensure_sharedmin(cms_ctx *c) {
     auto embed_all = [&](bool sums_only) -> int {
         for (int64_t k = 0; k < nslices; k++) {
             const int64_t p0 = k * slice, cnt = std::min<int64_t>(slice, c->n - p0);
-            if (nslices > 1) {
+            if (nslices > 1 && !lists) {
                 int r = build_walk_slice(c, CMS_STORE_F32, binned, (int)p0, (int)cnt);
                 if (r) return r;
             }
             EmbedData e = ed;
             e.ab = (const float *)c->d_ab;
+            if (lists)
+                e.ll = LeafLists{c->d_ll_pos, c->d_ll_val, c->d_ll_start + p0 * nseg, c->d_ll_end + p0,
+                                 c->d_ll_top + p0, c->npad};
             e.n = (int)cnt;
             e.npad = (cnt + 63) / 64 * 64;
             e.sums_only = sums_only ? 1 : 0;
@@ -512,6 +539,7 @@ const char *opts_error(const cms_opts *o) {
     if (o->metric < CMS_METRIC_WEIGHTED || o->metric > CMS_METRIC_BC) return "unknown metric";
     if (o->storage < CMS_STORE_F32 || o->storage > CMS_STORE_F16) return "unknown storage";
     if (o->embed_slice < 0) return "embed_slice must be >= 0";
+    if (o->embed_input != 0 && o->embed_input != 1) return "embed_input must be 0 or 1";
     return nullptr;
 }
 }  // namespace
@@ -572,6 +600,8 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_seg_last);
     dfree(c->d_toprecs);
     dfree(c->d_toprecs_u);
+    dfree(c->d_seg_lp);
+    dfree(c->d_topidx);
     dfree(c->d_rowptr);
     dfree(c->d_col);
     dfree(c->d_val);
@@ -683,13 +713,29 @@ int cms_set_tree(cms_ctx *c, int leaf_n, int order_n, const int32_t *o1, const f
     HIPCALL(c, upload_i32(c->d_seg_last, t.seg_last));
     {
         // the top nodes' records, contiguous, with their children's segments
+        // (leaf children: their top leaf index); the segments' first leaf
+        // positions and the top leaf index of every leaf-stream position
         std::vector<EmbedNode> top(t.top_nodes.size()), top_u(t.top_nodes.size());
+        std::vector<int32_t> topidx(std::max<size_t>(t.leaf_stream.size(), 1), -1), seg_lp(t.seg_first.size());
+        int ntl = 0;
         for (size_t k = 0; k < t.top_nodes.size(); k++) {
-            top[k] = emb[t.top_nodes[k]];
-            top_u[k] = emb_u[t.top_nodes[k]];
-            top[k].pad0 = top_u[k].pad0 = t.top_src[2 * k];
-            top[k].pad1 = top_u[k].pad1 = t.top_src[2 * k + 1];
+            const int i = t.top_nodes[k];
+            top[k] = emb[i];
+            top_u[k] = emb_u[i];
+            int32_t src[2] = {t.top_src[2 * k], t.top_src[2 * k + 1]};
+            int lp = t.nodes[i].leafpos;
+            for (int j = 0; j < 2; j++)
+                if (t.nodes[i].code & (j == 0 ? kC1Leaf : kC2Leaf)) {
+                    topidx[lp++] = ntl;
+                    src[j] = ntl++;
+                }
+            top[k].pad0 = top_u[k].pad0 = src[0];
+            top[k].pad1 = top_u[k].pad1 = src[1];
         }
+        for (size_t k = 0; k < t.seg_first.size(); k++) seg_lp[k] = t.nodes[t.seg_first[k]].leafpos;
+        c->ntop_leaves = ntl;
+        HIPCALL(c, upload_i32(c->d_topidx, topidx));
+        HIPCALL(c, upload_i32(c->d_seg_lp, seg_lp));
         HIPCALL(c, dalloc(c->d_toprecs, std::max<size_t>(top.size(), 1)));
         HIPCALL(c, dalloc(c->d_toprecs_u, std::max<size_t>(top.size(), 1)));
         if (!top.empty()) {

@@ -31,6 +31,9 @@ struct cms_ctx {
     int64_t nrows_u = 0;
     int32_t *d_seg_first = nullptr, *d_seg_last = nullptr;
     EmbedNode *d_toprecs = nullptr, *d_toprecs_u = nullptr;  // records of the nodes above the segments
+    int32_t *d_seg_lp = nullptr;   // each segment's first leaf-stream position
+    int32_t *d_topidx = nullptr;   // per leaf-stream position: top leaf index, or -1
+    int ntop_leaves = 0;           // leaves read by the nodes above the segments
     int64_t nrows = 0;
 
     int n = -1;
@@ -67,6 +70,11 @@ struct cms_ctx {
     double *d_mass = nullptr;
     double *d_xroot = nullptr;
     size_t cap_xroot = 0;
+    // leaf lists of the stored samples (embedding input)
+    int32_t *d_ll_pos = nullptr;
+    float *d_ll_val = nullptr, *d_ll_top = nullptr;
+    int64_t *d_ll_start = nullptr, *d_ll_end = nullptr;
+    size_t cap_ll_pos = 0, cap_ll_val = 0, cap_ll_top = 0, cap_ll_start = 0, cap_ll_end = 0;
     size_t cap_Y = 0, cap_bm = 0, cap_mass = 0;
     int64_t npad = 0, nwords = 0;
     int scale_bits = 0;

@@ -69,6 +69,90 @@ __global__ void sample_mass_kernel(const int64_t *__restrict__ rowptr, const int
     if (threadIdx.x == 0) mass[s] = m;
 }
 
+// Leaf lists (LeafLists): one workgroup per stored sample sorts its nonzero
+// leaves by leaf-stream position with a counting sort in LDS: a bitmap of
+// its positions, the exclusive popcount prefix over the bitmap words (rank
+// of each position), then each entry written at its rank. The same ranks
+// give the segments' first entries, and the top leaves' values go to their
+// dense rows.
+constexpr int kLeafListThreads = 256;
+__global__ __launch_bounds__(kLeafListThreads) void leaf_lists_kernel(
+    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col, const float *__restrict__ val,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ leaf_lp_ptr, const int32_t *__restrict__ leaf_lp,
+    int64_t nlp, const int32_t *__restrict__ seg_lp, int nseg, const int32_t *__restrict__ topidx,
+    int32_t *__restrict__ pos, float *__restrict__ lval, int64_t *__restrict__ start, int64_t *__restrict__ end,
+    float *__restrict__ top, int64_t top_stride) {
+    extern __shared__ uint32_t lsh[];  // bits[nwp], then pre[nwp]
+    __shared__ uint32_t part[kLeafListThreads];
+    const int64_t p = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nwp = (int)((nlp + 31) / 32);
+    uint32_t *bits = lsh, *pre = lsh + nwp;
+    const int64_t s = perm ? perm[p] : p;
+    const int64_t k0 = rowptr[s], k1 = rowptr[s + 1];
+    for (int w = tid; w < nwp; w += kLeafListThreads) bits[w] = 0u;
+    __syncthreads();
+    for (int64_t k = k0 + tid; k < k1; k += kLeafListThreads) {
+        const int32_t leaf = col[k];
+        for (int32_t t = leaf_lp_ptr[leaf]; t < leaf_lp_ptr[leaf + 1]; t++) {
+            const int32_t q = leaf_lp[t];
+            atomicOr(bits + (q >> 5), 1u << (q & 31));
+        }
+    }
+    __syncthreads();
+    // exclusive prefix of the words' popcounts: each thread a contiguous run
+    const int per = (nwp + kLeafListThreads - 1) / kLeafListThreads;
+    const int w0 = min(tid * per, nwp), w1 = min(w0 + per, nwp);
+    uint32_t sum = 0;
+    for (int w = w0; w < w1; w++) sum += __popc(bits[w]);
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < kLeafListThreads; off <<= 1) {
+        const uint32_t add = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - sum;
+    for (int w = w0; w < w1; w++) {
+        pre[w] = run;
+        run += __popc(bits[w]);
+    }
+    __syncthreads();
+    const uint32_t total = part[kLeafListThreads - 1];
+    auto rank = [&](int32_t q) -> uint32_t {
+        return pre[q >> 5] + __popc(bits[q >> 5] & ((1u << (q & 31)) - 1u));
+    };
+    for (int64_t k = k0 + tid; k < k1; k += kLeafListThreads) {
+        const int32_t leaf = col[k];
+        for (int32_t t = leaf_lp_ptr[leaf]; t < leaf_lp_ptr[leaf + 1]; t++) {
+            const int32_t q = leaf_lp[t];
+            const int64_t i = k0 + rank(q);
+            pos[i] = q;
+            lval[i] = val[k];
+            const int32_t ti = topidx[q];
+            if (ti >= 0) top[(int64_t)ti * top_stride + p] = val[k];
+        }
+    }
+    for (int k = tid; k < nseg; k += kLeafListThreads) {
+        const int32_t q = seg_lp[k];
+        start[p * nseg + k] = k0 + (q < nlp ? rank(q) : total);
+    }
+    if (tid == 0) end[p] = k0 + total;
+}
+
+hipError_t launch_leaf_lists(const int64_t *rowptr, const int32_t *col, const float *val, const int32_t *perm, int n,
+                             const int32_t *leaf_lp_ptr, const int32_t *leaf_lp, int64_t nlp, const int32_t *seg_lp,
+                             int nseg, const int32_t *topidx, int32_t *pos, float *lval, int64_t *start,
+                             int64_t *end, float *top, int64_t top_stride, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (nlp < 1 || nlp > kLeafListMaxPositions) return hipErrorInvalidValue;
+    const size_t lds = 2 * sizeof(uint32_t) * (size_t)((nlp + 31) / 32);
+    leaf_lists_kernel<<<n, kLeafListThreads, lds, st>>>(rowptr, col, val, perm, leaf_lp_ptr, leaf_lp, nlp, seg_lp,
+                                                         nseg, topidx, pos, lval, start, end, top, top_stride);
+    return hipGetLastError();
+}
+
 // Per-sample embedding. Every wave handles 64 consecutive samples
 // (lane = sample) and walks part of the schedule in FP64:
 //   embed_seg_kernel  one subtree segment per wave (grid.y = segment): its
@@ -172,11 +256,16 @@ struct LaneBitmapWriter {
 // either child present) and the edge coefficient is w, so the pair sum is
 // the weight shared by both samples. SUMS: only accumulate each sample's
 // FP64 column sum of y (first pass, to choose the fixed-point scale).
-template <bool UNW, bool SUMS>
+// SPARSE: the leaf values come from the leaf lists (each lane walks its
+// sample's list with a cursor, one entry prefetched), not the F32 walk
+// layout: the layout's 4 bytes per sample and leaf, almost all zeros, are
+// neither written nor read.
+template <bool UNW, bool SUMS, bool SPARSE>
 __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restrict__ nodes,
                                                        const int32_t *__restrict__ seg_first,
                                                        const int32_t *__restrict__ seg_last, int order_n,
-                                                       const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
+                                                       const float *__restrict__ ab, LeafLists ll, int nseg,
+                                                       int64_t nlp, int n, int64_t npad,
                                                        int64_t nwords, int64_t ystride, double scale,
                                                        uint32_t *__restrict__ Y,
                                                        uint32_t *__restrict__ bm, uint32_t *__restrict__ lbm, double *__restrict__ xroot,
@@ -194,6 +283,25 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
     LaneBitmapWriter bw;
     unsigned long long qsum = 0;
     double ysum = 0.0;
+    // SPARSE: the lane's list entry c (position q0, value v0) and the next
+    // one (q1, v1); a position past the list is INT32_MAX
+    int64_t c = 0, cend = 0;
+    int32_t q0 = INT32_MAX, q1 = INT32_MAX;
+    float v0 = 0.0f, v1 = 0.0f;
+    if constexpr (SPARSE) {
+        if (valid) {
+            c = ll.start[s * nseg + blockIdx.y];
+            cend = ll.end[s];
+        }
+        if (c < cend) {
+            q0 = ll.pos[c];
+            v0 = ll.val[c];
+        }
+        if (c + 1 < cend) {
+            q1 = ll.pos[c + 1];
+            v1 = ll.val[c + 1];
+        }
+    }
 
     for (int base = first; base <= last; base += kEmbedNodeChunk) {
         const int cnt = min(kEmbedNodeChunk, last + 1 - base);
@@ -202,11 +310,28 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
         const int lb = nodes[base].leafpos;
         const int le = (base + cnt < order_n) ? nodes[base + cnt].leafpos : (int)nlp;
         const int nl = le - lb;  // <= 2 * cnt
+        if constexpr (SPARSE) {
+            // zeros, then the lane's entries in [lb, le) (positions below lb
+            // were consumed by earlier chunks)
+#pragma unroll
+            for (int k = 0; k < kEmbedLeafChunk; k++) stage[k * 64 + lane] = 0.0f;
+            while (q0 < le) {
+                stage[(q0 - lb) * 64 + lane] = v0;
+                q0 = q1;
+                v0 = v1;
+                c++;
+                q1 = INT32_MAX;
+                if (c + 1 < cend) {
+                    q1 = ll.pos[c + 1];
+                    v1 = ll.val[c + 1];
+                }
+            }
+        }
         // leaf values in passes of 8 without per-value branches: a pass reads
         // 8 consecutive positions (the next chunk's leaves past this chunk's
         // last, into stage slots no node reads), clamped to the last
         // position only at the end of the leaf stream
-        for (int j0 = 0; j0 < nl; j0 += 8) {
+        for (int j0 = 0; j0 < (SPARSE ? 0 : nl); j0 += 8) {
             const float *pa = abs_ + (int64_t)(lb + j0) * 8;
             float v[8];
             if (lb + j0 + 8 <= nlp) {
@@ -273,9 +398,10 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
     if (valid && qsum) atomicAdd(colsum + s, qsum);
 }
 
-template <bool UNW, bool SUMS>
+template <bool UNW, bool SUMS, bool SPARSE>
 __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restrict__ top, int ntop,
-                                                       const float *__restrict__ ab, int64_t nlp, int n, int64_t npad,
+                                                       const float *__restrict__ ab, LeafLists ll, int64_t nlp,
+                                                       int n, int64_t npad,
                                                        int64_t nwords, int64_t ystride, double scale,
                                                        uint32_t *__restrict__ Y,
                                                        uint32_t *__restrict__ bm, uint32_t *__restrict__ lbm, const double *__restrict__ xroot,
@@ -332,7 +458,8 @@ __global__ __launch_bounds__(64) void embed_top_kernel(const EmbedNode *__restri
             for (int j = 0; j < 2; j++) {
                 const int32_t src = __builtin_amdgcn_readfirstlane(j == 0 ? tnodes[k].pad0 : tnodes[k].pad1);
                 if (code & (j == 0 ? kC1Leaf : kC2Leaf)) {
-                    const float a = abs_[lp * 8];
+                    // SPARSE: src is the leaf's top leaf index
+                    const float a = SPARSE ? ll.top[(int64_t)src * ll.top_stride + (valid ? s : 0)] : abs_[lp * 8];
                     xin[k][j] = valid ? (double)a : 0.0;
                     lp++;
                 } else if (src >= 0) {
@@ -807,7 +934,7 @@ hipError_t launch_sample_mass(const int64_t *rowptr, const int32_t *col, const f
     return hipGetLastError();
 }
 
-template <bool UNW, bool SUMS>
+template <bool UNW, bool SUMS, bool SPARSE>
 static hipError_t launch_embed_t(const EmbedData &e, hipStream_t st) {
     const int nslots = e.nslots > 0 ? e.nslots : 1;
     const size_t lds = (size_t)nslots * 64 * sizeof(double);
@@ -816,23 +943,29 @@ static hipError_t launch_embed_t(const EmbedData &e, hipStream_t st) {
     const unsigned waves = (unsigned)(e.npad / 64);
     const int64_t ystride = e.ystride ? e.ystride : e.npad;
     if (e.nseg > 0) {
-        embed_seg_kernel<UNW, SUMS><<<dim3(waves, (unsigned)e.nseg), 64, lds, st>>>(
-            e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.nlp, e.n, e.npad, e.nwords, ystride, e.scale,
-            e.Y, e.bm, e.lbm, e.xroot, e.colsum, e.colsum_f64);
+        embed_seg_kernel<UNW, SUMS, SPARSE><<<dim3(waves, (unsigned)e.nseg), 64, lds, st>>>(
+            e.nodes, e.seg_first, e.seg_last, e.order_n, e.ab, e.ll, e.nseg, e.nlp, e.n, e.npad, e.nwords, ystride,
+            e.scale, e.Y, e.bm, e.lbm, e.xroot, e.colsum, e.colsum_f64);
         hipError_t r = hipGetLastError();
         if (r != hipSuccess) return r;
     }
     if (e.ntop > 0)
-        embed_top_kernel<UNW, SUMS><<<waves, 64, lds, st>>>(e.top, e.ntop, e.ab, e.nlp,
-                                                            e.n, e.npad, e.nwords, ystride, e.scale, e.Y, e.bm,
-                                                            e.lbm, e.xroot, e.colsum, e.colsum_f64);
+        embed_top_kernel<UNW, SUMS, SPARSE><<<waves, 64, lds, st>>>(e.top, e.ntop, e.ab, e.ll, e.nlp, e.n, e.npad,
+                                                                    e.nwords, ystride, e.scale, e.Y, e.bm, e.lbm,
+                                                                    e.xroot, e.colsum, e.colsum_f64);
     return hipGetLastError();
+}
+
+template <bool SPARSE>
+static hipError_t launch_embed_s(const EmbedData &e, hipStream_t st) {
+    if (e.unweighted)
+        return e.sums_only ? launch_embed_t<true, true, SPARSE>(e, st) : launch_embed_t<true, false, SPARSE>(e, st);
+    return e.sums_only ? launch_embed_t<false, true, SPARSE>(e, st) : launch_embed_t<false, false, SPARSE>(e, st);
 }
 
 hipError_t launch_embed(const EmbedData &e, hipStream_t st) {
     if (e.n <= 0 || e.order_n <= 0) return hipSuccess;
-    if (e.unweighted) return e.sums_only ? launch_embed_t<true, true>(e, st) : launch_embed_t<true, false>(e, st);
-    return e.sums_only ? launch_embed_t<false, true>(e, st) : launch_embed_t<false, false>(e, st);
+    return e.sparse ? launch_embed_s<true>(e, st) : launch_embed_s<false>(e, st);
 }
 
 // Persistent launch: variant (chunk, waves) chosen by the caller / autotuner.

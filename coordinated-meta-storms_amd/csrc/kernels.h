@@ -203,6 +203,31 @@ struct EmbedNode {
 };
 static_assert(sizeof(EmbedNode) == 48, "EmbedNode must be 48 bytes");
 
+// The samples' nonzero leaves in leaf-stream order, per stored sample p
+// (built by launch_leaf_lists from the sample CSR): entries pos[i] / val[i]
+// for i in [start, end[p]) with ascending positions; start[p * nseg + k] =
+// the first entry at or after segment k's first leaf position; top[t *
+// top_stride + p] = the value of top leaf t (the t-th leaf the nodes above
+// the segments read, in schedule order), zero where the sample lacks it.
+struct LeafLists {
+    const int32_t *pos;
+    const float *val;
+    const int64_t *start;
+    const int64_t *end;
+    const float *top;
+    int64_t top_stride;
+};
+// Largest leaf stream the list builder handles (two LDS words per 32
+// positions in one workgroup, 62 KB); longer streams use the F32 walk layout.
+constexpr int64_t kLeafListMaxPositions = 7936 * 32;
+// Build the lists of stored samples [0, n) (perm: stored -> caller sample, or
+// null); topidx[q] = top leaf index of stream position q, or -1; seg_lp[k] =
+// segment k's first leaf position. ll.top must be zeroed by the caller.
+hipError_t launch_leaf_lists(const int64_t *rowptr, const int32_t *col, const float *val, const int32_t *perm, int n,
+                             const int32_t *leaf_lp_ptr, const int32_t *leaf_lp, int64_t nlp, const int32_t *seg_lp,
+                             int nseg, const int32_t *topidx, int32_t *pos, float *lval, int64_t *start,
+                             int64_t *end, float *top, int64_t top_stride, hipStream_t st);
+
 struct EmbedData {
     const EmbedNode *nodes;  // pad0 = segment whose output this node is (-1: none)
     int order_n;
@@ -210,11 +235,13 @@ struct EmbedData {
     const int32_t *seg_first, *seg_last;  // nseg subtree segments
     int nseg;
     // records of the ntop nodes above the segments, in schedule order, each
-    // with pad0 / pad1 = the segment whose output is its child 1 / 2 (-1:
-    // the child is its leaf or register)
+    // with pad0 / pad1 for child 1 / 2: a leaf child's top leaf index, else
+    // the segment whose output it is (-1: a register)
     const EmbedNode *top;
     int ntop;
-    const float *ab;  // F32 walk layout
+    const float *ab;  // F32 walk layout (when sparse is 0)
+    int sparse;       // read the leaf values from ll instead of ab
+    LeafLists ll;     // offset to the first sample embedded
     int64_t nlp;
     int n;
     int64_t npad;     // samples rounded up to 64 (one wave each); xroot stride

@@ -84,7 +84,9 @@ typedef struct cms_opts {
     int autotune;     /* SHAREDMIN: time the persistent-kernel variants once, keep the fastest */
     int variant;      /* SHAREDMIN: 1..16 force a persistent-kernel variant (0 = default/tuned) */
     int embed_slice;  /* SHAREDMIN: samples per embedding slice, multiple of 64 (0 = by a 16 GiB layout) */
-    int reserved[3];
+    int embed_input;  /* SHAREDMIN embedding input: 0 = the samples' leaf lists where the leaf stream
+                         has at most 253,952 positions, else the F32 walk layout; 1 = the walk layout */
+    int reserved[2];
 } cms_opts;
 
 typedef struct cms_ctx cms_ctx;

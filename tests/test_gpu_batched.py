@@ -122,16 +122,20 @@ def test_binned_tile_and_pairs(cms, data):
 @pytest.mark.parametrize("metric", [0, 1])
 def test_sliced_embedding_identical(cms, data, metric):
     # the embedding built in slices of 64 stored positions (as for sample
-    # sets whose walk layout exceeds the slice budget) gives the same integers
+    # sets whose walk layout exceeds the slice budget), and from the F32 walk
+    # layout instead of the leaf lists (as for leaf streams too long for the
+    # list builder), gives the same integers
     print(f"sliced embedding metric={metric}", flush=True)
     tree, abd = data
     ref = _plain(cms, tree, abd, metric)
     for bin_ in (0, 1):
-        with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric, bin=bin_, embed_slice=64) as ctx:
-            ctx.set_tree(tree)
-            ctx.set_samples_csr(abd)
-            got = ctx.all_pairs()
-        np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+        for slice_, input_ in ((64, 0), (0, 1), (64, 1)):
+            with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric, bin=bin_, embed_slice=slice_,
+                             embed_input=input_) as ctx:
+                ctx.set_tree(tree)
+                ctx.set_samples_csr(abd)
+                got = ctx.all_pairs()
+            np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
 @pytest.fixture(scope="module")

@@ -315,3 +315,13 @@ def test_cms_gen_writes_sparse_table(cms, tmp_path):
     assert b.is_sparse and a.n == b.n == 9 and a.f == b.f == 200
     db = cms.Database(str(tmp_path), "G")
     np.testing.assert_array_equal(a.abundance(db)[0].view(np.uint32), b.abundance(db)[0].view(np.uint32))
+
+
+@pytest.mark.parametrize("field,value", [("embed_input", 2), ("embed_input", -1), ("embed_slice", -64),
+                                         ("variant", 17), ("chunk", 12), ("tile_a", 64)])
+def test_context_options_refused_before_any_device_call(cms, field, value):
+    # cms_create checks the options first: an out-of-range field is
+    # CMS_E_INVAL on any machine, GPU or not
+    with pytest.raises(cms.CmsError) as e:
+        cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, **{field: value})
+    assert e.value.code == cms.E_INVAL
