@@ -850,21 +850,34 @@ __global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__res
     const uint32_t rank0 = wrank[0];
     const int half = lane >> 5, col = lane & 31;
     uint4 *out = crl + b * crs;
-    for (int k0 = 0; k0 < tot; k0 += 2) {
-        const int k = k0 + half;
-        const bool in = k < tot;
-        const int64_t row = wb * 32 + (in ? rows[k] : rows[k0]);
-        const uint32_t v = ((cols >> col) & 1u) ? Y[row * stride + b * kSmBlock + col] : 0u;
-        const uint64_t bal = __ballot(in && v != 0u);
-        const uint32_t mask = half ? (uint32_t)(bal >> 32) : (uint32_t)bal;
-        const uint32_t below = mask & ((1u << col) - 1u);
-        const int r = __popc(below), n = __popc(mask);
-        if (in) {
-            uint32_t *rec = reinterpret_cast<uint32_t *>(out + rank0 + (uint32_t)k);
-            if (v != 0u && r < kSmRecVals) rec[1 + r] = v;
-            if (col == 0) {
-                rec[0] = mask;
-                for (int f = n; f < kSmRecVals; f++) rec[1 + f] = 0u;
+    const bool mine = (cols >> col) & 1u;
+    const uint32_t *yc = Y + b * kSmBlock + col;
+    // entries k0 + 2 u + half, u < kRecUnroll: the rows' loads are all
+    // issued before the first is used
+    constexpr int kRecUnroll = 8;
+    for (int k0 = 0; k0 < tot; k0 += 2 * kRecUnroll) {
+        uint32_t v[kRecUnroll];
+#pragma unroll
+        for (int u = 0; u < kRecUnroll; u++) {
+            const int k = k0 + 2 * u + half;
+            v[u] = 0u;
+            if (mine && k < tot) v[u] = yc[(wb * 32 + rows[k]) * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < kRecUnroll; u++) {
+            const int k = k0 + 2 * u + half;
+            const bool in = k < tot;
+            const uint64_t bal = __ballot(in && v[u] != 0u);
+            const uint32_t mask = half ? (uint32_t)(bal >> 32) : (uint32_t)bal;
+            const uint32_t below = mask & ((1u << col) - 1u);
+            const int r = __popc(below), n = __popc(mask);
+            if (in) {
+                uint32_t *rec = reinterpret_cast<uint32_t *>(out + rank0 + (uint32_t)k);
+                if (v[u] != 0u && r < kSmRecVals) rec[1 + r] = v[u];
+                if (col == 0) {
+                    rec[0] = mask;
+                    for (int f = n; f < kSmRecVals; f++) rec[1 + f] = 0u;
+                }
             }
         }
     }
