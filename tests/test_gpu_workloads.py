@@ -184,3 +184,43 @@ def test_workload4_serving_vs_fp64_and_mix(cms, orc, wl4):
 def _stack(a, b):
     rp = np.concatenate([a.rowptr, a.rowptr[-1] + b.rowptr[1:]]).astype(np.int64)
     return rp, np.concatenate([a.col, b.col]).astype(np.int32), np.concatenate([a.val, b.val]).astype(np.float32)
+
+
+# ------------------------------------------------------------ workload 1
+
+def test_workload1_full_triangle(cms, orc, gpu_ok):
+    """The headline configuration at its full size: 10,000 samples, all
+    49,995,000 pairs with the default shared-minimum path (binned, split
+    tiles, leaf-list embedding input). 2,000 random pairs against the FP64
+    oracle; over the whole triangle every value is finite and in [0, 1], and
+    the bits equal a second run through different code (plain tiles,
+    variant 1, and the embedding read from the F32 walk layout)."""
+    tree, cp, counts, abd = cms.workload(1)
+    del counts
+    n = abd.n
+    _flush(f"workload 1: {n} samples, nnz {abd.rowptr[-1]}")
+    t0 = time.perf_counter()
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        got = ctx.all_pairs()
+    _flush(f"default path {time.perf_counter() - t0:.1f} s")
+    assert got.shape == (n * (n - 1) // 2,)
+    assert np.isfinite(got).all()
+    assert got.min() >= 0.0 and got.max() <= 1.0
+    rng = np.random.default_rng(11)
+    ii = rng.integers(0, n, 2000)
+    jj = rng.integers(0, n, 2000)
+    keep = ii != jj
+    ii, jj = np.minimum(ii, jj)[keep], np.maximum(ii, jj)[keep]
+    exp = _oracle_pairs(cms, orc, tree, abd, ii, jj)
+    mx, rms = _errs(got[cms.packed_index(ii, jj, n)], exp)
+    _flush(f"vs FP64 oracle on {len(ii)} pairs: max {mx:.3g} rms {rms:.3g}")
+    assert mx <= TOL and rms <= RMS
+    t0 = time.perf_counter()
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, variant=1, embed_input=1) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(abd)
+        other = ctx.all_pairs()
+    _flush(f"plain tiles, walk-layout input {time.perf_counter() - t0:.1f} s")
+    np.testing.assert_array_equal(got.view(np.uint32), other.view(np.uint32))
