@@ -259,7 +259,8 @@ struct LaneBitmapWriter {
 // SPARSE: the leaf values come from the leaf lists (each lane walks its
 // sample's list with a cursor, one entry prefetched), not the F32 walk
 // layout: the layout's 4 bytes per sample and leaf, almost all zeros, are
-// neither written nor read.
+// neither written nor read; and a row's entries of a 32-sample block are
+// stored only where the block has mass (EmbedData.sparse).
 template <bool UNW, bool SUMS, bool SPARSE>
 __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restrict__ nodes,
                                                        const int32_t *__restrict__ seg_first,
@@ -377,7 +378,14 @@ __global__ __launch_bounds__(64) void embed_seg_kernel(const EmbedNode *__restri
                     continue;
                 }
                 const uint32_t q = (uint32_t)rint(betas[j] * x[j] * scale);
-                Y[(int64_t)rows[j] * ystride + s] = q;
+                if constexpr (SPARSE) {
+                    // the lanes of a block with mass in this row
+                    const uint64_t has = __ballot(q != 0u);
+                    if ((lane < 32 ? (uint32_t)has : (uint32_t)(has >> 32)) != 0u)
+                        Y[(int64_t)rows[j] * ystride + s] = q;
+                } else {
+                    Y[(int64_t)rows[j] * ystride + s] = q;
+                }
                 qsum += q;
                 bw.add(rows[j], q, bm, lbm, blk0, nwords, lane);
             }
@@ -527,6 +535,7 @@ __device__ __forceinline__ uint32_t sm_square_tile(const SmData &d, const uint32
     rows.stride = d.npad;
     rows.ok = my_s < d.npad;
     rows.col4 = (uint32_t)((!CHECK || rows.ok) ? my_s : 0) * 4u;
+    rows.own = d.bm + (rows.ok ? my_s / kSmBlock : 0) * d.nwords;
     return sm_tile_accumulate<CHUNK>(A0, A1, B0, B1, 0, d.nwords, rows, L, S, lane, acc);
 }
 

@@ -240,7 +240,13 @@ struct EmbedData {
     const EmbedNode *top;
     int ntop;
     const float *ab;  // F32 walk layout (when sparse is 0)
-    int sparse;       // read the leaf values from ll instead of ab
+    // read the leaf values from ll instead of ab; then the subtree segments
+    // also store a row's entries of a 32-sample block only where the block
+    // has mass in that row (its bitmap bit): the tiles, light records and
+    // pair lists read no others, and the rectangle tiles zero what a lane's
+    // own block lacks. With ab every entry is stored (the serving loop packs
+    // whole query columns).
+    int sparse;
     LeafLists ll;     // offset to the first sample embedded
     int64_t nlp;
     int n;

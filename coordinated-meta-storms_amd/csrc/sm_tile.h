@@ -25,16 +25,23 @@ constexpr int kRawBufferFlags = 0x00020000;
 //
 // One matrix (all-pairs tiles): lane column `col` (row block sample for lanes
 // 0..31, column block sample for lanes 32..63) of Y with row stride `stride`.
-// CHECK: a lane whose column lies outside the matrix (ok false) loads column
-// 0 and stages zero.
+// CHECK (tiles at any sample offset, whose rows are those of the union of
+// two blocks' bitmaps): a lane whose column lies outside the matrix (ok
+// false) loads column 0 and stages zero, and a lane stages zero for rows its
+// own block's bitmap (own) lacks, where the embedding stored nothing.
 template <bool CHECK>
 struct SquareRows {
     const uint32_t *Y;
     int64_t stride;
     uint32_t col4;  // column * 4 (bytes)
     bool ok;
+    const uint32_t *own = nullptr;  // CHECK: the bitmap of this lane's block
     const uint32_t *win = nullptr;
-    __device__ __forceinline__ void window(int64_t rbase) { win = Y + rbase * stride; }
+    int64_t rb = 0;
+    __device__ __forceinline__ void window(int64_t rbase) {
+        win = Y + rbase * stride;
+        rb = rbase;
+    }
     __device__ __forceinline__ uint32_t load(uint32_t r) const {
         // a raw buffer whose base is the row (scalar registers) and whose
         // offset is the lane's column: the address costs scalar
@@ -43,7 +50,11 @@ struct SquareRows {
         const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<uint32_t *>(win + (uint64_t)r * (uint64_t)stride), (short)0, (int)0xffffffff, kRawBufferFlags);
         const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(row, (int)col4, 0, 0);
-        return (!CHECK || ok) ? v : 0u;
+        if constexpr (CHECK) {
+            const int64_t e = rb + r;
+            return (ok && ((own[e >> 5] >> (e & 31)) & 1u)) ? v : 0u;
+        }
+        return v;
     }
 };
 

@@ -138,6 +138,42 @@ def test_sliced_embedding_identical(cms, data, metric):
             np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+def test_unstored_entries_never_read(cms, data):
+    """The embedding stores a row's entries of a 32-sample block only where
+    the block has mass in that row; the rest of Y keeps what the memory held.
+    Here that is a far denser sample set embedded first by the same context
+    (same sizes, so the layout's memory is reused): the all-pairs tiles, the
+    pair list and rectangles at unaligned offsets (rows from the union of
+    two blocks' bitmaps, lanes masked by their own block's) must still give
+    the bits of a fresh context."""
+    print("unstored entries", flush=True)
+    tree, abd = data
+    n = abd.n
+    ref = _plain(cms, tree, abd)
+    cp = cms.synth_copy_numbers(3003, tree.leaf_n, 15)
+    dense = cms.counts_to_abundance(cms.synth_counts(2903, n, tree.leaf_n, 15000, 19000, 1.0, 30, 0.3), cp)
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=0) as ctx:
+        ctx.set_tree(tree)
+        ctx.set_samples_csr(dense)
+        ctx.all_pairs()
+        ctx.set_samples_csr(abd)
+        got = ctx.all_pairs()
+        np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+        for r0, nr, c0, nc in ((5, 40, 17, 50), (33, 70, 1, 90), (100, 103, 61, 142)):
+            t = ctx.tile(r0, nr, c0, nc)
+            r, c = np.meshgrid(np.arange(r0, r0 + nr), np.arange(c0, c0 + nc), indexing="ij")
+            exp = np.ones((nr, nc), np.float32)
+            m = r != c
+            exp[m] = ref[cms.packed_index(np.minimum(r, c)[m], np.maximum(r, c)[m], n)]
+            np.testing.assert_array_equal(t.view(np.uint32), exp.view(np.uint32))
+        rows = np.arange(0, n, 3, dtype=np.int32)
+        cols = ((rows * 7 + 11) % n).astype(np.int32)
+        keep = rows != cols
+        got = ctx.sim_pairs(rows[keep], cols[keep])
+        lo, hi = np.minimum(rows[keep], cols[keep]), np.maximum(rows[keep], cols[keep])
+        np.testing.assert_array_equal(got.view(np.uint32), ref[cms.packed_index(lo, hi, n)].view(np.uint32))
+
+
 @pytest.fixture(scope="module")
 def wide(cms, gpu_ok):
     # 20,500 samples: 205,761 block tiles, enough for the tuner to time its
