@@ -50,6 +50,14 @@ __device__ __forceinline__ void sm_decode_upper(int64_t t, int64_t nb, int64_t &
     cb = r + (t - (r * nb - r * (r - 1) / 2));
 }
 
+// Block row ra and column cb of tile t in the order of a launch over whole
+// block rows g (grouped_tile; row-major when g.r0 < 0).
+__device__ __forceinline__ void sm_tile_at(int64_t t, int64_t nb, const TileRows &g, int64_t &ra, int64_t &cb) {
+    int64_t rb, c;
+    sm_decode_upper(t, nb, rb, c);
+    grouped_tile(t, nb, g, rb, ra, cb);
+}
+
 __device__ __forceinline__ int64_t sm_xcd_remap(int64_t b, int64_t nwg) {
     const int64_t X = 8;
     int64_t xcd = b % X, idx = b / X, per = nwg / X, rem = nwg % X;
@@ -602,8 +610,9 @@ __device__ __forceinline__ void sm_store_pair(const SmData &d, uint32_t acc, int
 // Next tile of the persistent launches: tiles are dealt to the XCDs in runs
 // of kXcdRun consecutive tiles (run q to XCD q mod 8; workgroups are
 // dispatched to the XCDs round-robin), so the waves of one XCD work on few
-// row blocks at a time and share those blocks' Y lines in its L2 (at 10,000
-// samples: L2 hits 15 % -> 40 % of requests, HBM reads -30 %). The calling
+// blocks at a time and share those blocks' Y lines in its L2 (at 10,000
+// samples, row-major runs: L2 hits 15 % -> 40 % of requests, HBM reads
+// -30 %). The calling
 // wave (every lane; src and left stay wave-uniform) drains its own XCD's
 // counter (src, initially blockIdx.x mod 8), then each other XCD's; -1 when
 // all eight are past the range (left counts the counters not yet found
@@ -622,9 +631,11 @@ __device__ __forceinline__ int64_t sm_next_tile(unsigned long long *__restrict__
 }
 
 // All pairs, persistent: a fixed grid of waves pulls upper-triangle block
-// tiles [t_begin, t_begin + t_count) from a global counter (tiles are
-// row-major over blocks; with samples binned densest-first the heaviest tiles
-// come first). Every wave exits when the counter passes t_count.
+// tiles [t_begin, t_begin + t_count) from per-XCD counters (tiles in the
+// order of sm_tile_at: bands of kTileGroup block rows when the launch holds
+// whole rows, row-major otherwise; with samples binned densest-first the
+// heaviest tiles come first either way). Every wave exits when the counters
+// pass t_count.
 //   OUT 0: write the packed upper triangle in the caller's sample order
 //          (stored position -> sample id through d.perm, if any);
 //   OUT 1: write the packed triangle of stored positions, minus out_base
@@ -633,8 +644,8 @@ __device__ __forceinline__ int64_t sm_next_tile(unsigned long long *__restrict__
 //          sm_tile_split (records d.crl), the rest are staged.
 template <int CHUNK, int WAVES, int OUT, bool SPLIT>
 __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, int64_t t_begin, int64_t t_count,
-                                                   unsigned long long *__restrict__ work, float *__restrict__ out,
-                                                   int64_t out_base) {
+                                                   TileRows g, unsigned long long *__restrict__ work,
+                                                   float *__restrict__ out, int64_t out_base) {
     // entry rows (uint16) of one window: 2,048 (split tiles: kSplitList);
     // split tiles also hold the window's word records and the accumulator
     __shared__ __align__(16) uint16_t s_list[WAVES][SPLIT ? kSplitList : 2048];
@@ -648,7 +659,7 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
         const int64_t t = sm_next_tile(work, t_count, src, left, lane);
         if (t < 0) break;
         int64_t ra, cb;
-        sm_decode_upper(t_begin + (int64_t)t, nb, ra, cb);
+        sm_tile_at(t_begin + (int64_t)t, nb, g, ra, cb);
         // block indices are wave-uniform: keep them (and every address
         // derived from them) in scalar registers
         ra = __builtin_amdgcn_readfirstlane((int)ra);
@@ -688,19 +699,19 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
 
 template <int CHUNK, int WAVES, int OUT>
 __global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int64_t nb, int64_t t_begin,
-                                                                   int64_t t_count,
+                                                                   int64_t t_count, TileRows g,
                                                                    unsigned long long *__restrict__ work,
                                                                    float *__restrict__ out, int64_t out_base) {
-    sm_persistent_body<CHUNK, WAVES, OUT, false>(d, nb, t_begin, t_count, work, out, out_base);
+    sm_persistent_body<CHUNK, WAVES, OUT, false>(d, nb, t_begin, t_count, g, work, out, out_base);
 }
 
 // The split tiles held to 4 waves per SIMD (at most 128 VGPRs): their
 // sparse entries wait on record loads, and occupancy hides that latency.
 template <int CHUNK, int WAVES, int OUT>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(CHUNK <= 8 && WAVES <= 2 ? 4 : 1))) void sm_persistent_split_kernel(
-    SmData d, int64_t nb, int64_t t_begin, int64_t t_count, unsigned long long *__restrict__ work,
+    SmData d, int64_t nb, int64_t t_begin, int64_t t_count, TileRows g, unsigned long long *__restrict__ work,
     float *__restrict__ out, int64_t out_base) {
-    sm_persistent_body<CHUNK, WAVES, OUT, true>(d, nb, t_begin, t_count, work, out, out_base);
+    sm_persistent_body<CHUNK, WAVES, OUT, true>(d, nb, t_begin, t_count, g, work, out, out_base);
 }
 
 // Split tiles with PARTS waves per tile (1024 / PARTS pairs per wave): a
@@ -711,7 +722,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(CHUN
 // SIMD) for the 8-row chunks: at 128 their registers spill.
 template <int CHUNK, int PARTS, int OUT>
 __global__ __launch_bounds__(64 * PARTS) __attribute__((amdgpu_waves_per_eu(CHUNK <= 8 ? 3 : 1))) void sm_persistent_parts_kernel(
-    SmData d, int64_t nb, int64_t t_begin, int64_t t_count, unsigned long long *__restrict__ work,
+    SmData d, int64_t nb, int64_t t_begin, int64_t t_count, TileRows g, unsigned long long *__restrict__ work,
     float *__restrict__ out, int64_t out_base) {
     __shared__ __align__(16) uint16_t s_list[PARTS][kSplitList];
     __shared__ uint32_t s_stage[PARTS][CHUNK * 64];
@@ -731,7 +742,7 @@ __global__ __launch_bounds__(64 * PARTS) __attribute__((amdgpu_waves_per_eu(CHUN
         const long long t = s_tile;
         if (t < 0) break;
         int64_t ra, cb;
-        sm_decode_upper(t_begin + (int64_t)t, nb, ra, cb);
+        sm_tile_at(t_begin + (int64_t)t, nb, g, ra, cb);
         ra = __builtin_amdgcn_readfirstlane((int)ra);
         cb = __builtin_amdgcn_readfirstlane((int)cb);
         const int64_t abase = ra * kSmBlock, bbase = cb * kSmBlock;
@@ -996,21 +1007,22 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
                                       unsigned long long *work, float *out, int64_t out_base, int grid_waves,
                                       hipStream_t st) {
     if (t_count <= 0) return hipSuccess;
+    const TileRows g = host_tile_rows(t_begin, t_count, nb);
 #define CMS_PERSIST(CH, WV, SP)                                                                                  \
     {                                                                                                            \
         const int64_t wg = std::max<int64_t>(1, std::min<int64_t>((t_count + WV - 1) / WV, grid_waves / WV));   \
         if constexpr (SP)                                                                                        \
             sm_persistent_split_kernel<CH, WV, OUT><<<dim3((unsigned)wg), 64 * WV, 0, st>>>(                     \
-                d, nb, t_begin, t_count, work, out, out_base);                                                   \
+                d, nb, t_begin, t_count, g, work, out, out_base);                                                \
         else                                                                                                     \
             sm_persistent_kernel<CH, WV, OUT><<<dim3((unsigned)wg), 64 * WV, 0, st>>>(d, nb, t_begin, t_count,   \
-                                                                                   work, out, out_base);          \
+                                                                                   g, work, out, out_base);       \
     }
 #define CMS_PARTS(CH, PT)                                                                                        \
     {                                                                                                            \
         const int64_t wg = std::max<int64_t>(1, std::min<int64_t>(t_count, grid_waves / PT));                    \
         sm_persistent_parts_kernel<CH, PT, OUT><<<dim3((unsigned)wg), 64 * PT, 0, st>>>(d, nb, t_begin, t_count,  \
-                                                                                      work, out, out_base);       \
+                                                                                      g, work, out, out_base);    \
     }
     // the split tiles read the light records and the zero row; tiles shared
     // by several waves also the light prefixes

@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "cms_internal.h"
+#include "tile_order.h"
 
 namespace cms {
 
@@ -161,22 +162,6 @@ constexpr int kSmCrossVariants = 8;   // the serving cross kernel: plain tiles 1
 constexpr int kSmDefaultVariant = 7;  // without autotune: split tiles, 8-entry chunk, 1 wave per workgroup
 constexpr int kSmSharedTileVariant = 13;       // without autotune, below kSmSharedTileMaxTiles tiles: split
 constexpr int64_t kSmSharedTileMaxTiles = 4096;  // tiles shared by 4 waves, 8-entry chunk
-// Block row rb and column cb of upper-triangle block tile t (row-major over
-// an nb x nb grid; row r's tiles start at r nb - r (r - 1) / 2), by binary
-// search over the rows.
-inline void host_decode_upper(int64_t t, int64_t nb, int64_t &rb, int64_t &cb) {
-    auto start = [nb](int64_t r) { return r * nb - r * (r - 1) / 2; };
-    int64_t lo = 0, hi = nb - 1;  // the largest r with start(r) <= t
-    while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) / 2;
-        if (start(mid) <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    rb = lo;
-    cb = lo + (t - start(lo));
-}
 hipError_t launch_sm_persistent(const SmData &d, int variant, int64_t nb, int64_t t_begin, int64_t t_count,
                                 unsigned long long *work, float *out, int64_t out_cap, int out_mode,
                                 int64_t out_base, int grid_waves, hipStream_t st);
