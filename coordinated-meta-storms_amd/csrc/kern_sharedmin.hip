@@ -822,14 +822,9 @@ __global__ __launch_bounds__(64) void light_prefix_kernel(const uint32_t *__rest
     for (int64_t w0 = 0; w0 < nwords; w0 += 64) {
         const int64_t w = w0 + lane;
         const int c = (w < nwords) ? __popc(Lw[w]) : 0;
-        int incl = c;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += v;
-        }
+        const int incl = wave_incl_scan(c);
         if (w < nwords) P[w] = base + (uint32_t)(incl - c);
-        base += (uint32_t)__builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+        base += (uint32_t)__builtin_amdgcn_readlane(incl, 63);
     }
 }
 
@@ -854,13 +849,8 @@ __global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__res
     const int64_t w = wb + lane;
     const uint32_t bits = (w < nwords) ? lbm[b * nwords + w] : 0u;
     const int c = __popc(bits);
-    int incl = c;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
-    }
-    const int tot = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    const int incl = wave_incl_scan(c);
+    const int tot = __builtin_amdgcn_readlane(incl, 63);
     if (tot == 0) return;
     int pos = incl - c;
     // rank of the window's first light entry: the block prefix at its first word

@@ -85,6 +85,20 @@ struct CrossRows {
     }
 };
 
+// Inclusive prefix sum over the 64 lanes of a wave through DPP moves (row
+// shifts by 1, 2, 4 and 8 within each row of 16 lanes, then lane 15 of each
+// row added to the next row and lane 31 to the upper half): VALU work and no
+// LDS traffic, where a shuffle-based scan takes an LDS permute per step.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1 and 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
 // Per-entry pair operation of the tiles: acc += f(a0, b0) + f(a1, b1) for
 // two staged rows. kColSum: the tile also sums each lane's own column over
 // the visited rows and hands both to finish().
@@ -158,13 +172,8 @@ __device__ __forceinline__ uint32_t sm_tile_accumulate(const uint32_t *A0, const
             w = a & b;
         }
         const int cnt = __popc(w);
-        int incl = cnt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            int v = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += v;
-        }
-        const int tot = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+        const int incl = wave_incl_scan(cnt);
+        const int tot = __builtin_amdgcn_readlane(incl, 63);
         if (tot == 0) continue;
         visited += (uint32_t)tot;
         int pos = incl - cnt;
@@ -386,19 +395,10 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         // light in A << 16), (light A | light B << 16), (sparse | half light in B << 16)
         const int c1 = __popc(dw) | (__popc(hwa) << 16), c2 = __popc(la) | (__popc(lb) << 16),
                   c3 = __popc(sw) | (__popc(hwb) << 16);
-        int i1 = c1, i2 = c2, i3 = c3;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int v1 = __shfl_up(i1, off, 64), v2 = __shfl_up(i2, off, 64), v3 = __shfl_up(i3, off, 64);
-            if (lane >= off) {
-                i1 += v1;
-                i2 += v2;
-                i3 += v3;
-            }
-        }
-        const int t1 = __builtin_amdgcn_readfirstlane(__shfl(i1, 63, 64));
-        const int t2 = __builtin_amdgcn_readfirstlane(__shfl(i2, 63, 64));
-        const int t3 = __builtin_amdgcn_readfirstlane(__shfl(i3, 63, 64));
+        const int i1 = wave_incl_scan(c1), i2 = wave_incl_scan(c2), i3 = wave_incl_scan(c3);
+        const int t1 = __builtin_amdgcn_readlane(i1, 63);
+        const int t2 = __builtin_amdgcn_readlane(i2, 63);
+        const int t3 = __builtin_amdgcn_readlane(i3, 63);
         const int totd = t1 & 0xffff, tha = t1 >> 16, tots = t3 & 0xffff, thb = t3 >> 16, toth = tha + thb;
         uint32_t winA, winB;
         if constexpr (!STRIDED) {
@@ -504,7 +504,7 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
                     const uint32_t vl[3] = {q.y, q.z, q.w};  // zero past the record's samples
                     const int nl = __popc(q.x);
                     // the record's light samples; slots past nl repeat the
-                    // first (their values are zero, so their minima add 0)
+                    // first and are masked off below
                     uint32_t ml = q.x, il[3];
                     il[0] = (uint32_t)(__ffs(ml) - 1);
                     ml &= ml - 1u;
@@ -520,18 +520,20 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
 #pragma unroll
                     for (int t = 0; t < 3; t++) ci[t] = BL ? il[t] : il[t] * kAccStride;
                     asm volatile("" : "+v"(ci[0]), "+v"(ci[1]), "+v"(ci[2]));
+                    const bool l1 = nl > 1, l2 = nl > 2;
                     // per heavy sample with mass: its minimum with each light
                     // sample; lanes whose heavy value is zero sit out (exec
-                    // mask), which also keeps them off cells other lanes add to
+                    // mask), which also keeps them off cells other lanes add
+                    // to, and so do lanes past their record's light samples
 #pragma unroll
                     for (int g = 0; g < 8; g++) {
                         const uint32_t hq[4] = {h[g].x, h[g].y, h[g].z, h[g].w};
 #pragma unroll
                         for (int e = 0; e < 4; e++)
                             if (hq[e]) {
-#pragma unroll
-                                for (int t = 0; t < 3; t++)
-                                    atomicAdd(ACC + ci[t] + (g * 4 + e) * kStep, min(vl[t], hq[e]));
+                                atomicAdd(ACC + ci[0] + (g * 4 + e) * kStep, min(vl[0], hq[e]));
+                                if (l1) atomicAdd(ACC + ci[1] + (g * 4 + e) * kStep, min(vl[1], hq[e]));
+                                if (l2) atomicAdd(ACC + ci[2] + (g * 4 + e) * kStep, min(vl[2], hq[e]));
                             }
                     }
                     hp_lane += 32u * (uint32_t)nl;
