@@ -92,6 +92,7 @@ void drop_layouts(cms_ctx *c) {
     c->cap_ll_pos = c->cap_ll_val = c->cap_ll_top = c->cap_ll_start = c->cap_ll_end = 0;
     c->cap_Y = c->cap_bm = c->cap_mass = 0;
     c->sm_ready = false;
+    c->nrec_known = false;
     dfree(c->d_leafP);
     dfree(c->d_leafbm);
     dfree(c->d_leafstats);
@@ -343,12 +344,26 @@ int ensure_sharedmin(cms_ctx *c) {
     ed.scale = std::ldexp(1.0, K);
     rc = embed_all(false);
     if (rc) return rc;
-    // records of the light entries in rank order (every block holds at most
-    // nwords * 32 of them)
-    c->crs = std::max<int64_t>(c->nwords, 1) * 32;
+    // records of the light entries in rank order, packed block after block:
+    // the prefixes give each block's offset and the count, which sizes the
+    // allocation (the bound nwords * 32 per block is about 10x the count)
     HIPCALL(c, dreserve(c->d_pl, c->cap_pl, nbm));
-    HIPCALL(c, dreserve(c->d_crl, c->cap_crl, (size_t)std::max<int64_t>(nblk, 1) * c->crs));
-    HIPCALL(c, launch_light_records(c->d_Y, c->npad, c->d_lbm, nblk, c->nwords, c->d_pl, c->d_crl, c->crs, c->stream));
+    HIPCALL(c, dreserve(c->d_croff, c->cap_croff, (size_t)std::max<int64_t>(nblk, 1) + 1));
+    HIPCALL(c, launch_light_prefix(c->d_lbm, nblk, c->nwords, c->d_pl, c->d_croff, c->stream));
+    if (!c->nrec_known || c->nrec_metric != metric || c->nrec_binned != binned) {
+        int64_t nrec = 0;
+        if (nblk > 0) {
+            HIPCALL(c, hipMemcpyAsync(&nrec, c->d_croff + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+            HIPCALL(c, hipStreamSynchronize(c->stream));
+        }
+        c->nrec = nrec;
+        c->nrec_known = true;
+        c->nrec_metric = metric;
+        c->nrec_binned = binned;
+    }
+    HIPCALL(c, dreserve(c->d_crl, c->cap_crl, (size_t)std::max<int64_t>(c->nrec, 1)));
+    HIPCALL(c, launch_light_records(c->d_Y, c->npad, c->d_lbm, nblk, c->nwords, c->d_pl, c->d_croff, c->d_crl,
+                                    c->nrec, c->stream));
     c->sm_rows = nrows;
     c->sm_metric = metric;
     c->sm_ready = true;
@@ -371,7 +386,7 @@ SmData sm_data(const cms_ctx *c) {
     d.pos = c->sm_binned ? c->d_pos : nullptr;
     d.lbm = c->d_lbm;
     d.crl = c->d_crl;
-    d.crs = c->crs;
+    d.croff = c->d_croff;
     d.pl = c->d_pl;
     d.zrow = c->sm_rows;  // the zero row after the entries' rows
     return d;
@@ -608,6 +623,7 @@ void cms_destroy(cms_ctx *c) {
     dfree(c->d_lbm);
     dfree(c->d_pl);
     dfree(c->d_crl);
+    dfree(c->d_croff);
     dfree(c->d_out);
     dfree(c->d_prow);
     dfree(c->d_pcol);

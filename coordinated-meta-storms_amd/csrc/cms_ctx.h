@@ -64,9 +64,16 @@ struct cms_ctx {
     size_t cap_colsum = 0;
     uint32_t *d_Y = nullptr, *d_bm = nullptr;
     uint32_t *d_lbm = nullptr, *d_pl = nullptr;  // light-entry bitmaps and their per-word prefix
-    uint4 *d_crl = nullptr;                     // records of the light entries, per block in rank order
-    size_t cap_lbm = 0, cap_pl = 0, cap_crl = 0;
-    int64_t crs = 0;
+    uint4 *d_crl = nullptr;                     // records of the light entries, block after block in rank order
+    int64_t *d_croff = nullptr;                 // each block's first record in d_crl (nblocks + 1)
+    size_t cap_lbm = 0, cap_pl = 0, cap_crl = 0, cap_croff = 0;
+    // light records of the prepared samples, counted once per sample set,
+    // tree, metric and binning (the count sizes d_crl; later preparations of
+    // the same inputs reuse it without reading it back)
+    int64_t nrec = 0;
+    bool nrec_known = false;
+    int nrec_metric = -1;
+    bool nrec_binned = false;
     double *d_mass = nullptr;
     double *d_xroot = nullptr;
     size_t cap_xroot = 0;

@@ -641,7 +641,7 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     }
     // default: split tiles (8-row chunk, one wave per workgroup) when the
     // references carry light records, else the plain (16, 4)
-    const bool ref_light = d.lbm && d.crl && d.pl && d.zrow >= d.nrows;
+    const bool ref_light = d.lbm && d.crl && d.croff && d.pl && d.zrow >= d.nrows;
     s->variant = opts.variant > 0 ? opts.variant - 1 : (ref_light ? 6 : 1);
     if (s->variant >= 6 && !ref_light) {
         free_server(s);
@@ -730,7 +730,7 @@ int cms_server_create(cms_ctx *ref, const cms_server_opts *o, cms_server **out) 
     x.lbm = d.lbm;
     x.pl = d.pl;
     x.crl = d.crl;
-    x.crs = d.crs;
+    x.croff = d.croff;
     x.zrow = d.zrow;  // the references' zero row; QY's is the same row index
 
     s->worker = std::thread(worker_main, s);

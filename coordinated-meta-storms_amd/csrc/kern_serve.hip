@@ -130,7 +130,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             sm_tile_split<CHUNK, CrossRows, true>(x.qbm + qb * nw, x.bm + rb * nw, x.qlbm + qb * nw, x.lbm + rb * nw,
-                                                  x.qcrl + qb * x.qcrs, x.crl + rb * x.crs, nw, x.zrow, rows, H,
+                                                  x.qcrl + qb * x.qcrs, x.crl + x.croff[rb], nw, x.zrow, rows, H,
                                                   s_list[wave], s_words[wave], s_stage[wave], ACC, lane, acc, sc, c,
                                                   ks, x.qpl + qb * nw, x.pl + rb * nw);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
                 }
         } else {
             sm_tile_split<CHUNK, CrossRows, false>(x.qbm + qb * nw, x.bm + rb * nw, x.qlbm + qb * nw, x.lbm + rb * nw,
-                                                   x.qcrl + qb * x.qcrs, x.crl + rb * x.crs, nw, x.zrow, rows, H,
+                                                   x.qcrl + qb * x.qcrs, x.crl + x.croff[rb], nw, x.zrow, rows, H,
                                                    s_list[wave], s_words[wave], s_stage[wave], s_acc[wave], lane, acc,
                                                    sc);
 #pragma unroll
@@ -235,7 +235,7 @@ hipError_t launch_sm_cross(const CrossData &x, int variant, const int32_t *qbloc
     if (ntiles <= 0) return hipSuccess;
     if (rb0 < 0 || (rb0 + nrb) * kSmBlock > x.npad || x.qstride % kSmBlock) return hipErrorInvalidValue;
     const bool split_tiles = variant >= 6;
-    if (split_tiles && (!x.qlbm || !x.qpl || !x.qcrl || !x.lbm || !x.pl || !x.crl || x.zrow < 0))
+    if (split_tiles && (!x.qlbm || !x.qpl || !x.qcrl || !x.lbm || !x.pl || !x.crl || !x.croff || x.zrow < 0))
         return hipErrorInvalidValue;
     // split the rows of each tile until the step has about four units per
     // resident wave (parts of at least 64 bitmap words; split tiles: every

@@ -675,7 +675,7 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
             SplitCounts sc;
             const HeavyRows H{d.Y + abase, d.npad, d.Y + bbase, d.npad};
             sm_tile_split<CHUNK, SquareRows<false>>(d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords,
-                                                    d.lbm + cb * d.nwords, d.crl + ra * d.crs, d.crl + cb * d.crs,
+                                                    d.lbm + cb * d.nwords, d.crl + d.croff[ra], d.crl + d.croff[cb],
                                                     d.nwords, d.zrow, rows, H, s_list[wave], s_words[wave],
                                                     s_stage[wave], s_acc[wave], lane, acc, sc);
             if (lane == 0) split_add_counts(d.counter, sc);
@@ -756,7 +756,7 @@ __global__ __launch_bounds__(64 * PARTS) __attribute__((amdgpu_waves_per_eu(CHUN
         const HeavyRows H{d.Y + abase, d.npad, d.Y + bbase, d.npad};
         sm_tile_split<CHUNK, SquareRows<false>, true>(
             d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords, d.lbm + cb * d.nwords,
-            d.crl + ra * d.crs, d.crl + cb * d.crs, d.nwords, d.zrow, rows, H, s_list[wave], s_words[wave],
+            d.crl + d.croff[ra], d.crl + d.croff[cb], d.nwords, d.zrow, rows, H, s_list[wave], s_words[wave],
             s_stage[wave], s_acc, lane, acc, sc, wave, PARTS, d.pl + ra * d.nwords, d.pl + cb * d.nwords);
         if (lane == 0) split_add_counts(d.counter, sc);
         // every wave's staged sums join the sparse and half sums
@@ -832,15 +832,17 @@ __global__ __launch_bounds__(64) void light_prefix_kernel(const uint32_t *__rest
 // listed in LDS, then each half wave reads one entry's 32 values of Y (one
 // 128-byte line; with a mask, the columns whose bit is clear read as zero)
 // and writes its record {mask, values of the set bits in ascending sample
-// order, zeros after them} at the entry's rank. Y has row stride `stride`;
-// block b = blocks[blockIdx.x] (mask masks[blockIdx.x]), or blockIdx.x
-// without a list.
+// order, zeros after them} at the entry's rank after the block's first
+// record (crl + off[b] with packed offsets, crl + b * crs without). Y has row
+// stride `stride`; block b = blocks[blockIdx.x] (mask masks[blockIdx.x]), or
+// blockIdx.x without a list.
 __global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__restrict__ Y, int64_t stride,
                                                            const int32_t *__restrict__ blocks,
                                                            const uint32_t *__restrict__ masks,
                                                            const uint32_t *__restrict__ lbm,
                                                            const uint32_t *__restrict__ PL, int64_t nwords,
-                                                           uint4 *__restrict__ crl, int64_t crs) {
+                                                           const int64_t *__restrict__ off,
+                                                           uint4 *__restrict__ crl, int64_t crs, int64_t cap) {
     __shared__ uint16_t rows[2048];
     __shared__ uint32_t wrank[64];
     const int64_t b = blocks ? blocks[blockIdx.x] : blockIdx.x, wb = (int64_t)blockIdx.y * 64;
@@ -859,7 +861,8 @@ __global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__res
     __syncthreads();
     const uint32_t rank0 = wrank[0];
     const int half = lane >> 5, col = lane & 31;
-    uint4 *out = crl + b * crs;
+    const int64_t first = off ? off[b] : b * crs;
+    uint4 *out = crl + first;
     const bool mine = (cols >> col) & 1u;
     const uint32_t *yc = Y + b * kSmBlock + col;
     // entries k0 + 2 u + half, u < kRecUnroll: the rows' loads are all
@@ -881,7 +884,7 @@ __global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__res
             const uint32_t mask = half ? (uint32_t)(bal >> 32) : (uint32_t)bal;
             const uint32_t below = mask & ((1u << col) - 1u);
             const int r = __popc(below), n = __popc(mask);
-            if (in) {
+            if (in && first + rank0 + k < cap) {
                 uint32_t *rec = reinterpret_cast<uint32_t *>(out + rank0 + (uint32_t)k);
                 if (v[u] != 0u && r < kSmRecVals) rec[1 + r] = v[u];
                 if (col == 0) {
@@ -893,15 +896,43 @@ __global__ __launch_bounds__(64) void light_records_kernel(const uint32_t *__res
     }
 }
 
-hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t *lbm, int64_t nblk, int64_t nwords,
-                                uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st) {
+// Packed record offsets: off[b] = light entries of blocks 0 .. b - 1 (each
+// block's count from its last prefix word), off[nblk] = all. One wave.
+__global__ __launch_bounds__(64) void light_offsets_kernel(const uint32_t *__restrict__ lbm,
+                                                           const uint32_t *__restrict__ PL, int64_t nblk,
+                                                           int64_t nwords, int64_t *__restrict__ off) {
+    const int lane = threadIdx.x;
+    int64_t base = 0;
+    for (int64_t b0 = 0; b0 < nblk; b0 += 64) {
+        const int64_t b = b0 + lane;
+        const int64_t last = b * nwords + nwords - 1;
+        // a block has at most nwords * 32 light entries; 64 of them sum below 2^31
+        // for nwords < 2^20 (checked by the launcher)
+        const int c = (b < nblk) ? (int)(PL[last] + (uint32_t)__popc(lbm[last])) : 0;
+        const int incl = wave_incl_scan(c);
+        if (b < nblk) off[b] = base + (int64_t)(incl - c);
+        base += (int64_t)__builtin_amdgcn_readlane(incl, 63);
+    }
+    if (lane == 0) off[nblk] = base;
+}
+
+hipError_t launch_light_prefix(const uint32_t *lbm, int64_t nblk, int64_t nwords, uint32_t *PL, int64_t *off,
+                               hipStream_t st) {
     if (nblk <= 0 || nwords <= 0) return hipSuccess;
-    if (crs < nwords * 32 || npad < nblk * kSmBlock) return hipErrorInvalidValue;
+    if (nwords >= ((int64_t)1 << 20)) return hipErrorInvalidValue;
     light_prefix_kernel<<<dim3((unsigned)nblk), 64, 0, st>>>(lbm, nwords, nullptr, PL);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    light_offsets_kernel<<<1, 64, 0, st>>>(lbm, PL, nblk, nwords, off);
+    return hipGetLastError();
+}
+
+hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t *lbm, int64_t nblk, int64_t nwords,
+                                const uint32_t *PL, const int64_t *off, uint4 *crl, int64_t cap, hipStream_t st) {
+    if (nblk <= 0 || nwords <= 0) return hipSuccess;
+    if (!off || npad < nblk * kSmBlock) return hipErrorInvalidValue;
     light_records_kernel<<<dim3((unsigned)nblk, (unsigned)((nwords + 63) / 64)), 64, 0, st>>>(
-        Y, npad, nullptr, nullptr, lbm, PL, nwords, crl, crs);
+        Y, npad, nullptr, nullptr, lbm, PL, nwords, off, crl, 0, cap);
     return hipGetLastError();
 }
 
@@ -914,7 +945,7 @@ hipError_t launch_light_records_listed(const uint32_t *Y, int64_t stride, const 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     light_records_kernel<<<dim3((unsigned)nblocks, (unsigned)((nwords + 63) / 64)), 64, 0, st>>>(
-        Y, stride, blocks, masks, lbm, PL, nwords, crl, crs);
+        Y, stride, blocks, masks, lbm, PL, nwords, nullptr, crl, crs, INT64_MAX);
     return hipGetLastError();
 }
 
@@ -1016,7 +1047,7 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
     }
     // the split tiles read the light records and the zero row; tiles shared
     // by several waves also the light prefixes
-    if (variant >= 6 && (!d.lbm || !d.crl || d.zrow < d.nrows)) return hipErrorInvalidValue;
+    if (variant >= 6 && (!d.lbm || !d.crl || !d.croff || d.zrow < d.nrows)) return hipErrorInvalidValue;
     if (variant >= 12 && !d.pl) return hipErrorInvalidValue;
     switch (variant) {
     case 0: CMS_PERSIST(8, 4, false); break;

@@ -98,13 +98,14 @@ struct SmData {
     const int32_t *perm;          // stored position -> caller's sample id (null = identity)
     const int32_t *pos;           // caller's sample id -> stored position (null = identity)
     // Light entries: bit r of lbm[b * nwords + r / 32] is set when block b
-    // has 1 to kSmRecVals samples with mass at entry r; crl[b * crs + k] =
+    // has 1 to kSmRecVals samples with mass at entry r; crl[croff[b] + k] =
     // {mask, v0, v1, v2} for block b's k-th light entry (bit j of mask:
     // Y[r][32 b + j] > 0, v = those values in ascending sample order, zeros
-    // after the last).
+    // after the last). The blocks' records are packed: croff[b] = light
+    // entries of the blocks before b.
     const uint32_t *lbm;          // null = none (the split tiles need them)
     const uint4 *crl;
-    int64_t crs;                  // record stride per block (>= its light entries)
+    const int64_t *croff;         // nblocks + 1 offsets into crl
     const uint32_t *pl;           // pl[b * nwords + w] = light entries of block b before word w
     // An all-zero row of Y (index >= nrows; -1 = none): the split tiles
     // stage it in the slots of a short last chunk, so every chunk runs the
@@ -147,11 +148,16 @@ constexpr SmVariant kSmVariantTable[kSmVariants] = {
     {8, 4, 0, 1},  {16, 4, 0, 1}, {32, 4, 0, 1}, {16, 2, 0, 1}, {32, 2, 0, 1}, {16, 8, 0, 1},
     {8, 2, 1, 1},  {8, 1, 1, 1},  {8, 4, 1, 1},  {16, 2, 1, 1}, {16, 1, 1, 1}, {16, 4, 1, 1},
     {8, 2, 1, 2},  {8, 4, 1, 4},  {16, 2, 1, 2}, {16, 4, 1, 4}};
-// Light-entry prefix (PL[b][w] = light entries of block b before word w) and
-// the records of every block's light entries in rank order, from Y and the
-// light bitmaps; crs >= the most light entries of any block.
+// Light-entry prefix of every block (PL[b][w] = light entries of block b
+// before word w) and the packed record offsets (off[b] = light entries of
+// blocks 0 .. b - 1; off[nblk] = all of them), from the light bitmaps.
+hipError_t launch_light_prefix(const uint32_t *lbm, int64_t nblk, int64_t nwords, uint32_t *PL, int64_t *off,
+                               hipStream_t st);
+// The records of every block's light entries in rank order at crl + off[b],
+// from Y and the light bitmaps (after launch_light_prefix); crl holds cap
+// records, and records past them are not written.
 hipError_t launch_light_records(const uint32_t *Y, int64_t npad, const uint32_t *lbm, int64_t nblk, int64_t nwords,
-                                uint32_t *PL, uint4 *crl, int64_t crs, hipStream_t st);
+                                const uint32_t *PL, const int64_t *off, uint4 *crl, int64_t cap, hipStream_t st);
 // The same for the listed blocks of a matrix of row stride `stride` (a
 // multiple of 32), the columns whose bit of masks[k] is clear reading as
 // zero (the serving query matrix: inactive slots).
@@ -319,14 +325,15 @@ struct CrossData {
     double inv_scale;                   // 1 / (100 * 2^K)
     unsigned long long *counter;        // optional: counters as SmData.counter (kCnt*)
     // split tiles (cross variants 7, 8): light bitmaps, prefixes and records
-    // of both sides (as SmData: lbm / pl / crl, crs), and a row index that
-    // is all zero in QY and Y
+    // of both sides (the references' as SmData: lbm / pl / crl, croff; the
+    // query blocks' records at qcrl + qb * qcrs), and a row index that is all
+    // zero in QY and Y
     const uint32_t *qlbm, *qpl;
     const uint4 *qcrl;
     int64_t qcrs;
     const uint32_t *lbm, *pl;
     const uint4 *crl;
-    int64_t crs;
+    const int64_t *croff;
     int64_t zrow;
 };
 

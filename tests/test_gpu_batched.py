@@ -270,3 +270,36 @@ def test_default_shares_few_tiles(cms, data):
             assert st.variant == (14 if variant == 0 else 8)
             assert st.entries_sparse > 0 and st.entries_half > 0
     np.testing.assert_array_equal(outs[0].view(np.uint32), outs[8].view(np.uint32))
+
+
+def test_light_records_recounted(cms, data):
+    """The light records are packed block after block in an allocation sized
+    by their count, which a context counts once per sample set, metric and
+    binning and reuses while those stay: a sparse set, then a denser one
+    (more records: the allocation grows), then the sparse one again, the
+    metric switched and the binning switched, all in one context, each give
+    the bits of a fresh context."""
+    print("light records recounted", flush=True)
+    tree, abd = data
+    n = abd.n
+    cp = cms.synth_copy_numbers(3003, tree.leaf_n, 15)
+    dense = cms.counts_to_abundance(cms.synth_counts(2904, n, tree.leaf_n, 3000, 8000, 1.0, 30, 0.3), cp)
+    fresh = {}
+    for name, s in (("sparse", abd), ("dense", dense)):
+        for metric in (0, 1):
+            for bin_ in (0, 1):
+                with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=metric, bin=bin_) as ctx:
+                    ctx.set_tree(tree)
+                    ctx.set_samples_csr(s)
+                    fresh[name, metric, bin_] = ctx.all_pairs()
+    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, metric=0, bin=0) as ctx:
+        ctx.set_tree(tree)
+        for name, s, metric, bin_ in (("sparse", abd, 0, 0), ("dense", dense, 0, 0), ("sparse", abd, 0, 0),
+                                      ("sparse", abd, 1, 0), ("sparse", abd, 1, 1), ("dense", dense, 1, 1),
+                                      ("dense", dense, 0, 1)):
+            ctx.set_opts(cms.KERNEL_SHAREDMIN, metric=metric, bin=bin_)
+            ctx.set_samples_csr(s)
+            for _ in range(2):  # the second preparation reuses the count
+                ctx.prepare(force=True)
+                got = ctx.all_pairs()
+                np.testing.assert_array_equal(got.view(np.uint32), fresh[name, metric, bin_].view(np.uint32))
