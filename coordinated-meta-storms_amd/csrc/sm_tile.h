@@ -367,8 +367,11 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
     uint32_t *const trash = S + lane;
     uint32_t cs = 0;                // this lane's column summed over the staged rows
     uint32_t baseA = 0, baseB = 0;  // light entries of each block before the window (ranks)
-    uint32_t pairs_lane = 0;        // this lane's sparse pairs with mass in the window (<= 32 x 9)
-    uint32_t hp_lane = 0;           // this lane's half pairs evaluated in the window (<= 32 x 96)
+    // this lane's sparse pairs with mass and half pairs evaluated, summed
+    // over the tile's windows and reduced over the wave once at the end (at
+    // most 24 entries x 96 pairs per window and lane: far below 2^32)
+    uint32_t pairs_lane = 0;
+    uint32_t hp_lane = 0;
     // the four bitmap words of the lane in the next window, loaded one window
     // ahead so their latency hides behind the current window's work
     uint32_t na = 0u, nb = 0u, nla = 0u, nlb = 0u;
@@ -542,17 +545,6 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
         };
         half(totd, tha, std::false_type{});
         half(totd + tha, thb, std::true_type{});
-        if (tots | toth) {
-            uint32_t p = pairs_lane, h = hp_lane;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                p += (uint32_t)__shfl_xor((int)p, off, 64);
-                h += (uint32_t)__shfl_xor((int)h, off, 64);
-            }
-            cnt.pairs += (uint32_t)__builtin_amdgcn_readfirstlane((int)p);
-            cnt.half_pairs += (uint32_t)__builtin_amdgcn_readfirstlane((int)h);
-            pairs_lane = hp_lane = 0u;
-        }
         if (totd == 0) continue;
         for (int j = 0; j < totd; j += CHUNK) {
             // a short last chunk holds zero rows past the list end (|0 - 0|
@@ -588,6 +580,16 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }
+    }
+    {
+        uint32_t p = pairs_lane, h = hp_lane;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            p += (uint32_t)__shfl_xor((int)p, off, 64);
+            h += (uint32_t)__shfl_xor((int)h, off, 64);
+        }
+        cnt.pairs += (uint32_t)__builtin_amdgcn_readfirstlane((int)p);
+        cnt.half_pairs += (uint32_t)__builtin_amdgcn_readfirstlane((int)h);
     }
     uint32_t sa[4], sb[4];
 #pragma unroll
