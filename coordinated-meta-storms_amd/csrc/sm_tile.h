@@ -25,11 +25,15 @@ constexpr int kRawBufferFlags = 0x00020000;
 //
 // One matrix (all-pairs tiles): lane column `col` (row block sample for lanes
 // 0..31, column block sample for lanes 32..63) of Y with row stride `stride`.
+// R32: every row offset the tile forms (window-relative rows, the zero row
+// included) times the stride fits in 32 bits, so the row offset is a 32-bit
+// product (the launcher checks; a 64-bit product costs a wave about ten more
+// scalar instructions per staged row).
 // CHECK (tiles at any sample offset, whose rows are those of the union of
 // two blocks' bitmaps): a lane whose column lies outside the matrix (ok
 // false) loads column 0 and stages zero, and a lane stages zero for rows its
 // own block's bitmap (own) lacks, where the embedding stored nothing.
-template <bool CHECK>
+template <bool CHECK, bool R32 = false>
 struct SquareRows {
     const uint32_t *Y;
     int64_t stride;
@@ -47,8 +51,9 @@ struct SquareRows {
         // offset is the lane's column: the address costs scalar
         // instructions only (a global load of the same row would add a 64-bit
         // vector address per load)
-        const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<uint32_t *>(win + (uint64_t)r * (uint64_t)stride), (short)0, (int)0xffffffff, kRawBufferFlags);
+        const uint32_t *rp = R32 ? win + (uint32_t)(r * (uint32_t)stride) : win + (uint64_t)r * (uint64_t)stride;
+        const __amdgpu_buffer_rsrc_t row =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(rp), (short)0, (int)0xffffffff, kRawBufferFlags);
         const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(row, (int)col4, 0, 0);
         if constexpr (CHECK) {
             const int64_t e = rb + r;
