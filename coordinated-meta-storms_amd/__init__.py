@@ -63,7 +63,7 @@ class Opts(C.Structure):
     _fields_ = [("kernel", C.c_int), ("storage", C.c_int), ("tile_a", C.c_int), ("tile_b", C.c_int),
                 ("chunk", C.c_int), ("use_graph", C.c_int), ("metric", C.c_int), ("bin", C.c_int),
                 ("autotune", C.c_int), ("variant", C.c_int), ("embed_slice", C.c_int),
-                ("embed_input", C.c_int), ("reserved", C.c_int * 2)]
+                ("embed_input", C.c_int), ("wide_rows", C.c_int), ("reserved", C.c_int)]
 
 
 class Stats(C.Structure):

@@ -377,6 +377,7 @@ SmData sm_data(const cms_ctx *c) {
     d.nrows = c->sm_rows;
     d.npad = c->npad;
     d.nwords = c->nwords;
+    d.wide_rows = c->opts.wide_rows;
     d.n = c->n;
     d.inv_scale = 1.0 / (100.0 * std::ldexp(1.0, c->scale_bits));
     d.counter = c->d_counter;
@@ -555,6 +556,7 @@ const char *opts_error(const cms_opts *o) {
     if (o->storage < CMS_STORE_F32 || o->storage > CMS_STORE_F16) return "unknown storage";
     if (o->embed_slice < 0) return "embed_slice must be >= 0";
     if (o->embed_input != 0 && o->embed_input != 1) return "embed_input must be 0 or 1";
+    if (o->wide_rows != 0 && o->wide_rows != 1) return "wide_rows must be 0 or 1";
     return nullptr;
 }
 }  // namespace

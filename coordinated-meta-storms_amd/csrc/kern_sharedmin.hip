@@ -642,7 +642,7 @@ __device__ __forceinline__ int64_t sm_next_tile(unsigned long long *__restrict__
 //          (a band buffer of a streamed run).
 //   SPLIT: entries light in both blocks (d.lbm) take the sparse path of
 //          sm_tile_split (records d.crl), the rest are staged.
-template <int CHUNK, int WAVES, int OUT, bool SPLIT, bool R32 = false>
+template <int CHUNK, int WAVES, int OUT, bool SPLIT, bool SOFF = false>
 __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, int64_t t_begin, int64_t t_count,
                                                    TileRows g, unsigned long long *__restrict__ work,
                                                    float *__restrict__ out, int64_t out_base) {
@@ -667,14 +667,14 @@ __device__ __forceinline__ void sm_persistent_body(const SmData &d, int64_t nb, 
         const int64_t abase = ra * kSmBlock, bbase = cb * kSmBlock;
         uint32_t acc[4][4];
         if constexpr (SPLIT) {
-            SquareRows<false, R32> rows;
+            SquareRows<false, SOFF> rows;
             rows.Y = d.Y;
             rows.stride = d.npad;
             rows.ok = true;
             rows.col4 = (uint32_t)((lane < 32) ? abase + lane : bbase + (lane - 32)) * 4u;
             SplitCounts sc;
             const HeavyRows H{d.Y + abase, d.npad, d.Y + bbase, d.npad};
-            sm_tile_split<CHUNK, SquareRows<false, R32>>(d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords,
+            sm_tile_split<CHUNK, SquareRows<false, SOFF>>(d.bm + ra * d.nwords, d.bm + cb * d.nwords, d.lbm + ra * d.nwords,
                                                     d.lbm + cb * d.nwords, d.crl + d.croff[ra], d.crl + d.croff[cb],
                                                     d.nwords, d.zrow, rows, H, s_list[wave], s_words[wave],
                                                     s_stage[wave], s_acc[wave], lane, acc, sc);
@@ -707,11 +707,11 @@ __global__ __launch_bounds__(64 * WAVES) void sm_persistent_kernel(SmData d, int
 
 // The split tiles held to 4 waves per SIMD (at most 128 VGPRs): their
 // sparse entries wait on record loads, and occupancy hides that latency.
-template <int CHUNK, int WAVES, int OUT, bool R32>
+template <int CHUNK, int WAVES, int OUT, bool SOFF>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(CHUNK <= 8 && WAVES <= 2 ? 4 : 1))) void sm_persistent_split_kernel(
     SmData d, int64_t nb, int64_t t_begin, int64_t t_count, TileRows g, unsigned long long *__restrict__ work,
     float *__restrict__ out, int64_t out_base) {
-    sm_persistent_body<CHUNK, WAVES, OUT, true, R32>(d, nb, t_begin, t_count, g, work, out, out_base);
+    sm_persistent_body<CHUNK, WAVES, OUT, true, SOFF>(d, nb, t_begin, t_count, g, work, out, out_base);
 }
 
 // Split tiles with PARTS waves per tile (1024 / PARTS pairs per wave): a
@@ -1029,10 +1029,9 @@ static hipError_t launch_persistent_t(const SmData &d, int variant, int64_t nb, 
                                       hipStream_t st) {
     if (t_count <= 0) return hipSuccess;
     const TileRows g = host_tile_rows(t_begin, t_count, nb);
-    // the split tiles' row offsets: window-relative rows (list rows below
-    // kSplitList, or the zero row, at most its index) times the stride
-    const uint64_t rmax = (uint64_t)std::max<int64_t>(d.zrow, kSplitList);
-    const bool r32 = d.zrow >= 0 && (rmax + 1) * (uint64_t)d.npad < (1ull << 32);
+    // the split tiles' rows as 32-bit byte offsets from the window's first
+    // row: list rows (below kSplitList) times the stride times 4
+    const bool r32 = !d.wide_rows && (uint64_t)kSplitList * (uint64_t)d.npad * 4ull < (1ull << 32);
 #define CMS_PERSIST(CH, WV, SP)                                                                                  \
     {                                                                                                            \
         const int64_t wg = std::max<int64_t>(1, std::min<int64_t>((t_count + WV - 1) / WV, grid_waves / WV));   \

@@ -90,6 +90,7 @@ struct SmData {
     int64_t nrows;
     int64_t npad;
     int64_t nwords;
+    int wide_rows;          // split tiles: 64-bit row addresses whatever the size (cms_opts.wide_rows)
     int n;                  // samples
     double inv_scale;       // 1 / (100 * 2^K)
     unsigned long long *counter;  // optional: entries visited (summed over tiles)

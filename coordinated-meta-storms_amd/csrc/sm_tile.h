@@ -25,16 +25,20 @@ constexpr int kRawBufferFlags = 0x00020000;
 //
 // One matrix (all-pairs tiles): lane column `col` (row block sample for lanes
 // 0..31, column block sample for lanes 32..63) of Y with row stride `stride`.
-// R32: every row offset the tile forms (window-relative rows, the zero row
-// included) times the stride fits in 32 bits, so the row offset is a 32-bit
-// product (the launcher checks; a 64-bit product costs a wave about ten more
-// scalar instructions per staged row).
+// SOFF: the rows are read through one buffer resource per window (its first
+// row) with the row's byte offset in the instruction's 32-bit scalar offset,
+// which needs every list row (below kSplitList) times the stride times 4 to
+// fit in 32 bits (the launcher checks); the tiles then stage no zero row
+// (kMaskTail: a short last chunk loads the window's first row in its empty
+// slots and clears them). Otherwise each row gets its own resource from a
+// 64-bit row address, about ten more scalar instructions per staged row.
 // CHECK (tiles at any sample offset, whose rows are those of the union of
 // two blocks' bitmaps): a lane whose column lies outside the matrix (ok
 // false) loads column 0 and stages zero, and a lane stages zero for rows its
 // own block's bitmap (own) lacks, where the embedding stored nothing.
-template <bool CHECK, bool R32 = false>
+template <bool CHECK, bool SOFF = false>
 struct SquareRows {
+    static constexpr bool kMaskTail = SOFF;
     const uint32_t *Y;
     int64_t stride;
     uint32_t col4;  // column * 4 (bytes)
@@ -51,10 +55,17 @@ struct SquareRows {
         // offset is the lane's column: the address costs scalar
         // instructions only (a global load of the same row would add a 64-bit
         // vector address per load)
-        const uint32_t *rp = R32 ? win + (uint32_t)(r * (uint32_t)stride) : win + (uint64_t)r * (uint64_t)stride;
-        const __amdgpu_buffer_rsrc_t row =
-            __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(rp), (short)0, (int)0xffffffff, kRawBufferFlags);
-        const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(row, (int)col4, 0, 0);
+        uint32_t v;
+        if constexpr (SOFF) {
+            const __amdgpu_buffer_rsrc_t w = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<uint32_t *>(win), (short)0, (int)0xffffffff, kRawBufferFlags);
+            v = __builtin_amdgcn_raw_buffer_load_b32(w, (int)col4, (int)(r * ((uint32_t)stride * 4u)), 0);
+        } else {
+            const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<uint32_t *>(win + (uint64_t)r * (uint64_t)stride), (short)0, (int)0xffffffff,
+                kRawBufferFlags);
+            v = __builtin_amdgcn_raw_buffer_load_b32(row, (int)col4, 0, 0);
+        }
         if constexpr (CHECK) {
             const int64_t e = rb + r;
             return (ok && ((own[e >> 5] >> (e & 31)) & 1u)) ? v : 0u;
@@ -66,6 +77,7 @@ struct SquareRows {
 // Two matrices (serving): lanes 0..31 read QY (query slots of one block,
 // row stride qs), lanes 32..63 read Y (references of one block, stride ys).
 struct CrossRows {
+    static constexpr bool kMaskTail = false;
     const uint32_t *QY;
     int64_t qs;
     const uint32_t *Y;
@@ -446,8 +458,9 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
             for (int k = 0; k < CHUNK; k += 2) {
                 const uint32_t pr =
                     (uint32_t)__builtin_amdgcn_readfirstlane((int)*reinterpret_cast<const uint32_t *>(L + j + k));
-                const uint32_t r0 = (j + k < totd) ? (pr & 0xffffu) : zr;
-                const uint32_t r1 = (j + k + 1 < totd) ? (pr >> 16) : zr;
+                const uint32_t zslot = Rows::kMaskTail ? 0u : zr;
+                const uint32_t r0 = (j + k < totd) ? (pr & 0xffffu) : zslot;
+                const uint32_t r1 = (j + k + 1 < totd) ? (pr >> 16) : zslot;
                 v[k] = win.load(r0);
                 v[k + 1] = win.load(r1);
             }
@@ -555,6 +568,13 @@ __device__ __forceinline__ void sm_tile_split(const uint32_t *A0, const uint32_t
             // a short last chunk holds zero rows past the list end (|0 - 0|
             // and the column sums gain nothing), so every chunk runs the same
             // unrolled accumulation
+            if constexpr (Rows::kMaskTail) {
+                if (j + CHUNK > totd) {
+#pragma unroll
+                    for (int k = 0; k < CHUNK; k++)
+                        if (j + k >= totd) v[k] = 0u;
+                }
+            }
 #pragma unroll
             for (int k = 0; k < CHUNK; k++) cs += v[k];
             __builtin_amdgcn_wave_barrier();

@@ -86,7 +86,10 @@ typedef struct cms_opts {
     int embed_slice;  /* SHAREDMIN: samples per embedding slice, multiple of 64 (0 = by a 16 GiB layout) */
     int embed_input;  /* SHAREDMIN embedding input: 0 = the samples' leaf lists where the leaf stream
                          has at most 253,952 positions, else the F32 walk layout; 1 = the walk layout */
-    int reserved[2];
+    int wide_rows;    /* SHAREDMIN split tiles: 0 = rows read as 32-bit offsets from each bitmap window's
+                         first row below 699,051 padded samples, 64-bit row addresses above; 1 = always
+                         64-bit row addresses (same results) */
+    int reserved;
 } cms_opts;
 
 typedef struct cms_ctx cms_ctx;

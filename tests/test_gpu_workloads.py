@@ -226,35 +226,36 @@ def test_workload1_full_triangle(cms, orc, gpu_ok):
     np.testing.assert_array_equal(got.view(np.uint32), other.view(np.uint32))
 
 
-def test_split_tiles_64bit_row_offsets(cms, orc, gpu_ok):
-    """The split tiles form a staged row's offset as a 32-bit product when
-    every window-relative row (the zero row's index at most) times the
-    sample stride fits in 32 bits, and as a 64-bit product otherwise. On the
-    99,322-leaf tree (198,642 rows) that bound is passed at 21,622 padded
-    samples: workload 1's generator at 21,700 samples takes the 64-bit form.
-    Its whole triangle must equal the plain tiles' bits (their loader has
-    no such switch), and 2,000 random pairs the FP64 oracle."""
-    tree, cp, counts, abd = cms.workload(1, samples=21700)
+def test_split_tiles_row_forms(cms, orc, gpu_ok):
+    """The split tiles read a staged row through one buffer resource per
+    bitmap window with the row's 32-bit byte offset (below 699,051 padded
+    samples, where every list row times the stride times 4 fits in 32
+    bits; a short last chunk's empty slots load the window's first row and
+    are cleared), or from a 64-bit row address (larger sets, or
+    cms_opts.wide_rows = 1, which takes it at any size). On workload 1's
+    generator at 3,000 samples both forms must give the plain tiles' bits
+    over the whole triangle, and the FP64 oracle on 2,000 random pairs."""
+    tree, cp, counts, abd = cms.workload(1, samples=3000)
     del counts
     n = abd.n
-    _flush(f"64-bit row offsets: {n} samples, nnz {abd.rowptr[-1]}")
-    with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1) as ctx:
-        ctx.set_tree(tree)
-        ctx.set_samples_csr(abd)
-        got = ctx.all_pairs()
-        assert ctx.stats().variant >= 7  # a split-tile variant ran
+    _flush(f"row forms: {n} samples, nnz {abd.rowptr[-1]}")
     with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, variant=1) as ctx:
         ctx.set_tree(tree)
         ctx.set_samples_csr(abd)
-        other = ctx.all_pairs()
-    np.testing.assert_array_equal(got.view(np.uint32), other.view(np.uint32))
-    del other
+        plain = ctx.all_pairs()
     rng = np.random.default_rng(12)
     ii = rng.integers(0, n, 2000)
     jj = rng.integers(0, n, 2000)
     keep = ii != jj
     ii, jj = np.minimum(ii, jj)[keep], np.maximum(ii, jj)[keep]
     exp = _oracle_pairs(cms, orc, tree, abd, ii, jj)
-    mx, rms = _errs(got[cms.packed_index(ii, jj, n)], exp)
-    _flush(f"vs FP64 oracle on {len(ii)} pairs: max {mx:.3g} rms {rms:.3g}")
-    assert mx <= TOL and rms <= RMS
+    for wide in (0, 1):
+        with cms.Context(0, kernel=cms.KERNEL_SHAREDMIN, bin=1, wide_rows=wide) as ctx:
+            ctx.set_tree(tree)
+            ctx.set_samples_csr(abd)
+            got = ctx.all_pairs()
+            assert ctx.stats().variant >= 7  # a split-tile variant ran
+        np.testing.assert_array_equal(got.view(np.uint32), plain.view(np.uint32))
+        mx, rms = _errs(got[cms.packed_index(ii, jj, n)], exp)
+        _flush(f"wide_rows={wide}: vs FP64 oracle on {len(ii)} pairs: max {mx:.3g} rms {rms:.3g}")
+        assert mx <= TOL and rms <= RMS

@@ -318,7 +318,8 @@ def test_cms_gen_writes_sparse_table(cms, tmp_path):
 
 
 @pytest.mark.parametrize("field,value", [("embed_input", 2), ("embed_input", -1), ("embed_slice", -64),
-                                         ("variant", 17), ("chunk", 12), ("tile_a", 64)])
+                                         ("variant", 17), ("chunk", 12), ("tile_a", 64), ("wide_rows", 2),
+                                         ("wide_rows", -1)])
 def test_context_options_refused_before_any_device_call(cms, field, value):
     # cms_create checks the options first: an out-of-range field is
     # CMS_E_INVAL on any machine, GPU or not
